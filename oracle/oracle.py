@@ -1,0 +1,201 @@
+"""ctypes access to the oracle and to the compiled reference (oracle/_ref).
+
+TEST INFRASTRUCTURE ONLY. Tests, the smoke check and the benchmark's
+CPU-baseline leg use this module as the checker. The product never imports it.
+
+``Oracle("oracle")`` loads ``oracle/liboracle_boxqp.so`` (the CPU restatement in
+boxqp_oracle.cpp); ``Oracle("reference")`` loads ``oracle/_ref/libref_boxqp.so``
+(the reference's own sources plus ref_shim.cpp). Both expose the same calls.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {
+    "oracle": (os.path.join(HERE, "liboracle_boxqp.so"), "orc_"),
+    "reference": (os.path.join(HERE, "_ref", "libref_boxqp.so"), "ref_"),
+}
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("nterms", C.c_int64),
+        ("hrow", C.POINTER(C.c_int64)), ("hcol", C.POINTER(C.c_int64)), ("hval", C.POINTER(C.c_double)),
+        ("cost", C.POINTER(C.c_double)), ("lower", C.POINTER(C.c_double)), ("upper", C.POINTER(C.c_double)),
+        ("offset", C.c_double), ("maximize", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("status", C.c_char * 32), ("algorithm", C.c_char * 32),
+        ("objective", C.c_double), ("primal_residual", C.c_double),
+        ("stationarity_residual", C.c_double), ("complementarity_residual", C.c_double),
+        ("dual_bound", C.c_double), ("relative_gap", C.c_double),
+        ("iterations", C.c_uint64), ("optimality_verified", C.c_int32), ("positive_diagonal", C.c_int32),
+    ]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["status"] = self.status.decode()
+        d["algorithm"] = self.algorithm.decode()
+        d["optimality_verified"] = bool(self.optimality_verified)
+        return d
+
+
+class Options(C.Structure):
+    _fields_ = [("time_limit", C.c_double), ("tolerance", C.c_double),
+                ("iteration_limit", C.c_uint64), ("threads", C.c_int32)]
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class BoxQP:
+    """A box QP in the reference's triplet form (qp.hpp:6-10): one entry per
+    lower-triangle position with the full value; numpy arrays are kept alive here."""
+
+    def __init__(self, n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False):
+        self.n = int(n)
+        self.hrow = np.ascontiguousarray(hrow, dtype=np.int64)
+        self.hcol = np.ascontiguousarray(hcol, dtype=np.int64)
+        self.hval = np.ascontiguousarray(hval, dtype=np.float64)
+        self.cost = np.ascontiguousarray(cost, dtype=np.float64)
+        self.lower = np.ascontiguousarray(lower, dtype=np.float64)
+        self.upper = np.ascontiguousarray(upper, dtype=np.float64)
+        self.offset = float(offset)
+        self.maximize = bool(maximize)
+
+    def struct(self):
+        return Problem(self.n, len(self.hval), _ptr(self.hrow, C.c_int64), _ptr(self.hcol, C.c_int64),
+                       _ptr(self.hval, C.c_double), _ptr(self.cost, C.c_double), _ptr(self.lower, C.c_double),
+                       _ptr(self.upper, C.c_double), self.offset, int(self.maximize))
+
+
+class Oracle:
+    def __init__(self, kind="oracle"):
+        path, prefix = LIBS[kind]
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (make -C oracle {'ref' if kind == 'reference' else 'all'})")
+        self.kind = kind
+        self.lib = C.CDLL(path)
+        self.p = prefix
+        f = self._fn
+        f("last_error").restype = C.c_char_p
+        f("check_convex").argtypes = [C.POINTER(Problem)]
+        f("check").argtypes = [C.POINTER(Problem), C.POINTER(C.c_double), C.c_double, C.POINTER(Result)]
+        f("build_csr").argtypes = [C.POINTER(Problem), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        f("solve_box").argtypes = [C.POINTER(Problem), C.POINTER(Options), C.POINTER(Result), C.POINTER(C.c_double)]
+        f("validate").argtypes = [C.POINTER(Problem)]
+        if kind == "oracle":
+            self.lib.orc_lipschitz.restype = C.c_double
+            self.lib.orc_lipschitz.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+            self.lib.orc_spmv.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+            self.lib.orc_betas.argtypes = [C.c_int64, C.POINTER(C.c_double)]
+            self.lib.orc_fista_steps.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double), C.c_int64,
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+            self.lib.orc_gen_tridiagonal.restype = C.c_double
+            self.lib.orc_gen_tridiagonal.argtypes = [C.c_int64] + [C.POINTER(C.c_int64)] * 2 + \
+                [C.POINTER(C.c_double)] * 5
+
+    def _fn(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def _raise(self, rc):
+        if rc != 0:
+            raise OracleError(rc, self._fn("last_error")().decode())
+
+    def validate(self, q):
+        s = q.struct()
+        self._raise(self._fn("validate")(C.byref(s)))
+
+    def check_convex(self, q):
+        s = q.struct()
+        v = self._fn("check_convex")(C.byref(s))
+        if v < 0:
+            self._raise(-v)
+        return bool(v)
+
+    def check(self, q, x, tol=1e-7):
+        s = q.struct()
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = Result()
+        self._raise(self._fn("check")(C.byref(s), _ptr(x, C.c_double), tol, C.byref(r)))
+        return r.as_dict()
+
+    def build_csr(self, q):
+        s = q.struct()
+        start = np.zeros(q.n + 1, dtype=np.int32)
+        cap = max(1, 2 * len(q.hval))
+        index = np.zeros(cap, dtype=np.int32)
+        value = np.zeros(cap, dtype=np.float64)
+        nnz = C.c_int64(0)
+        self._raise(self._fn("build_csr")(C.byref(s), _ptr(start, C.c_int32), _ptr(index, C.c_int32),
+                                          _ptr(value, C.c_double), C.byref(nnz)))
+        return start, index[:nnz.value].copy(), value[:nnz.value].copy()
+
+    def solve_box(self, q, time_limit=30.0, tolerance=1e-7, iteration_limit=200000, threads=1):
+        s = q.struct()
+        o = Options(time_limit, tolerance, iteration_limit, threads)
+        r = Result()
+        x = np.zeros(q.n, dtype=np.float64)
+        self._raise(self._fn("solve_box")(C.byref(s), C.byref(o), C.byref(r), _ptr(x, C.c_double)))
+        d = r.as_dict()
+        d["primal"] = x
+        return d
+
+    # -- oracle-only helpers ------------------------------------------------
+    def lipschitz(self, start, value):
+        return self.lib.orc_lipschitz(len(start) - 1, _ptr(start, C.c_int32), _ptr(value, C.c_double))
+
+    def spmv(self, start, index, value, x, threads=1):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(len(start) - 1, dtype=np.float64)
+        self.lib.orc_spmv(len(start) - 1, _ptr(start, C.c_int32), _ptr(index, C.c_int32), _ptr(value, C.c_double),
+                          _ptr(x, C.c_double), _ptr(y, C.c_double), threads)
+        return y
+
+    def betas(self, count):
+        b = np.zeros(count, dtype=np.float64)
+        self.lib.orc_betas(count, _ptr(b, C.c_double))
+        return b
+
+    def fista_steps(self, start, index, value, c, lower, upper, lipschitz, betas, x, w, threads=1):
+        """Applies len(betas) CPU steps to copies of (x, w); returns the new pair."""
+        x = np.array(x, dtype=np.float64)
+        w = np.array(w, dtype=np.float64)
+        betas = np.ascontiguousarray(betas, dtype=np.float64)
+        c, lower, upper = (np.ascontiguousarray(a, dtype=np.float64) for a in (c, lower, upper))
+        self.lib.orc_fista_steps(len(start) - 1, _ptr(start, C.c_int32), _ptr(index, C.c_int32),
+                                 _ptr(value, C.c_double), _ptr(c, C.c_double), _ptr(lower, C.c_double),
+                                 _ptr(upper, C.c_double), lipschitz, _ptr(betas, C.c_double), len(betas),
+                                 _ptr(x, C.c_double), _ptr(w, C.c_double), threads)
+        return x, w
+
+    def gen_tridiagonal(self, n):
+        m = 2 * n - 1
+        hrow = np.zeros(m, dtype=np.int64)
+        hcol = np.zeros(m, dtype=np.int64)
+        hval = np.zeros(m)
+        cost, lower, upper, opt = (np.zeros(n) for _ in range(4))
+        f = self.lib.orc_gen_tridiagonal(n, _ptr(hrow, C.c_int64), _ptr(hcol, C.c_int64), _ptr(hval, C.c_double),
+                                         _ptr(cost, C.c_double), _ptr(lower, C.c_double), _ptr(upper, C.c_double),
+                                         _ptr(opt, C.c_double))
+        return BoxQP(n, hrow, hcol, hval, cost, lower, upper), opt, f
+
+
+def available(kind):
+    return os.path.exists(LIBS[kind][0])

@@ -1,0 +1,64 @@
+/* Flat C structures shared by the oracle restatement (boxqp_oracle.cpp) and the
+ * shim that drives the compiled reference sources (ref_shim.cpp).
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product links this header.
+ *
+ * A box QP here is the reference's QuadraticProblem restricted to no rows
+ * (solver/include/qp.hpp:6-10, solver/include/mps.hpp:8-21):
+ *   min  sign * (offset + c'x + 1/2 x'Hx),  lower <= x <= upper,
+ * with H given as triplets, one entry per lower-triangle position holding the
+ * full value; off-diagonal entries are mirrored when H is assembled.
+ */
+#ifndef HGQP_ORACLE_ABI_H
+#define HGQP_ORACLE_ABI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_problem {
+    int64_t n;               /* variables */
+    int64_t nterms;          /* Hessian triplets */
+    const int64_t* hrow;     /* [nterms] */
+    const int64_t* hcol;     /* [nterms] */
+    const double* hval;      /* [nterms] */
+    const double* cost;      /* [n] */
+    const double* lower;     /* [n] */
+    const double* upper;     /* [n] */
+    double offset;
+    int32_t maximize;        /* 0 = minimize, 1 = maximize */
+} orc_problem;
+
+typedef struct orc_result {
+    char status[32];
+    char algorithm[32];
+    double objective;
+    double primal_residual;
+    double stationarity_residual;
+    double complementarity_residual;
+    double dual_bound;
+    double relative_gap;
+    uint64_t iterations;
+    int32_t optimality_verified;
+    int32_t positive_diagonal;   /* 1 when the separable dual bound was used */
+} orc_result;
+
+typedef struct orc_options {
+    double time_limit;        /* seconds, QpOptions::time_limit (qp.hpp:25) */
+    double tolerance;         /* QpOptions::tolerance */
+    uint64_t iteration_limit; /* QpOptions::iteration_limit */
+    int32_t threads;          /* QpOptions::threads */
+} orc_options;
+
+/* Return codes: 0 ok, 1 invalid argument (reference std::invalid_argument),
+ * 2 runtime error, 3 path not restated (zero-Hessian LP fallback). */
+#define ORC_OK 0
+#define ORC_INVALID_ARGUMENT 1
+#define ORC_RUNTIME_ERROR 2
+#define ORC_NOT_RESTATED 3
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+/* Thin extern "C" shim over the reference solver's own sources, compiled from
+ * where they lie in the reference checkout (see oracle/Makefile) into
+ * oracle/_ref/libref_boxqp.so.
+ *
+ * TEST INFRASTRUCTURE ONLY: it lets tests compare the oracle restatement with
+ * the reference itself, and lets the CPU-baseline leg time the reference's
+ * own CPU path. It exports the same entry points as boxqp_oracle.cpp with a
+ * ref_ prefix and the same flat structures (oracle_abi.h).
+ *
+ * Reference interfaces used: sovereign::solve_accelerated_box
+ * (include/accelerated_qp.hpp:17), check_qp_solution and check_convex_hessian
+ * (include/qp.hpp:33-34), validate_linear_problem (include/mps.hpp:31),
+ * CsrMatrix (include/sparse_matrix.hpp:5-34).
+ */
+#include "oracle_abi.h"
+
+#include "accelerated_qp.hpp"
+#include "qp.hpp"
+#include "sparse_matrix.hpp"
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace {
+thread_local std::string g_error;
+
+sovereign::QuadraticProblem to_reference(const orc_problem* p) {
+    if (!p || p->n < 0 || p->nterms < 0) throw std::invalid_argument("Invalid problem handle");
+    sovereign::QuadraticProblem q;
+    auto& lp = q.linear;
+    std::size_t n = static_cast<std::size_t>(p->n);
+    lp.name = "box_qp";
+    lp.names.resize(n);
+    for (std::size_t j = 0; j < n; ++j) lp.names[j] = "x" + std::to_string(j);
+    lp.cost.assign(p->cost, p->cost + n);
+    lp.lower.assign(p->lower, p->lower + n);
+    lp.upper.assign(p->upper, p->upper + n);
+    lp.integer.assign(n, false);
+    lp.offset = p->offset;
+    lp.maximize = p->maximize != 0;
+    q.hessian.reserve(static_cast<std::size_t>(p->nterms));
+    for (int64_t k = 0; k < p->nterms; ++k) {
+        if (p->hrow[k] < 0 || p->hcol[k] < 0) throw std::invalid_argument("Invalid Hessian");
+        q.hessian.push_back({static_cast<std::size_t>(p->hrow[k]), static_cast<std::size_t>(p->hcol[k]), p->hval[k]});
+    }
+    return q;
+}
+
+void export_result(const sovereign::QpResult& r, orc_result* out) {
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->status, r.status.c_str(), sizeof(out->status) - 1);
+    std::strncpy(out->algorithm, r.algorithm.c_str(), sizeof(out->algorithm) - 1);
+    out->objective = r.objective;
+    out->primal_residual = r.primal_residual;
+    out->stationarity_residual = r.stationarity_residual;
+    out->complementarity_residual = r.complementarity_residual;
+    out->dual_bound = r.dual_bound;
+    out->relative_gap = r.relative_gap;
+    out->iterations = r.iterations;
+    out->optimality_verified = r.optimality_verified ? 1 : 0;
+    out->positive_diagonal = -1; /* not exposed by the reference */
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ORC_OK;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return ORC_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return ORC_RUNTIME_ERROR;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error(void) { return g_error.c_str(); }
+
+int ref_validate(const orc_problem* p) {
+    return guarded([&] { sovereign::validate_linear_problem(to_reference(p).linear); });
+}
+
+int ref_check_convex(const orc_problem* p) {
+    int verdict = 0;
+    int rc = guarded([&] { verdict = sovereign::check_convex_hessian(to_reference(p)) ? 1 : 0; });
+    return rc == ORC_OK ? verdict : -rc;
+}
+
+int ref_check(const orc_problem* p, const double* x, double tol, orc_result* out) {
+    return guarded([&] {
+        auto q = to_reference(p);
+        sovereign::QpResult r;
+        r.primal.assign(x, x + q.linear.cost.size());
+        sovereign::check_qp_solution(q, r, tol);
+        export_result(r, out);
+    });
+}
+
+int ref_build_csr(const orc_problem* p, int32_t* start, int32_t* index, double* value, int64_t* nnz) {
+    return guarded([&] {
+        auto q = to_reference(p);
+        double sign = q.linear.maximize ? -1 : 1;
+        std::vector<sovereign::SparseVector> rows(q.linear.cost.size());
+        for (auto h : q.hessian) {
+            rows[h.row].push_back({h.col, sign * h.value});
+            if (h.row != h.col) rows[h.col].push_back({h.row, sign * h.value});
+        }
+        sovereign::CsrMatrix m(rows.size(), rows);
+        for (std::size_t i = 0; i < m.start.size(); ++i) start[i] = m.start[i];
+        for (std::size_t k = 0; k < m.index.size(); ++k) { index[k] = m.index[k]; value[k] = m.value[k]; }
+        *nnz = static_cast<int64_t>(m.value.size());
+    });
+}
+
+int ref_solve_box(const orc_problem* p, const orc_options* opt, orc_result* out, double* primal_out) {
+    return guarded([&] {
+        auto q = to_reference(p);
+        sovereign::QpOptions o;
+        o.time_limit = opt->time_limit;
+        o.tolerance = opt->tolerance;
+        o.iteration_limit = static_cast<std::size_t>(opt->iteration_limit);
+        o.threads = opt->threads;
+        auto r = sovereign::solve_accelerated_box(q, o, false);
+        export_result(r, out);
+        if (primal_out && r.primal.size() == q.linear.cost.size())
+            std::memcpy(primal_out, r.primal.data(), r.primal.size() * sizeof(double));
+    });
+}
+
+}  // extern "C"
