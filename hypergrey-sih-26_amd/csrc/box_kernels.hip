@@ -1,0 +1,592 @@
+// Resident box-QP kernels for gfx950 (CDNA4, wave64).
+//
+// One sparse pass per launch. A wavefront owns one row block (box_kernels.hpp):
+//   load   : lane l streams entries k0 + l + 64*j, j < kLoads, of the block's
+//            contiguous CSR span (coalesced 512-byte value and 256-byte index
+//            requests), gathers the input vector at the column and writes the
+//            rounded product to the wavefront's LDS staging area;
+//   reduce : ORDERED - the lane owning a row adds its products from LDS in
+//            stored column order, one rounding per add, exactly as
+//            CsrMatrix::multiply (sparse_matrix.hpp:30) does;
+//            TREE    - 2..64 lanes share a row and combine by shuffles;
+//   finish : the row's lane applies the epilogue: the projected gradient step
+//            and extrapolation (accelerated_qp.cpp:54-56) for a step launch,
+//            or the residual / objective / dual-bound terms of
+//            check_qp_solution (qp_admm.cpp:63-117) for a check launch.
+// The gradient therefore never reaches HBM. Grid-wide reductions (restart test,
+// check report) write one partial per workgroup; the last workgroup to arrive
+// combines the partials in workgroup order, so results are deterministic.
+//
+// This is the CSR-stream scheme of Greathouse & Daga ("Efficient sparse
+// matrix-vector multiplication on GPUs using the CSR storage format", SC 2014),
+// written for wave64 with the solver step fused into the row epilogue.
+#include "box_kernels.hpp"
+
+#include <cmath>
+#include <limits>
+
+namespace hgqp {
+namespace {
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ inline T load_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// std::max(lo, std::min(up, v)) with the comparisons of <algorithm>
+// (accelerated_qp.cpp:54; qp_admm.cpp:21).
+template <class T>
+__device__ inline T clip(T v, T lo, T up) {
+    T m = (v < up) ? v : up;
+    return (lo < m) ? m : lo;
+}
+
+// ---- double-double accumulation (Dekker / Knuth two-sum, FMA two-product) ----
+struct DD {
+    double hi, lo;
+};
+
+__device__ inline DD dd_add(DD a, DD b) {
+#pragma clang fp contract(off)
+    double s = a.hi + b.hi;
+    if (!isfinite(s)) return {s, 0.0};
+    double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e = e + a.lo + b.lo;
+    double h = s + e;
+    return {h, e - (h - s)};
+}
+
+__device__ inline DD dd_from(double a) { return {a, 0.0}; }
+
+__device__ inline DD dd_prod(double a, double b) {
+    double p = a * b;
+    if (!isfinite(p)) return {p, 0.0};
+    return {p, fma(a, b, -p)};
+}
+
+__device__ inline double shfl_xor(double v, int m) { return __shfl_xor(v, m, kWave); }
+__device__ inline float shfl_xor(float v, int m) { return __shfl_xor(v, m, kWave); }
+__device__ inline int shfl_xor(int v, int m) { return __shfl_xor(v, m, kWave); }
+
+__device__ inline double wave_sum(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor(v, m));
+    return v;
+}
+__device__ inline int wave_or(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v |= shfl_xor(v, m);
+    return v;
+}
+__device__ inline DD wave_dd(DD v) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        DD o{shfl_xor(v.hi, m), shfl_xor(v.lo, m)};
+        v = dd_add(v, o);
+    }
+    return v;
+}
+
+// ---- the shared sparse pass --------------------------------------------------
+// Calls epi(row, sum) once per row of block b from the lane that owns the row.
+template <class T, bool ORDERED, class Epi>
+__device__ inline void row_block(const Matrix<T>& m, const T* __restrict__ gat, int b, T* __restrict__ lds,
+                                 int lane, Epi& epi) {
+    const int r0 = m.blk[b], r1 = m.blk[b + 1];
+    const int k0 = m.start[r0], k1 = m.start[r1];
+    if (k1 - k0 <= kCap) {
+        T p[kLoads];
+        int cidx[kLoads];
+#pragma unroll
+        for (int j = 0; j < kLoads; ++j) {
+            const int k = k0 + j * kWave + lane;
+            cidx[j] = k < k1 ? load_stream(m.col + k) : -1;
+            p[j] = k < k1 ? load_stream(m.val + k) : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < kLoads; ++j) {
+            T g = cidx[j] >= 0 ? gat[cidx[j]] : T(0);
+            lds[j * kWave + lane] = p[j] * g;  // rounded product, as in the CPU sum
+        }
+        wave_sync();
+        const int nrows = r1 - r0;
+        if (ORDERED || nrows > kWave / 2) {
+            for (int r = r0 + lane; r < r1; r += kWave) {
+                const int a = m.start[r] - k0, e = m.start[r + 1] - k0;
+                T s = 0;
+                for (int q = a; q < e; ++q) s += lds[q];
+                epi(r, s);
+            }
+        } else {
+            // 2..64 lanes per row: lpr = 64 / next_pow2(nrows)
+            int lg = 0;
+            while ((1 << lg) < nrows) ++lg;
+            const int lpr = kWave >> lg;
+            const int r = r0 + lane / lpr, sub = lane % lpr;
+            T s = 0;
+            if (r < r1) {
+                const int a = m.start[r] - k0, e = m.start[r + 1] - k0;
+                for (int q = a + sub; q < e; q += lpr) s += lds[q];
+            }
+            for (int msk = lpr >> 1; msk >= 1; msk >>= 1) s += shfl_xor(s, msk);
+            if (sub == 0 && r < r1) epi(r, s);
+        }
+        wave_sync();
+    } else {
+        // one long row, streamed in kCap chunks
+        T s = 0;
+        for (int base = k0; base < k1; base += kCap) {
+            if (ORDERED) {
+#pragma unroll
+                for (int j = 0; j < kLoads; ++j) {
+                    const int k = base + j * kWave + lane;
+                    lds[j * kWave + lane] = k < k1 ? load_stream(m.val + k) * gat[load_stream(m.col + k)] : T(0);
+                }
+                wave_sync();
+                if (lane == 0) {
+                    const int e = min(kCap, k1 - base);
+                    for (int q = 0; q < e; ++q) s += lds[q];
+                }
+                wave_sync();
+            } else {
+#pragma unroll
+                for (int j = 0; j < kLoads; ++j) {
+                    const int k = base + j * kWave + lane;
+                    if (k < k1) s += load_stream(m.val + k) * gat[load_stream(m.col + k)];
+                }
+            }
+        }
+        if (!ORDERED) {
+            for (int msk = 32; msk >= 1; msk >>= 1) s += shfl_xor(s, msk);
+        }
+        if (lane == 0) epi(r0, s);
+    }
+}
+
+// ---- step ---------------------------------------------------------------------
+template <class T>
+struct StepEpilogue {
+    const T* __restrict__ gat;  // point the gradient was taken at
+    const T* __restrict__ xin;
+    const T* __restrict__ c;
+    const T* __restrict__ lo;
+    const T* __restrict__ up;
+    T* __restrict__ xout;
+    T* __restrict__ wout;
+    T lipschitz;
+    T beta;
+    double dot;  // restart test: sum (w - next)(next - x)
+    __device__ inline void operator()(int r, T g) {
+#pragma clang fp contract(off)
+        const T wv = gat[r], xv = xin[r];
+        const T next = clip(wv - (g + c[r]) / lipschitz, lo[r], up[r]);
+        wout[r] = next + beta * (next - xv);
+        xout[r] = next;
+        dot += double(wv - next) * double(next - xv);
+    }
+};
+
+// t-recurrence of accelerated_qp.cpp:47 with IEEE double operations, so the
+// device sequence equals the host's bit for bit.
+__device__ inline void momentum_of(double t, double* next_t, double* beta) {
+#pragma clang fp contract(off)
+    double nt = (1 + sqrt(1 + 4 * t * t)) / 2;
+    *next_t = nt;
+    *beta = (t - 1) / nt;
+}
+
+template <class T, bool ORDERED, int MOM>
+__global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T> bufs, Control* ctrl,
+                                                      double* partials, int parity, double beta_imm) {
+    __shared__ T stage[kWavesPerGroup][kCap];
+    __shared__ double wave_dot[kWavesPerGroup];
+    __shared__ int last;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    double beta = beta_imm, next_t = 0;
+    int use_x = 0;
+    if (MOM != kMomentumImmediate) {
+        if (ctrl->done) return;  // grid-uniform
+        momentum_of(ctrl->t[parity], &next_t, &beta);
+        use_x = ctrl->use_x[parity];
+    }
+    StepEpilogue<T> epi;
+    epi.gat = use_x ? bufs.x[parity] : bufs.w[parity];
+    epi.xin = bufs.x[parity];
+    epi.c = bufs.c;
+    epi.lo = bufs.lo;
+    epi.up = bufs.up;
+    epi.xout = bufs.x[parity ^ 1];
+    epi.wout = bufs.w[parity ^ 1];
+    epi.lipschitz = bufs.lipschitz;
+    epi.beta = T(beta);
+    epi.dot = 0;
+    const int b = blockIdx.x * kWavesPerGroup + wave;
+    if (b < m.nblocks) row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi);
+    if (MOM == kMomentumReference) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctrl->t[parity ^ 1] = next_t;
+            ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
+            ctrl->use_x[parity ^ 1] = 0;
+        }
+    } else if (MOM == kMomentumRestart) {
+        double d = wave_sum(epi.dot);
+        if (lane == 0) wave_dot[wave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0;
+            for (int w = 0; w < kWavesPerGroup; ++w) s += wave_dot[w];
+            partials[blockIdx.x] = s;
+            __threadfence();
+            last = atomicAdd(&ctrl->arrive, 1u) == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            double s = 0;
+            for (int g = threadIdx.x; g < (int)gridDim.x; g += kGroup) s += __builtin_nontemporal_load(partials + g);
+            s = wave_sum(s);
+            if (lane == 0) wave_dot[wave] = s;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double tot = 0;
+                for (int w = 0; w < kWavesPerGroup; ++w) tot += wave_dot[w];
+                const int restart = tot > 0;
+                ctrl->t[parity ^ 1] = restart ? 1.0 : next_t;
+                ctrl->use_x[parity ^ 1] = restart;
+                ctrl->restarts += restart;
+                ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
+                ctrl->arrive = 0;
+            }
+        }
+    }
+}
+
+// ---- check (FP64) ---------------------------------------------------------------
+struct CheckEpilogue {
+    const double* __restrict__ x;
+    const double* __restrict__ c;
+    const double* __restrict__ lo;
+    const double* __restrict__ up;
+    const int* __restrict__ start;
+    const int* __restrict__ col;
+    const double* __restrict__ val;
+    int diagonal;
+    double stat, viol;
+    DD value, tangent, separable;
+    int bound_inf, bad;
+    __device__ inline void operator()(int r, double hx) {
+#pragma clang fp contract(off)
+        const double xv = x[r], cv = c[r], l = lo[r], u = up[r];
+        if (!isfinite(xv)) { bad = 1; return; }
+        // objective term 1/2 x (Hx) + c x  (qp_admm.cpp:79)
+        DD t = dd_prod(xv, hx);
+        t.hi *= 0.5;
+        t.lo *= 0.5;
+        value = dd_add(value, dd_add(t, dd_from(cv * xv)));
+        // stationarity (qp_admm.cpp:94-97)
+        const double g = hx + cv;
+        double res = fabs(xv - clip(xv - g, l, u)) / (1 + fabs(xv));
+        if (!isfinite(g) || !isfinite(res)) res = INFINITY;
+        stat = fmax(stat, res);
+        // tangent dual bound (qp_admm.cpp:100)
+        if (g != 0) {
+            const double target = g > 0 ? l : u;
+            if (isfinite(target)) tangent = dd_add(tangent, dd_from(g * (target - xv)));
+            else bound_inf = 1;
+        }
+        // separable bound for a positive diagonal Hessian (qp_admm.cpp:106-112)
+        if (diagonal) {
+            double d = 0;
+            for (int k = start[r]; k < start[r + 1]; ++k)
+                if (col[k] == r) d = val[k];
+            const double mnz = clip(-cv / d, l, u);
+            DD s = dd_prod(0.5 * d * mnz, mnz);
+            separable = dd_add(separable, dd_add(s, dd_prod(cv, mnz)));
+        }
+        // scaled bound violation, original_violation (mps.cpp:284-295)
+        if (isfinite(l)) viol = fmax(viol, (l - xv) / (1 + fabs(l) + fabs(xv)));
+        if (isfinite(u)) viol = fmax(viol, (xv - u) / (1 + fabs(u) + fabs(xv)));
+    }
+};
+
+constexpr int kPartial = 10;  // doubles per workgroup partial of a check
+
+template <bool ORDERED>
+__global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuffers<double> bufs, Control* ctrl,
+                                                       double* partials, int parity, CheckParams prm) {
+    __shared__ double stage[kWavesPerGroup][kCap];
+    __shared__ double red[kWavesPerGroup][kPartial];
+    __shared__ int last;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (prm.set_done && ctrl->done) return;  // grid-uniform
+    CheckEpilogue epi;
+    epi.x = bufs.x[parity];
+    epi.c = bufs.c;
+    epi.lo = bufs.lo;
+    epi.up = bufs.up;
+    epi.start = m.start;
+    epi.col = m.col;
+    epi.val = m.val;
+    epi.diagonal = prm.diagonal;
+    epi.stat = 0;
+    epi.viol = 0;
+    epi.value = epi.tangent = epi.separable = DD{0, 0};
+    epi.bound_inf = epi.bad = 0;
+    const int b = blockIdx.x * kWavesPerGroup + wave;
+    if (b < m.nblocks) row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+    // workgroup partial
+    {
+        double stat = wave_max(epi.stat), viol = wave_max(epi.viol);
+        DD v = wave_dd(epi.value), tg = wave_dd(epi.tangent), sp = wave_dd(epi.separable);
+        int flags = wave_or(epi.bound_inf | (epi.bad << 1));
+        if (lane == 0) {
+            double* o = red[wave];
+            o[0] = stat; o[1] = viol; o[2] = v.hi; o[3] = v.lo; o[4] = tg.hi; o[5] = tg.lo;
+            o[6] = sp.hi; o[7] = sp.lo; o[8] = flags; o[9] = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc[kPartial];
+        for (int q = 0; q < kPartial; ++q) acc[q] = red[0][q];
+        for (int w = 1; w < kWavesPerGroup; ++w) {
+            acc[0] = fmax(acc[0], red[w][0]);
+            acc[1] = fmax(acc[1], red[w][1]);
+            for (int q = 2; q < 8; q += 2) {
+                DD s = dd_add(DD{acc[q], acc[q + 1]}, DD{red[w][q], red[w][q + 1]});
+                acc[q] = s.hi; acc[q + 1] = s.lo;
+            }
+            acc[8] = double(int(acc[8]) | int(red[w][8]));
+        }
+        for (int q = 0; q < kPartial; ++q) partials[blockIdx.x * kPartial + q] = acc[q];
+        __threadfence();
+        last = atomicAdd(&ctrl->check_arrive, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // last workgroup: fixed-order combination of all partials
+    double stat = 0, viol = 0;
+    DD v{0, 0}, tg{0, 0}, sp{0, 0};
+    int flags = 0;
+    for (int g = threadIdx.x; g < (int)gridDim.x; g += kGroup) {
+        const double* p = partials + g * kPartial;
+        stat = fmax(stat, __builtin_nontemporal_load(p + 0));
+        viol = fmax(viol, __builtin_nontemporal_load(p + 1));
+        v = dd_add(v, DD{__builtin_nontemporal_load(p + 2), __builtin_nontemporal_load(p + 3)});
+        tg = dd_add(tg, DD{__builtin_nontemporal_load(p + 4), __builtin_nontemporal_load(p + 5)});
+        sp = dd_add(sp, DD{__builtin_nontemporal_load(p + 6), __builtin_nontemporal_load(p + 7)});
+        flags |= int(__builtin_nontemporal_load(p + 8));
+    }
+    stat = wave_max(stat);
+    viol = wave_max(viol);
+    v = wave_dd(v);
+    tg = wave_dd(tg);
+    sp = wave_dd(sp);
+    flags = wave_or(flags);
+    __syncthreads();
+    if (lane == 0) {
+        double* o = red[wave];
+        o[0] = stat; o[1] = viol; o[2] = v.hi; o[3] = v.lo; o[4] = tg.hi; o[5] = tg.lo;
+        o[6] = sp.hi; o[7] = sp.lo; o[8] = flags;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < kWavesPerGroup; ++w) {
+        red[0][0] = fmax(red[0][0], red[w][0]);
+        red[0][1] = fmax(red[0][1], red[w][1]);
+        for (int q = 2; q < 8; q += 2) {
+            DD s = dd_add(DD{red[0][q], red[0][q + 1]}, DD{red[w][q], red[w][q + 1]});
+            red[0][q] = s.hi; red[0][q + 1] = s.lo;
+        }
+        red[0][8] = double(int(red[0][8]) | int(red[w][8]));
+    }
+    const double* o = red[0];
+    const int fl = int(o[8]);
+    const double nan = __builtin_nan("");
+    double objective = nan, primal = INFINITY, stationarity = INFINITY, bound = -INFINITY, gap = INFINITY;
+    int verified = 0;
+    if (!(fl & 2)) {
+        // value = sign*offset + sum(1/2 x Hx + c x) in the normalized sense (qp_admm.cpp:71,79)
+        DD value = dd_add(dd_from(prm.sign * prm.offset), DD{o[2], o[3]});
+        objective = prm.sign * (value.hi + value.lo);
+        primal = o[1];
+        stationarity = o[0];
+        DD bnd;
+        bool bnd_finite = true;
+        if (prm.diagonal) bnd = dd_add(dd_from(prm.sign * prm.offset), DD{o[6], o[7]});
+        else if (fl & 1) bnd_finite = false;
+        else bnd = dd_add(value, DD{o[4], o[5]});
+        if (bnd_finite) {
+            bound = bnd.hi + bnd.lo;
+            DD diff = dd_add(value, DD{-bnd.hi, -bnd.lo});
+            double dv = diff.hi + diff.lo;
+            gap = isfinite(bound) ? fabs(dv) / (1 + fabs(value.hi + value.lo)) : INFINITY;
+        }
+        verified = isfinite(objective) && fmax(primal, fmax(stationarity, 0.0)) <= prm.tol;
+        if (prm.diagonal) verified = verified && gap <= prm.tol;
+    }
+    const int finite = isfinite(objective) && isfinite(stationarity);
+    ctrl->objective = objective;
+    ctrl->primal_residual = primal;
+    ctrl->stationarity = stationarity;
+    ctrl->dual_bound = bound;
+    ctrl->relative_gap = gap;
+    ctrl->verified = verified;
+    ctrl->finite = finite;
+    ctrl->check_iter = ctrl->iter[parity];
+    ctrl->check_parity = parity;
+    ctrl->checks += 1;
+    if (prm.set_done) {
+        if (verified) ctrl->done = kVerified;
+        else if (!finite) ctrl->done = kNumerical;
+    }
+    ctrl->check_arrive = 0;
+}
+
+// ---- check of the FP32 phase: stationarity only, decides the switch to FP64 ----
+struct Check32Epilogue {
+    const float* __restrict__ x;
+    const float* __restrict__ c;
+    const float* __restrict__ lo;
+    const float* __restrict__ up;
+    double stat;
+    __device__ inline void operator()(int r, float hx) {
+        const float xv = x[r];
+        const float g = hx + c[r];
+        float res = fabsf(xv - clip(xv - g, lo[r], up[r])) / (1 + fabsf(xv));
+        if (!isfinite(g) || !isfinite(res) || !isfinite(xv)) res = INFINITY;
+        stat = fmax(stat, double(res));
+    }
+};
+
+template <bool ORDERED>
+__global__ __launch_bounds__(kGroup) void check32_kernel(Matrix<float> m, StepBuffers<float> bufs, Control* ctrl,
+                                                         double* partials, int parity, CheckParams prm) {
+    __shared__ float stage[kWavesPerGroup][kCap];
+    __shared__ double red[kWavesPerGroup];
+    __shared__ int last;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (ctrl->done) return;
+    Check32Epilogue epi{bufs.x[parity], bufs.c, bufs.lo, bufs.up, 0.0};
+    const int b = blockIdx.x * kWavesPerGroup + wave;
+    if (b < m.nblocks) row_block<float, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+    double s = wave_max(epi.stat);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red[0];
+        for (int w = 1; w < kWavesPerGroup; ++w) a = fmax(a, red[w]);
+        partials[blockIdx.x] = a;
+        __threadfence();
+        last = atomicAdd(&ctrl->check_arrive, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double a = 0;
+    for (int g = threadIdx.x; g < (int)gridDim.x; g += kGroup) a = fmax(a, __builtin_nontemporal_load(partials + g));
+    a = wave_max(a);
+    __syncthreads();
+    if (lane == 0) red[wave] = a;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < kWavesPerGroup; ++w) red[0] = fmax(red[0], red[w]);
+    const double stat = red[0];
+    ctrl->stationarity = stat;
+    ctrl->check_iter = ctrl->iter[parity];
+    ctrl->check_parity = parity;
+    ctrl->checks += 1;
+    if (!(stat <= ctrl->best_stat * 0.5)) ctrl->stall += 1;
+    else ctrl->stall = 0;
+    if (stat < ctrl->best_stat) ctrl->best_stat = stat;
+    // The FP32 phase ends at the switch threshold, on stagnation (FP32 rounding
+    // floor reached) or on a non-finite residual; the FP64 check then decides.
+    if (!(stat > prm.switch_stat) || ctrl->stall >= 4) ctrl->done = kPhaseEnd;
+    ctrl->check_arrive = 0;
+}
+
+template <class T>
+__global__ void init_kernel(int64_t n, const T* lo, const T* up, T* x, T* w) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        T v = clip(T(0), lo[j], up[j]);
+        x[j] = v;
+        w[j] = v;
+    }
+}
+
+template <class To, class From>
+__global__ void convert_kernel(int64_t n, const From* src, To* dst) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        dst[j] = To(src[j]);
+}
+
+int elementwise_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return int(g < 8192 ? (g < 1 ? 1 : g) : 8192);
+}
+
+}  // namespace
+
+int groups_for(int nblocks) { return (nblocks + kWavesPerGroup - 1) / kWavesPerGroup; }
+
+template <class T>
+void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
+                 int momentum, bool ordered, double beta, hipStream_t stream) {
+    dim3 grid(groups_for(m.nblocks)), block(kGroup);
+#define HGQP_STEP(ORD, MOM) hipLaunchKernelGGL((step_kernel<T, ORD, MOM>), grid, block, 0, stream, m, b, ctrl, partials, parity, beta)
+    if (ordered) {
+        if (momentum == kMomentumImmediate) HGQP_STEP(true, kMomentumImmediate);
+        else if (momentum == kMomentumReference) HGQP_STEP(true, kMomentumReference);
+        else HGQP_STEP(true, kMomentumRestart);
+    } else {
+        if (momentum == kMomentumImmediate) HGQP_STEP(false, kMomentumImmediate);
+        else if (momentum == kMomentumReference) HGQP_STEP(false, kMomentumReference);
+        else HGQP_STEP(false, kMomentumRestart);
+    }
+#undef HGQP_STEP
+}
+
+template void launch_step<double>(const Matrix<double>&, const StepBuffers<double>&, Control*, double*, int, int,
+                                  bool, double, hipStream_t);
+template void launch_step<float>(const Matrix<float>&, const StepBuffers<float>&, Control*, double*, int, int, bool,
+                                 double, hipStream_t);
+
+void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
+                  int parity, bool ordered, const CheckParams& params, hipStream_t stream) {
+    dim3 grid(groups_for(m.nblocks)), block(kGroup);
+    if (ordered) hipLaunchKernelGGL(check_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+    else hipLaunchKernelGGL(check_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+}
+
+void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
+                    int parity, bool ordered, const CheckParams& params, hipStream_t stream) {
+    dim3 grid(groups_for(m.nblocks)), block(kGroup);
+    if (ordered) hipLaunchKernelGGL(check32_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+    else hipLaunchKernelGGL(check32_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+}
+
+template <class T>
+void launch_init(int64_t n, const T* lo, const T* up, T* x, T* w, hipStream_t stream) {
+    hipLaunchKernelGGL(init_kernel<T>, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, lo, up, x, w);
+}
+template void launch_init<double>(int64_t, const double*, const double*, double*, double*, hipStream_t);
+template void launch_init<float>(int64_t, const float*, const float*, float*, float*, hipStream_t);
+
+template <class To, class From>
+void launch_convert(int64_t n, const From* src, To* dst, hipStream_t stream) {
+    hipLaunchKernelGGL((convert_kernel<To, From>), dim3(elementwise_grid(n)), dim3(256), 0, stream, n, src, dst);
+}
+template void launch_convert<float, double>(int64_t, const double*, float*, hipStream_t);
+template void launch_convert<double, float>(int64_t, const float*, double*, hipStream_t);
+template void launch_convert<double, double>(int64_t, const double*, double*, hipStream_t);
+
+}  // namespace hgqp

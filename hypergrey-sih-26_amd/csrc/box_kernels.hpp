@@ -1,0 +1,101 @@
+// Device data layout and kernel launchers of the resident box engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hgqp {
+
+// Row blocks: consecutive rows whose stored entries fit one wavefront's
+// staging area (kCap entries, kLoads per lane), at most kMaxRows rows; a row
+// longer than kCap forms a block of its own and is streamed in kCap chunks.
+constexpr int kWave = 64;
+constexpr int kWavesPerGroup = 4;
+constexpr int kGroup = kWave * kWavesPerGroup;
+constexpr int kLoads = 8;
+constexpr int kCap = kWave * kLoads;
+constexpr int kMaxRows = 256;
+
+// Momentum source of a step launch.
+enum Momentum : int {
+    kMomentumImmediate = 0,  // beta passed by the host (BoxIteration::step)
+    kMomentumReference = 1,  // t-recurrence on the device, no restart
+    kMomentumRestart = 2,    // t-recurrence with gradient restart
+};
+
+// Done codes written by check kernels.
+enum Done : int { kRunning = 0, kVerified = 1, kNumerical = 2, kPhaseEnd = 3 };
+
+// One per handle, in device memory. Fields indexed [parity] are double
+// buffered: a step launch of parity p reads slot p and writes slot p^1, so no
+// reader of a slot runs concurrently with its writer.
+struct Control {
+    double t[2];
+    int64_t iter[2];
+    int32_t use_x[2];  // 1: the step gathers x instead of the extrapolated point (after a restart)
+    int32_t done;
+    int32_t restarts;
+    uint32_t arrive;        // last-group counter of restart steps
+    uint32_t check_arrive;  // last-group counter of checks
+    int64_t check_iter;     // iteration of the last evaluated check
+    int32_t check_parity;   // buffer parity of the last evaluated check
+    int32_t checks;
+    int32_t verified;
+    int32_t finite;
+    int32_t stall;          // FP32 phase: checks without progress
+    int32_t pad;
+    double objective, primal_residual, stationarity, dual_bound, relative_gap;
+    double best_stat;       // FP32 phase: best stationarity seen
+};
+
+template <class T>
+struct Matrix {
+    int nblocks;         // row blocks
+    const int* blk;      // [nblocks + 1] first row of each block
+    const int* start;    // [n + 1]
+    const int* col;      // [nnz]
+    const T* val;        // [nnz]
+};
+
+template <class T>
+struct StepBuffers {
+    const T* c;
+    const T* lo;
+    const T* up;
+    T* x[2];
+    T* w[2];
+    T lipschitz;
+};
+
+struct CheckParams {
+    double tol;
+    double sign;        // -1 when the original model maximizes
+    double offset;      // original objective constant
+    int diagonal;       // 1: Hessian is positive diagonal (separable dual bound)
+    int set_done;       // 1: a verified / non-finite check stops the loop
+    double switch_stat; // FP32 phase: stationarity that ends the phase
+};
+
+// Launchers. parity selects x[parity], w[parity] as inputs; step launches
+// write x[parity^1], w[parity^1]. Every launch goes to `stream`.
+template <class T>
+void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
+                 int momentum, bool ordered, double beta, hipStream_t stream);
+
+void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
+                  int parity, bool ordered, const CheckParams& params, hipStream_t stream);
+
+void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
+                    int parity, bool ordered, const CheckParams& params, hipStream_t stream);
+
+// x[0] = w[0] = clip(0, lower, upper)
+template <class T>
+void launch_init(int64_t n, const T* lo, const T* up, T* x, T* w, hipStream_t stream);
+
+// dst = (To)src, elementwise
+template <class To, class From>
+void launch_convert(int64_t n, const From* src, To* dst, hipStream_t stream);
+
+int groups_for(int nblocks);
+
+}  // namespace hgqp

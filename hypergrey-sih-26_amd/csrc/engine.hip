@@ -1,0 +1,625 @@
+// Resident box engine: device memory, the HIP stream, the device-resident solve
+// loop and the C ABI of include/hgqp.h for single instances.
+#include "box_kernels.hpp"
+#include "hgqp.h"
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace hgqp {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& m) { g_last_error = m; }
+void clear_last_error() { g_last_error.clear(); }
+const char* last_error() { return g_last_error.c_str(); }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    int code = e == hipErrorOutOfMemory ? HGQP_OUT_OF_MEMORY
+             : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? HGQP_NO_DEVICE : HGQP_DEVICE_ERROR;
+    throw Error(code, std::string("HIP: ") + what + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(call) hip_check((call), #call)
+
+// Checks that a HIP device is present and is gfx950; the library carries
+// gfx950 code only.
+void require_device() {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) throw Error(HGQP_NO_DEVICE, "No HIP device available");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw Error(HGQP_NO_DEVICE, std::string("hgqp is built for gfx950; device is ") + prop.gcnArchName);
+}
+
+// One hipMalloc per handle; sub-buffers are 256-byte aligned slices.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    void reserve(size_t bytes) {
+        size = bytes;
+        HIPCHECK(hipMalloc(reinterpret_cast<void**>(&base), bytes));
+    }
+    template <class T>
+    T* take(size_t count) {
+        size_t off = (used + 255) & ~size_t(255);
+        size_t bytes = count * sizeof(T);
+        if (off + bytes > size) throw Error(HGQP_DEVICE_ERROR, "internal: arena overflow");
+        used = off + bytes;
+        return reinterpret_cast<T*>(base + off);
+    }
+    static size_t span(size_t count, size_t elem) { return ((count * elem + 255) & ~size_t(255)) + 256; }
+    ~Arena() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+std::vector<int> row_blocks(int64_t n, const int32_t* start) {
+    std::vector<int> blk{0};
+    int64_t r = 0;
+    while (r < n) {
+        const int64_t rs = r;
+        if (start[r + 1] - start[r] > kCap) {
+            ++r;
+        } else {
+            int64_t cnt = 0;
+            while (r < n && r - rs < kMaxRows && cnt + (start[r + 1] - start[r]) <= kCap) {
+                cnt += start[r + 1] - start[r];
+                ++r;
+            }
+        }
+        blk.push_back(static_cast<int>(r));
+    }
+    return blk;
+}
+
+}  // namespace hgqp
+
+using namespace hgqp;
+
+struct hgqp_box {
+    int64_t n = 0, nnz = 0;
+    int nblocks = 0, groups = 0;
+    hgqp_box_options opt{};
+    int diagonal = 0;
+    double lipschitz = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    int* blk = nullptr;
+    int* start = nullptr;
+    int* col = nullptr;
+    double* val = nullptr;
+    float* val32 = nullptr;
+    double *c = nullptr, *lo = nullptr, *up = nullptr;
+    float *c32 = nullptr, *lo32 = nullptr, *up32 = nullptr;
+    double* x[2] = {nullptr, nullptr};
+    double* w[2] = {nullptr, nullptr};
+    float* x32[2] = {nullptr, nullptr};
+    float* w32[2] = {nullptr, nullptr};
+    Control* ctrl = nullptr;
+    double* partials = nullptr;
+    Control* host_slots = nullptr;  // pinned read-back ring
+    static constexpr int kSlots = 8;
+    hipEvent_t slot_event[kSlots] = {};
+    int parity = 0;
+    // captured check periods, keyed by (precision, start parity, steps)
+    struct Graph {
+        int precision, parity, steps, momentum;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    std::vector<hipEvent_t> events;  // timing pairs
+
+    Matrix<double> m64() const { return Matrix<double>{nblocks, blk, start, col, val}; }
+    Matrix<float> m32() const { return Matrix<float>{nblocks, blk, start, col, val32}; }
+    StepBuffers<double> b64() const {
+        return StepBuffers<double>{c, lo, up, {x[0], x[1]}, {w[0], w[1]}, lipschitz};
+    }
+    StepBuffers<float> b32() const {
+        return StepBuffers<float>{c32, lo32, up32, {x32[0], x32[1]}, {w32[0], w32[1]}, float(lipschitz)};
+    }
+    bool ordered() const { return opt.reduce == HGQP_REDUCE_ORDERED; }
+
+    ~hgqp_box() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (auto e : events) (void)hipEventDestroy(e);
+        for (auto e : slot_event)
+            if (e) (void)hipEventDestroy(e);
+        if (host_slots) (void)hipHostFree(host_slots);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void check_box(const hgqp_box* b) {
+    if (!b) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_box handle");
+}
+
+// Writes the host view of the control block; used outside captured work only.
+void put_control(hgqp_box* b, const Control& c) {
+    std::memcpy(&b->host_slots[0], &c, sizeof(Control));
+    HIPCHECK(hipMemcpyAsync(b->ctrl, &b->host_slots[0], sizeof(Control), hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+}
+
+Control get_control(hgqp_box* b) {
+    HIPCHECK(hipMemcpyAsync(&b->host_slots[0], b->ctrl, sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    Control c;
+    std::memcpy(&c, &b->host_slots[0], sizeof(Control));
+    return c;
+}
+
+Control fresh_control() {
+    Control c;
+    std::memset(&c, 0, sizeof(c));
+    c.t[0] = c.t[1] = 1.0;
+    c.best_stat = std::numeric_limits<double>::infinity();
+    c.objective = std::numeric_limits<double>::quiet_NaN();
+    c.primal_residual = c.stationarity = c.relative_gap = std::numeric_limits<double>::infinity();
+    c.dual_bound = -std::numeric_limits<double>::infinity();
+    return c;
+}
+
+CheckParams check_params(const hgqp_box* b, double tol, int set_done, double switch_stat) {
+    CheckParams p;
+    p.tol = tol;
+    p.sign = b->opt.maximize ? -1.0 : 1.0;
+    p.offset = b->opt.offset;
+    p.diagonal = b->diagonal;
+    p.set_done = set_done;
+    p.switch_stat = switch_stat;
+    return p;
+}
+
+void fill_report(const Control& c, hgqp_check_report* r) {
+    r->objective = c.objective;
+    r->primal_residual = c.primal_residual;
+    r->stationarity_residual = c.stationarity;
+    r->complementarity_residual = std::isfinite(c.objective) ? 0.0 : std::numeric_limits<double>::infinity();
+    r->dual_bound = c.dual_bound;
+    r->relative_gap = c.relative_gap;
+    r->optimality_verified = c.verified;
+    r->finite = c.finite;
+}
+
+// Enqueues one check period: `steps` steps from parity b->parity, then a check.
+void enqueue_period(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
+    const bool ord = b->ordered();
+    for (int s = 0; s < steps; ++s) {
+        if (precision == HGQP_PRECISION_FP64)
+            launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+        else
+            launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+        b->parity ^= 1;
+    }
+    if (precision == HGQP_PRECISION_FP64)
+        launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
+    else
+        launch_check32(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
+}
+
+// Timed variant: an event pair around every step launch.
+void enqueue_period_timed(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp,
+                          size_t* next_event) {
+    const bool ord = b->ordered();
+    for (int s = 0; s < steps; ++s) {
+        while (b->events.size() < *next_event + 2) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            b->events.push_back(e);
+        }
+        HIPCHECK(hipEventRecord(b->events[*next_event], b->stream));
+        if (precision == HGQP_PRECISION_FP64)
+            launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+        else
+            launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+        HIPCHECK(hipEventRecord(b->events[*next_event + 1], b->stream));
+        *next_event += 2;
+        b->parity ^= 1;
+    }
+    if (precision == HGQP_PRECISION_FP64)
+        launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
+    else
+        launch_check32(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
+}
+
+hipGraphExec_t period_graph(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
+    for (auto& g : b->graphs)
+        if (g.precision == precision && g.parity == b->parity && g.steps == steps && g.momentum == momentum)
+            return g.exec;
+    const int p0 = b->parity;
+    hipGraph_t graph;
+    HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_period(b, precision, steps, momentum, cp);
+    HIPCHECK(hipStreamEndCapture(b->stream, &graph));
+    hipGraphExec_t exec;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(e);
+    b->parity = p0;
+    b->graphs.push_back({precision, p0, steps, momentum, exec});
+    return exec;
+}
+
+struct LoopOutcome {
+    bool time_up = false;
+    uint64_t iterations = 0;  // iterations enqueued
+};
+
+// Runs periods of one precision until a check sets `done`, the iteration limit
+// or the time limit. `it` counts iterations of the whole solve.
+LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, uint64_t limit, int check_every,
+                      const CheckParams& cp, std::chrono::steady_clock::time_point t0, double time_limit,
+                      size_t* timed_events) {
+    LoopOutcome out;
+    const int lag = 2;
+    std::vector<int> pending;  // slot indices of enqueued read-backs, oldest first
+    int next_slot = 1;         // slot 0 is kept for synchronous transfers
+    bool stop = false;
+    while (it < limit && !stop) {
+        const int steps = static_cast<int>(std::min<uint64_t>(check_every, limit - it));
+        if (timed_events) enqueue_period_timed(b, precision, steps, momentum, cp, timed_events);
+        else if (b->opt.use_graph && steps == check_every) {
+            hipGraphExec_t g = period_graph(b, precision, steps, momentum, cp);
+            HIPCHECK(hipGraphLaunch(g, b->stream));
+            b->parity ^= (steps & 1);
+        } else {
+            enqueue_period(b, precision, steps, momentum, cp);
+        }
+        it += steps;
+        const int slot = next_slot;
+        next_slot = next_slot % (hgqp_box::kSlots - 1) + 1;
+        HIPCHECK(hipMemcpyAsync(&b->host_slots[slot], b->ctrl, sizeof(Control), hipMemcpyDeviceToHost, b->stream));
+        HIPCHECK(hipEventRecord(b->slot_event[slot], b->stream));
+        pending.push_back(slot);
+        while ((int)pending.size() > lag || (it >= limit && !pending.empty())) {
+            const int s = pending.front();
+            pending.erase(pending.begin());
+            HIPCHECK(hipEventSynchronize(b->slot_event[s]));
+            if (b->host_slots[s].done != kRunning) { stop = true; break; }
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= time_limit) {
+                out.time_up = true;
+                stop = true;
+                break;
+            }
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    out.iterations = it;
+    return out;
+}
+
+// Small host-side edits of the control block between phases (not captured).
+void edit_control(hgqp_box* b, void (*edit)(Control&)) {
+    Control c = get_control(b);
+    edit(c);
+    put_control(b, c);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hgqp_last_error(void) { return hgqp::last_error(); }
+
+void hgqp_box_options_default(hgqp_box_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = HGQP_PRECISION_FP64;
+    o->momentum = HGQP_MOMENTUM_REFERENCE;
+    o->reduce = HGQP_REDUCE_ORDERED;
+    o->use_graph = 1;
+    o->timing = 0;
+    o->maximize = 0;
+    o->offset = 0;
+    o->diagonal_hessian = -1;
+}
+
+void hgqp_solve_options_default(hgqp_solve_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->tolerance = 1e-7;
+    o->time_limit = 30;
+    o->iteration_limit = 200000;
+    o->check_every = 25;
+    o->fp32_switch = 0;
+}
+
+int hgqp_device_name(char* buffer, size_t length) {
+    return guarded([&] {
+        if (!buffer || length == 0) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_device_name: empty buffer");
+        buffer[0] = 0;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw Error(HGQP_NO_DEVICE, "No HIP device available");
+        int dev = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, dev));
+        std::string name = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+                           std::to_string(prop.multiProcessorCount) + " CUs, " +
+                           std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
+        std::strncpy(buffer, name.c_str(), length - 1);
+        buffer[length - 1] = 0;
+    });
+}
+
+int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
+                    const double* cost, const double* lower, const double* upper, double lipschitz,
+                    const hgqp_box_options* options, hgqp_box** out) {
+    return guarded([&] {
+        if (!out) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: null output handle");
+        *out = nullptr;
+        if (n <= 0 || n > 2000000000ll || nnz < 0 || nnz > 2000000000ll)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: need 0 < n and 0 <= nnz <= 2e9");
+        if (!start || (nnz && (!index || !value)) || !cost || !lower || !upper)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: null array");
+        if (!(lipschitz > 0) || !std::isfinite(lipschitz))
+            throw Error(HGQP_UNSUPPORTED, "hgqp_box_create: the GPU engine needs a positive finite Lipschitz bound "
+                                          "(zero-Hessian models use the CPU LP path)");
+        if (start[0] != 0 || start[n] != nnz) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: malformed CSR start");
+        for (int64_t i = 0; i < n; ++i)
+            if (start[i + 1] < start[i]) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: CSR start not monotone");
+        bool diag = true;
+        for (int64_t i = 0; i < n; ++i) {
+            bool has_pos_diag = false;
+            for (int32_t k = start[i]; k < start[i + 1]; ++k) {
+                if (index[k] < 0 || index[k] >= n) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column out of range");
+                if (index[k] == i) has_pos_diag = value[k] > 0;
+                else diag = false;
+            }
+            diag = diag && has_pos_diag;
+        }
+        for (int64_t j = 0; j < n; ++j)
+            if (std::isnan(lower[j]) || std::isnan(upper[j]) || lower[j] > upper[j])
+                throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: bounds must satisfy lower <= upper");
+        require_device();
+        hgqp_box_options opt;
+        if (options) opt = *options;
+        else hgqp_box_options_default(&opt);
+        if (opt.precision != HGQP_PRECISION_FP64 && opt.precision != HGQP_PRECISION_FP32_POLISH)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown precision");
+        if (opt.momentum != HGQP_MOMENTUM_REFERENCE && opt.momentum != HGQP_MOMENTUM_RESTART)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown momentum schedule");
+        if (opt.reduce != HGQP_REDUCE_ORDERED && opt.reduce != HGQP_REDUCE_TREE)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown reduction order");
+
+        auto b = std::make_unique<hgqp_box>();
+        b->n = n;
+        b->nnz = nnz;
+        b->opt = opt;
+        b->lipschitz = lipschitz;
+        b->diagonal = opt.diagonal_hessian < 0 ? (diag ? 1 : 0) : (opt.diagonal_hessian ? 1 : 0);
+        std::vector<int> blk = row_blocks(n, start);
+        b->nblocks = static_cast<int>(blk.size()) - 1;
+        b->groups = groups_for(b->nblocks);
+        const bool f32 = opt.precision == HGQP_PRECISION_FP32_POLISH;
+        const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+        size_t bytes = Arena::span(blk.size(), 4) + Arena::span(un + 1, 4) + Arena::span(unnz, 4) +
+                       Arena::span(unnz, 8) + 3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) +
+                       Arena::span(1, sizeof(Control)) + Arena::span(size_t(b->groups) * 10 + 16, 8);
+        if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
+        HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->arena.reserve(bytes);
+        b->blk = b->arena.take<int>(blk.size());
+        b->start = b->arena.take<int>(un + 1);
+        b->col = b->arena.take<int>(unnz);
+        b->val = b->arena.take<double>(unnz);
+        b->c = b->arena.take<double>(un);
+        b->lo = b->arena.take<double>(un);
+        b->up = b->arena.take<double>(un);
+        for (int p = 0; p < 2; ++p) {
+            b->x[p] = b->arena.take<double>(un);
+            b->w[p] = b->arena.take<double>(un);
+        }
+        b->ctrl = b->arena.take<Control>(1);
+        b->partials = b->arena.take<double>(size_t(b->groups) * 10 + 16);
+        if (f32) {
+            b->val32 = b->arena.take<float>(unnz);
+            b->c32 = b->arena.take<float>(un);
+            b->lo32 = b->arena.take<float>(un);
+            b->up32 = b->arena.take<float>(un);
+            for (int p = 0; p < 2; ++p) {
+                b->x32[p] = b->arena.take<float>(un);
+                b->w32[p] = b->arena.take<float>(un);
+            }
+        }
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(Control) * hgqp_box::kSlots,
+                               hipHostMallocDefault));
+        for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHECK(hipMemcpyAsync(b->blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, b->stream));
+        HIPCHECK(hipMemcpyAsync(b->start, start, (un + 1) * 4, hipMemcpyHostToDevice, b->stream));
+        if (nnz) {
+            HIPCHECK(hipMemcpyAsync(b->col, index, un ? size_t(nnz) * 4 : 0, hipMemcpyHostToDevice, b->stream));
+            HIPCHECK(hipMemcpyAsync(b->val, value, size_t(nnz) * 8, hipMemcpyHostToDevice, b->stream));
+        }
+        HIPCHECK(hipMemcpyAsync(b->c, cost, un * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
+        HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
+        if (f32) {
+            if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);
+            launch_convert<float, double>(n, b->c, b->c32, b->stream);
+            launch_convert<float, double>(n, b->lo, b->lo32, b->stream);
+            launch_convert<float, double>(n, b->up, b->up32, b->stream);
+        }
+        launch_init<double>(n, b->lo, b->up, b->x[0], b->w[0], b->stream);
+        HIPCHECK(hipGetLastError());
+        put_control(b.get(), fresh_control());  // synchronizes: the caller's arrays may be released
+        *out = b.release();
+    });
+}
+
+int hgqp_box_step(hgqp_box* b, double momentum) {
+    return guarded([&] {
+        check_box(b);
+        launch_step<double>(b->m64(), b->b64(), nullptr, nullptr, b->parity, kMomentumImmediate, b->ordered(), momentum,
+                            b->stream);
+        HIPCHECK(hipGetLastError());
+        b->parity ^= 1;
+    });
+}
+
+int hgqp_box_primal(hgqp_box* b, double* x, int64_t n) {
+    return guarded([&] {
+        check_box(b);
+        if (!x || n != b->n) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_primal: output length must equal n");
+        HIPCHECK(hipMemcpyAsync(x, b->x[b->parity], size_t(n) * 8, hipMemcpyDeviceToHost, b->stream));
+        HIPCHECK(hipStreamSynchronize(b->stream));
+    });
+}
+
+int hgqp_box_check(hgqp_box* b, double tolerance, hgqp_check_report* report) {
+    return guarded([&] {
+        check_box(b);
+        if (!report || !(tolerance > 0) || !std::isfinite(tolerance))
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_check: need a report and a positive tolerance");
+        launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(),
+                     check_params(b, tolerance, 0, 0.0), b->stream);
+        HIPCHECK(hipGetLastError());
+        fill_report(get_control(b), report);
+    });
+}
+
+int hgqp_box_reset(hgqp_box* b) {
+    return guarded([&] {
+        check_box(b);
+        b->parity = 0;
+        launch_init<double>(b->n, b->lo, b->up, b->x[0], b->w[0], b->stream);
+        HIPCHECK(hipGetLastError());
+        put_control(b, fresh_control());
+    });
+}
+
+int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_report* report) {
+    return guarded([&] {
+        check_box(b);
+        if (!report) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_solve: null report");
+        hgqp_solve_options so;
+        if (options) so = *options;
+        else hgqp_solve_options_default(&so);
+        if (!(so.tolerance > 0) || !std::isfinite(so.tolerance) || so.time_limit < 0 || !std::isfinite(so.time_limit) ||
+            so.check_every < 1)
+            throw Error(HGQP_INVALID_ARGUMENT, "Invalid box QP options");
+        std::memset(report, 0, sizeof(*report));
+        const auto t0 = std::chrono::steady_clock::now();
+        const int momentum = b->opt.momentum == HGQP_MOMENTUM_RESTART ? kMomentumRestart : kMomentumReference;
+        const bool f32 = b->opt.precision == HGQP_PRECISION_FP32_POLISH;
+        const double switch_stat = so.fp32_switch > 0 ? so.fp32_switch : 10 * so.tolerance;
+        const CheckParams cp64 = check_params(b, so.tolerance, 1, 0.0);
+        const CheckParams cp32 = check_params(b, so.tolerance, 1, switch_stat);
+        size_t timed = 0;
+        size_t* timed_ptr = b->opt.timing ? &timed : nullptr;
+        struct Span { uint64_t first, end; int precision; size_t ev_first, ev_end; };
+        std::vector<Span> spans;
+
+        // fresh momentum state at the handle's current iterate (accelerated_qp.cpp:37)
+        Control c0 = fresh_control();
+        put_control(b, c0);
+        uint64_t it = 0;
+        bool time_up = false;
+        uint64_t fp32_its = 0;
+        // check at iteration 0 (accelerated_qp.cpp:40)
+        launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), cp64, b->stream);
+        HIPCHECK(hipGetLastError());
+        Control c = get_control(b);
+        if (c.done == kRunning && f32 && so.iteration_limit > 0) {
+            launch_convert<float, double>(b->n, b->x[b->parity], b->x32[b->parity], b->stream);
+            launch_convert<float, double>(b->n, b->w[b->parity], b->w32[b->parity], b->stream);
+            HIPCHECK(hipGetLastError());
+            const size_t ev0 = timed;
+            auto o = run_phase(b, HGQP_PRECISION_FP32_POLISH, momentum, it, so.iteration_limit, so.check_every, cp32, t0,
+                               so.time_limit, timed_ptr);
+            time_up = o.time_up;
+            c = get_control(b);
+            // steps after the deciding FP32 check returned early without flipping buffers
+            if (c.done != kRunning) {
+                it = uint64_t(c.check_iter);
+                b->parity = c.check_parity;
+            }
+            fp32_its = it;
+            spans.push_back({0, it, HGQP_PRECISION_FP32_POLISH, ev0, timed});
+            launch_convert<double, float>(b->n, b->x32[b->parity], b->x[b->parity], b->stream);
+            launch_convert<double, float>(b->n, b->x32[b->parity], b->w[b->parity], b->stream);
+            Control cc = c;
+            cc.done = kRunning;
+            cc.t[b->parity] = 1.0;
+            cc.use_x[b->parity] = 0;
+            cc.iter[b->parity] = int64_t(it);
+            put_control(b, cc);
+            launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), cp64, b->stream);
+            HIPCHECK(hipGetLastError());
+            c = get_control(b);
+        }
+        if (c.done == kRunning && !time_up && it < so.iteration_limit) {
+            const uint64_t first = it;
+            const size_t ev0 = timed;
+            auto o = run_phase(b, HGQP_PRECISION_FP64, momentum, it, so.iteration_limit, so.check_every, cp64, t0,
+                               so.time_limit, timed_ptr);
+            time_up = o.time_up;
+            c = get_control(b);
+            if (c.done != kRunning) b->parity = c.check_parity;
+            spans.push_back({first, uint64_t(c.check_iter), HGQP_PRECISION_FP64, ev0, timed});
+        }
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        report->iterations = uint64_t(c.check_iter);
+        report->fp32_iterations = fp32_its;
+        report->restarts = c.restarts;
+        report->checks = uint64_t(c.checks);
+        fill_report(c, &report->check);
+        if (c.done == kVerified) report->status = HGQP_STATUS_OPTIMAL;
+        else if (c.done == kNumerical) report->status = HGQP_STATUS_NUMERICAL_ERROR;
+        else if (time_up) report->status = HGQP_STATUS_TIME_LIMIT;
+        else report->status = HGQP_STATUS_ITERATION_LIMIT;
+        report->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // event pairs of steps that did work: iteration indices [first, end) of each span
+        for (const Span& s : spans) {
+            const uint64_t work = s.end > s.first ? s.end - s.first : 0;
+            double ms = 0;
+            uint64_t cnt = 0;
+            for (size_t e = s.ev_first; e + 1 < s.ev_end && cnt < work; e += 2, ++cnt) {
+                float t = 0;
+                HIPCHECK(hipEventElapsedTime(&t, b->events[e], b->events[e + 1]));
+                ms += t;
+            }
+            if (s.precision == HGQP_PRECISION_FP64) { report->step_kernel_ms += ms; report->step_launches += cnt; }
+            else { report->step32_kernel_ms += ms; report->step32_launches += cnt; }
+        }
+    });
+}
+
+int hgqp_box_destroy(hgqp_box* b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipStreamSynchronize(b->stream);
+        delete b;
+    });
+}
+
+int hgqp_box_info_get(const hgqp_box* b, hgqp_box_info* info) {
+    return guarded([&] {
+        check_box(b);
+        if (!info) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_info_get: null info");
+        info->n = b->n;
+        info->nnz = b->nnz;
+        info->row_blocks = b->nblocks;
+        info->device_bytes = int64_t(b->arena.size);
+        info->lipschitz = b->lipschitz;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// Library-internal helpers: error transport across the C ABI and HIP checks.
+#pragma once
+#include "hgqp.h"
+
+#include <cstdint>
+#include <exception>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+namespace hgqp {
+
+// An error carrying an HGQP_* code. No exception crosses the C ABI: every
+// extern "C" entry runs its body through guarded(), which stores the message
+// for hgqp_last_error() and returns the code.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& message);
+void clear_last_error();
+const char* last_error();
+
+template <class F>
+int guarded(F&& body) {
+    try {
+        body();
+        return HGQP_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        set_last_error(e.what());
+        return HGQP_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return HGQP_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return HGQP_DEVICE_ERROR;
+    } catch (...) {
+        set_last_error("unknown error");
+        return HGQP_DEVICE_ERROR;
+    }
+}
+
+}  // namespace hgqp

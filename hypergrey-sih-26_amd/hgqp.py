@@ -1,0 +1,315 @@
+"""Python binding of the hgqp C ABI (include/hgqp.h) through ctypes.
+
+The library is ``libhgqp.so`` next to this file, built for gfx950 by
+``make -C hypergrey-sih-26_amd``. Loading fails loudly when it is missing:
+there is no other implementation behind these calls.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libhgqp.so")
+
+OK, INVALID_ARGUMENT, NO_DEVICE, OUT_OF_MEMORY, DEVICE_ERROR, BUSY, UNSUPPORTED = range(7)
+STATUS = ["optimal", "iteration_limit", "time_limit", "numerical_error", "infeasible_bounds",
+          "unsupported_curvature", "running"]
+PRECISION_FP64, PRECISION_FP32_POLISH = 0, 1
+MOMENTUM_REFERENCE, MOMENTUM_RESTART = 0, 1
+REDUCE_ORDERED, REDUCE_TREE = 0, 1
+
+
+class HgqpError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"hgqp error {code}: {message}")
+        self.code = code
+
+
+class InvalidArgument(HgqpError, ValueError):
+    pass
+
+
+class BoxOptions(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
+                ("use_graph", C.c_int32), ("timing", C.c_int32), ("maximize", C.c_int32),
+                ("offset", C.c_double), ("diagonal_hessian", C.c_int32), ("reserved", C.c_int32)]
+
+
+class CheckReport(C.Structure):
+    _fields_ = [("objective", C.c_double), ("primal_residual", C.c_double),
+                ("stationarity_residual", C.c_double), ("complementarity_residual", C.c_double),
+                ("dual_bound", C.c_double), ("relative_gap", C.c_double),
+                ("optimality_verified", C.c_int32), ("finite", C.c_int32)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["optimality_verified"] = bool(d["optimality_verified"])
+        d["finite"] = bool(d["finite"])
+        return d
+
+
+class SolveOptions(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("time_limit", C.c_double), ("iteration_limit", C.c_uint64),
+                ("check_every", C.c_int32), ("reserved", C.c_int32), ("fp32_switch", C.c_double)]
+
+
+class BoxReport(C.Structure):
+    _fields_ = [("status", C.c_int32), ("restarts", C.c_int32), ("iterations", C.c_uint64),
+                ("fp32_iterations", C.c_uint64), ("check", CheckReport), ("solve_seconds", C.c_double),
+                ("step_kernel_ms", C.c_double), ("step_launches", C.c_uint64),
+                ("step32_kernel_ms", C.c_double), ("step32_launches", C.c_uint64), ("checks", C.c_uint64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "check"}
+        d["status"] = STATUS[self.status]
+        d.update(self.check.as_dict())
+        return d
+
+
+class BoxInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("row_blocks", C.c_int64), ("device_bytes", C.c_int64),
+                ("lipschitz", C.c_double)]
+
+
+class Problem(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nterms", C.c_int64),
+                ("hrow", C.POINTER(C.c_int64)), ("hcol", C.POINTER(C.c_int64)), ("hval", C.POINTER(C.c_double)),
+                ("cost", C.POINTER(C.c_double)), ("lower", C.POINTER(C.c_double)), ("upper", C.POINTER(C.c_double)),
+                ("offset", C.c_double), ("maximize", C.c_int32), ("reserved", C.c_int32)]
+
+
+class QpOptions(C.Structure):
+    _fields_ = [("time_limit", C.c_double), ("tolerance", C.c_double), ("iteration_limit", C.c_uint64),
+                ("threads", C.c_int32), ("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32)]
+
+
+class QpResult(C.Structure):
+    _fields_ = [("status", C.c_char * 32), ("algorithm", C.c_char * 32), ("objective", C.c_double),
+                ("primal_residual", C.c_double), ("stationarity_residual", C.c_double),
+                ("complementarity_residual", C.c_double), ("dual_bound", C.c_double), ("relative_gap", C.c_double),
+                ("iterations", C.c_uint64), ("optimality_verified", C.c_int32), ("restarts", C.c_int32),
+                ("fp32_iterations", C.c_uint64), ("solve_seconds", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["status"] = self.status.decode()
+        d["algorithm"] = self.algorithm.decode()
+        d["optimality_verified"] = bool(self.optimality_verified)
+        return d
+
+
+class Planted(C.Structure):
+    _fields_ = [("n", C.c_int64), ("draws", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64),
+                ("lower", C.c_double), ("upper", C.c_double)]
+
+
+P_I32, P_I64, P_F64 = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+
+# Exported symbols and their argument types; tests check that every symbol
+# declared in include/hgqp.h is exported.
+SIGNATURES = {
+    "hgqp_last_error": ([], C.c_char_p),
+    "hgqp_device_name": ([C.c_char_p, C.c_size_t], C.c_int),
+    "hgqp_box_options_default": ([C.POINTER(BoxOptions)], None),
+    "hgqp_solve_options_default": ([C.POINTER(SolveOptions)], None),
+    "hgqp_qp_options_default": ([C.POINTER(QpOptions)], None),
+    "hgqp_box_create": ([C.c_int64, C.c_int64, P_I32, P_I32, P_F64, P_F64, P_F64, P_F64, C.c_double,
+                         C.POINTER(BoxOptions), C.POINTER(C.c_void_p)], C.c_int),
+    "hgqp_box_step": ([C.c_void_p, C.c_double], C.c_int),
+    "hgqp_box_primal": ([C.c_void_p, P_F64, C.c_int64], C.c_int),
+    "hgqp_box_check": ([C.c_void_p, C.c_double, C.POINTER(CheckReport)], C.c_int),
+    "hgqp_box_solve": ([C.c_void_p, C.POINTER(SolveOptions), C.POINTER(BoxReport)], C.c_int),
+    "hgqp_box_reset": ([C.c_void_p], C.c_int),
+    "hgqp_box_destroy": ([C.c_void_p], C.c_int),
+    "hgqp_box_info_get": ([C.c_void_p, C.POINTER(BoxInfo)], C.c_int),
+    "hgqp_solve_qp": ([C.POINTER(Problem), C.POINTER(QpOptions), C.POINTER(QpResult), P_F64], C.c_int),
+    "hgqp_csr_lipschitz": ([C.c_int64, P_I32, P_F64], C.c_double),
+    "hgqp_gen_tridiagonal": ([C.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64], C.c_int),
+    "hgqp_gen_planted": ([C.POINTER(Planted), P_I64, P_I32, P_I32, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64],
+                         C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libhgqp.so once; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise ImportError(f"{LIBRARY} is not built; run `make -C {HERE}` (hipcc, gfx950)")
+        L = C.CDLL(LIBRARY)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _raise(rc):
+    if rc != OK:
+        msg = lib().hgqp_last_error().decode()
+        if rc == INVALID_ARGUMENT:
+            raise InvalidArgument(rc, msg)
+        raise HgqpError(rc, msg)
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    _raise(lib().hgqp_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def box_options(**kw):
+    o = BoxOptions()
+    lib().hgqp_box_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def solve_options(**kw):
+    o = SolveOptions()
+    lib().hgqp_solve_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Box:
+    """A resident box engine (hgqp_box_*). H is the mirrored, sign-normalized CSR."""
+
+    def __init__(self, start, index, value, cost, lower, upper, lipschitz, **options):
+        self._keep = [np.ascontiguousarray(start, dtype=np.int32), np.ascontiguousarray(index, dtype=np.int32),
+                      np.ascontiguousarray(value, dtype=np.float64), np.ascontiguousarray(cost, dtype=np.float64),
+                      np.ascontiguousarray(lower, dtype=np.float64), np.ascontiguousarray(upper, dtype=np.float64)]
+        s, i, v, c, l, u = self._keep
+        self.n = len(c)
+        self.handle = C.c_void_p()
+        o = box_options(**options)
+        _raise(lib().hgqp_box_create(self.n, len(v), _ptr(s, C.c_int32), _ptr(i, C.c_int32), _ptr(v, C.c_double),
+                                     _ptr(c, C.c_double), _ptr(l, C.c_double), _ptr(u, C.c_double), float(lipschitz),
+                                     C.byref(o), C.byref(self.handle)))
+        self._keep = None  # the library copied everything
+
+    def step(self, momentum):
+        _raise(lib().hgqp_box_step(self.handle, float(momentum)))
+
+    def primal(self):
+        x = np.empty(self.n, dtype=np.float64)
+        _raise(lib().hgqp_box_primal(self.handle, _ptr(x, C.c_double), self.n))
+        return x
+
+    def check(self, tolerance=1e-7):
+        r = CheckReport()
+        _raise(lib().hgqp_box_check(self.handle, tolerance, C.byref(r)))
+        return r.as_dict()
+
+    def solve(self, **kw):
+        o = solve_options(**kw)
+        r = BoxReport()
+        _raise(lib().hgqp_box_solve(self.handle, C.byref(o), C.byref(r)))
+        return r.as_dict()
+
+    def reset(self):
+        _raise(lib().hgqp_box_reset(self.handle))
+
+    def info(self):
+        i = BoxInfo()
+        _raise(lib().hgqp_box_info_get(self.handle, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def close(self):
+        if self.handle:
+            _raise(lib().hgqp_box_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def qp_options(**kw):
+    o = QpOptions()
+    lib().hgqp_qp_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def solve_qp(n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False, **options):
+    """hgqp_solve_qp: the whole-model entry (solve_accelerated_box with the HIP engine)."""
+    arrs = [np.ascontiguousarray(hrow, dtype=np.int64), np.ascontiguousarray(hcol, dtype=np.int64),
+            np.ascontiguousarray(hval, dtype=np.float64), np.ascontiguousarray(cost, dtype=np.float64),
+            np.ascontiguousarray(lower, dtype=np.float64), np.ascontiguousarray(upper, dtype=np.float64)]
+    hr, hc, hv, c, l, u = arrs
+    p = Problem(int(n), len(hv), _ptr(hr, C.c_int64), _ptr(hc, C.c_int64), _ptr(hv, C.c_double),
+                _ptr(c, C.c_double), _ptr(l, C.c_double), _ptr(u, C.c_double), float(offset), int(bool(maximize)), 0)
+    o = qp_options(**options)
+    r = QpResult()
+    x = np.zeros(int(n), dtype=np.float64)
+    _raise(lib().hgqp_solve_qp(C.byref(p), C.byref(o), C.byref(r), _ptr(x, C.c_double)))
+    d = r.as_dict()
+    d["primal"] = x
+    return d
+
+
+def gen_tridiagonal(n):
+    """The reference's seeded tridiagonal family; returns (triplets..., optimum, objective)."""
+    m = 2 * n - 1
+    hrow, hcol = np.zeros(m, np.int64), np.zeros(m, np.int64)
+    hval = np.zeros(m)
+    cost, lower, upper, opt = (np.zeros(n) for _ in range(4))
+    f = C.c_double()
+    _raise(lib().hgqp_gen_tridiagonal(n, _ptr(hrow, C.c_int64), _ptr(hcol, C.c_int64), _ptr(hval, C.c_double),
+                                      _ptr(cost, C.c_double), _ptr(lower, C.c_double), _ptr(upper, C.c_double),
+                                      _ptr(opt, C.c_double), C.byref(f)))
+    return dict(n=n, hrow=hrow, hcol=hcol, hval=hval, cost=cost, lower=lower, upper=upper, optimum=opt,
+                objective=f.value)
+
+
+def gen_planted(n, draws, seed, lower=-1.0, upper=1.0):
+    """Planted sparse box QP (full CSR plus the lower-triangle triplets)."""
+    cap = n + 2 * n * draws
+    start = np.zeros(n + 1, np.int32)
+    index = np.zeros(cap, np.int32)
+    value = np.zeros(cap)
+    cost, lo, up, opt = (np.zeros(n) for _ in range(4))
+    nnz, f = C.c_int64(), C.c_double()
+    spec = Planted(n, draws, 0, seed, lower, upper)
+    _raise(lib().hgqp_gen_planted(C.byref(spec), C.byref(nnz), _ptr(start, C.c_int32), _ptr(index, C.c_int32),
+                                  _ptr(value, C.c_double), _ptr(cost, C.c_double), _ptr(lo, C.c_double),
+                                  _ptr(up, C.c_double), _ptr(opt, C.c_double), C.byref(f)))
+    index = index[:nnz.value].copy()
+    value = value[:nnz.value].copy()
+    return dict(n=n, start=start, index=index, value=value, cost=cost, lower=lo, upper=up, optimum=opt,
+                objective=f.value)
+
+
+def lower_triplets(start, index, value):
+    """Lower-triangle triplets (row, col, value), col <= row, of a symmetric CSR."""
+    n = len(start) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(start))
+    keep = index <= rows
+    return rows[keep], index[keep].astype(np.int64), value[keep]
+
+
+def lipschitz(start, value):
+    """hgqp_csr_lipschitz: max_i sum_k |H_ik| summed in stored order (accelerated_qp.cpp:25-26)."""
+    start = np.ascontiguousarray(start, dtype=np.int32)
+    value = np.ascontiguousarray(value, dtype=np.float64)
+    return lib().hgqp_csr_lipschitz(len(start) - 1, _ptr(start, C.c_int32), _ptr(value, C.c_double))

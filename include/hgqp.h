@@ -1,0 +1,262 @@
+/* hgqp — MI355X (gfx950) box-constrained convex QP backend, C ABI.
+ *
+ *   min  1/2 x'Hx + c'x   subject to   lower <= x <= upper
+ *
+ * This is the drop-in boundary for the reference solver's optional GPU engine
+ * (02-software/solver/ of the reference repository). Every entry point below
+ * names the reference interface it replaces. Conventions:
+ *   - every call returns an int status (HGQP_OK = 0); no exception crosses
+ *     the ABI; hgqp_last_error() returns the calling thread's last message;
+ *   - arrays are borrowed for the duration of the call and copied into the
+ *     library's own device memory;
+ *   - indices are int32 as in the reference CsrMatrix (sparse_matrix.hpp:7,14);
+ *   - a handle is not thread-safe; distinct handles may be used from distinct
+ *     threads. Each handle owns one HIP stream.
+ */
+#ifndef HGQP_H
+#define HGQP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HGQP_OK 0
+#define HGQP_INVALID_ARGUMENT 1   /* reference: std::invalid_argument        */
+#define HGQP_NO_DEVICE 2          /* reference: "No CUDA device available"   */
+#define HGQP_OUT_OF_MEMORY 3
+#define HGQP_DEVICE_ERROR 4       /* reference: std::runtime_error("CUDA: ")*/
+#define HGQP_BUSY 5               /* admission refused (budget / slots)      */
+#define HGQP_UNSUPPORTED 6        /* e.g. zero Hessian on the GPU engine     */
+
+/* ---- solver outcome (QpResult::status strings, accelerated_qp.cpp) ----- */
+#define HGQP_STATUS_OPTIMAL 0
+#define HGQP_STATUS_ITERATION_LIMIT 1
+#define HGQP_STATUS_TIME_LIMIT 2
+#define HGQP_STATUS_NUMERICAL_ERROR 3
+#define HGQP_STATUS_INFEASIBLE_BOUNDS 4
+#define HGQP_STATUS_UNSUPPORTED_CURVATURE 5
+#define HGQP_STATUS_RUNNING 6
+
+/* ---- iteration arithmetic ---------------------------------------------- */
+#define HGQP_PRECISION_FP64 0        /* FP64 values and iterates             */
+#define HGQP_PRECISION_FP32_POLISH 1 /* FP32 iterations, FP64 checks, FP64 polish */
+
+/* ---- momentum schedule --------------------------------------------------- */
+#define HGQP_MOMENTUM_REFERENCE 0 /* t-recurrence, no restart (accelerated_qp.cpp:37,47) */
+#define HGQP_MOMENTUM_RESTART 1   /* same recurrence, reset when the step reverses
+                                     the previous direction (gradient restart)   */
+
+/* Reduction order of the sparse product.
+ * ORDERED sums each row in stored column order, one rounding per product and
+ * per add, exactly as CsrMatrix::multiply does (sparse_matrix.hpp:30); with
+ * the FP64 reference schedule the iterates are then bit-identical to the
+ * reference CPU path. TREE lets lanes share a row (faster on long rows). */
+#define HGQP_REDUCE_ORDERED 0
+#define HGQP_REDUCE_TREE 1
+
+typedef struct hgqp_box hgqp_box;
+
+typedef struct hgqp_box_options {
+    int32_t precision;       /* HGQP_PRECISION_*                                  */
+    int32_t momentum;        /* HGQP_MOMENTUM_*                                   */
+    int32_t reduce;          /* HGQP_REDUCE_*                                     */
+    int32_t use_graph;       /* 1: replay check periods as captured HIP graphs    */
+    int32_t timing;          /* 1: HIP events around every step launch            */
+    int32_t maximize;        /* objective sense of the original model             */
+    double offset;           /* objective constant of the original model          */
+    int32_t diagonal_hessian;/* 1 if every off-diagonal triplet is zero (enables the
+                                separable dual bound of qp_admm.cpp:106-112), 0 if
+                                not, -1 derive from the stored matrix              */
+    int32_t reserved;
+} hgqp_box_options;
+
+void hgqp_box_options_default(hgqp_box_options* options);
+
+typedef struct hgqp_check_report {
+    double objective;               /* original sense, qp_admm.cpp:80             */
+    double primal_residual;         /* original_violation, mps.cpp:279-304        */
+    double stationarity_residual;   /* qp_admm.cpp:94-97                          */
+    double complementarity_residual;/* 0 for a box QP                             */
+    double dual_bound;              /* minimization sense, qp_admm.cpp:100-113    */
+    double relative_gap;            /* qp_admm.cpp:114                            */
+    int32_t optimality_verified;    /* qp_admm.cpp:115-116                        */
+    int32_t finite;                 /* objective and stationarity finite          */
+} hgqp_check_report;
+
+typedef struct hgqp_solve_options {
+    double tolerance;        /* QpOptions::tolerance, default 1e-7 (qp.hpp:25)    */
+    double time_limit;       /* seconds, default 30                               */
+    uint64_t iteration_limit;/* default 200000                                    */
+    int32_t check_every;     /* check cadence, 25 in the reference (accelerated_qp.cpp:40) */
+    int32_t reserved;
+    double fp32_switch;      /* FP32_POLISH: switch to FP64 once the FP32 check's
+                                stationarity falls below this (0 = 10*tolerance)  */
+} hgqp_solve_options;
+
+void hgqp_solve_options_default(hgqp_solve_options* options);
+
+typedef struct hgqp_box_report {
+    int32_t status;                 /* HGQP_STATUS_*                              */
+    int32_t restarts;               /* momentum resets (MOMENTUM_RESTART)          */
+    uint64_t iterations;            /* at the deciding check (QpResult::iterations)*/
+    uint64_t fp32_iterations;       /* of which ran in FP32                        */
+    hgqp_check_report check;        /* the deciding check                          */
+    double solve_seconds;           /* host wall time inside hgqp_box_solve        */
+    double step_kernel_ms;          /* summed HIP-event time of FP64 step launches
+                                       that did work (timing=1)                    */
+    uint64_t step_launches;         /* FP64 step launches counted                  */
+    double step32_kernel_ms;        /* same for FP32 step launches                 */
+    uint64_t step32_launches;
+    uint64_t checks;                /* checks evaluated                            */
+} hgqp_box_report;
+
+/* ---- device ------------------------------------------------------------- */
+
+/* Replaces cuda_device_name() (accelerated_qp.hpp:16; cuda_box.cu:38). */
+int hgqp_device_name(char* buffer, size_t length);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* hgqp_last_error(void);
+
+/* ---- resident box engine ------------------------------------------------ */
+
+/* Replaces make_cuda_box(const CsrMatrix&, cost, lower, upper, lipschitz)
+ * (accelerated_qp.hpp:14-15; cuda_box.cu:27-28,34-37). H is the mirrored,
+ * merged, sign-normalized CSR (start[n+1], index[nnz], value[nnz]); cost is
+ * sign-normalized (accelerated_qp.cpp:20-24). x and the extrapolated point
+ * start at clip(0, lower, upper) (cuda_box.cu:26). */
+int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
+                    const double* cost, const double* lower, const double* upper, double lipschitz,
+                    const hgqp_box_options* options, hgqp_box** out);
+
+/* Replaces BoxIteration::step(double momentum) (accelerated_qp.hpp:11;
+ * cuda_box.cu:29): one projected gradient step from the extrapolated point and
+ * the extrapolation with the given momentum. Asynchronous. */
+int hgqp_box_step(hgqp_box* box, double momentum);
+
+/* Replaces BoxIteration::primal() (accelerated_qp.hpp:12; cuda_box.cu:31):
+ * copies x (length n) to the host; synchronizes the handle's stream. */
+int hgqp_box_primal(hgqp_box* box, double* x, int64_t n);
+
+/* FP64 restatement of check_qp_solution (qp_admm.cpp:63-117) evaluated on the
+ * device for the current x. Synchronizes. */
+int hgqp_box_check(hgqp_box* box, double tolerance, hgqp_check_report* report);
+
+/* Device-resident form of the solve loop of solve_accelerated_box
+ * (accelerated_qp.cpp:37-59): the check runs on the device at iteration 0,
+ * every check_every iterations and at the limit; the loop stops at the first
+ * verified check, a non-finite check, the iteration limit or the time limit.
+ * x stays on the device; read it with hgqp_box_primal. */
+int hgqp_box_solve(hgqp_box* box, const hgqp_solve_options* options, hgqp_box_report* report);
+
+/* Resets x and the extrapolated point to clip(0, lower, upper) and the
+ * momentum state, so the same handle can solve again. */
+int hgqp_box_reset(hgqp_box* box);
+
+/* Replaces the destructor of CudaBox (cuda_box.cu:13). */
+int hgqp_box_destroy(hgqp_box* box);
+
+/* Static facts of a handle: rows, stored entries, row blocks, device bytes. */
+typedef struct hgqp_box_info {
+    int64_t n, nnz, row_blocks;
+    int64_t device_bytes;
+    double lipschitz;
+} hgqp_box_info;
+int hgqp_box_info_get(const hgqp_box* box, hgqp_box_info* info);
+
+/* ---- whole-model entry -------------------------------------------------- */
+
+/* A box QP in the reference's model form (qp.hpp:6-10, mps.hpp:8-21): Hessian
+ * triplets hold one lower-triangle entry per position with the full value. */
+typedef struct hgqp_problem {
+    int64_t n;
+    int64_t nterms;
+    const int64_t* hrow;
+    const int64_t* hcol;
+    const double* hval;
+    const double* cost;
+    const double* lower;
+    const double* upper;
+    double offset;
+    int32_t maximize;
+    int32_t reserved;
+} hgqp_problem;
+
+typedef struct hgqp_qp_options {
+    double time_limit;        /* QpOptions (qp.hpp:24-29)                        */
+    double tolerance;
+    uint64_t iteration_limit;
+    int32_t threads;          /* validated as in accelerated_qp.cpp:17            */
+    int32_t precision;        /* HGQP_PRECISION_*                                 */
+    int32_t momentum;         /* HGQP_MOMENTUM_*                                  */
+    int32_t reduce;           /* HGQP_REDUCE_*                                    */
+} hgqp_qp_options;
+
+void hgqp_qp_options_default(hgqp_qp_options* options);
+
+typedef struct hgqp_qp_result {
+    char status[32];          /* QpResult::status string                          */
+    char algorithm[32];       /* "hip_fista_box", "hip_fista_box_fp32_polish"      */
+    double objective;
+    double primal_residual;
+    double stationarity_residual;
+    double complementarity_residual;
+    double dual_bound;
+    double relative_gap;
+    uint64_t iterations;
+    int32_t optimality_verified;
+    int32_t restarts;
+    uint64_t fp32_iterations;
+    double solve_seconds;
+} hgqp_qp_result;
+
+/* Replaces solve_accelerated_box(q, options, cuda=true)
+ * (accelerated_qp.hpp:17; accelerated_qp.cpp:13-61) with the HIP engine.
+ * primal_out (length n) may be NULL. Invalid models and options return
+ * HGQP_INVALID_ARGUMENT as the reference throws std::invalid_argument; a zero
+ * Hessian returns HGQP_UNSUPPORTED (accelerated_qp.cpp:28). */
+int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, hgqp_qp_result* result,
+                  double* primal_out);
+
+/* L = max over rows of the sum of |value| in stored order
+ * (accelerated_qp.cpp:25-26), for callers that assemble the CSR themselves. */
+double hgqp_csr_lipschitz(int64_t n, const int32_t* start, const double* value);
+
+/* ---- planted instance generators (inputs for tests and benchmarks) ------ */
+
+/* The reference's seeded tridiagonal family (scale_benchmark.cpp:13-18):
+ * 2n-1 triplets, bounds [0,3], planted optimum; returns the planted objective
+ * (long double sum rounded to double) through *objective. */
+int hgqp_gen_tridiagonal(int64_t n, int64_t* hrow, int64_t* hcol, double* hval, double* cost, double* lower,
+                         double* upper, double* optimum, double* objective);
+
+/* Planted sparse box QP: each row draws `draws` distinct off-diagonal columns
+ * with N(0, 0.1^2) values, symmetrized by union (both draws of a pair summed);
+ * diagonal = sum of |off-diagonal| in the row + U(0.1, 1); x* has 30% of the
+ * variables at lower, 30% at upper, 40% interior; c = g* - H x* with g* > 0 at
+ * lower, < 0 at upper, 0 inside. Bounds are the scalars lower/upper.
+ * index and value must hold n + 2*n*draws entries (an upper bound); *nnz
+ * receives the stored count. The CSR holds both triangles with ascending
+ * columns and no zeros, i.e. exactly what the reference assembles
+ * (accelerated_qp.cpp:21-24) from the lower-triangle triplets (row, col,
+ * value) of the entries with col <= row. Every row draws from its own stream
+ * seeded from (seed, row), so the instance does not depend on thread count. */
+typedef struct hgqp_planted {
+    int64_t n;
+    int32_t draws;          /* off-diagonal draws per row                          */
+    int32_t reserved;
+    uint64_t seed;
+    double lower, upper;    /* scalar box                                          */
+} hgqp_planted;
+
+int hgqp_gen_planted(const hgqp_planted* spec, int64_t* nnz, int32_t* start, int32_t* index, double* value,
+                     double* cost, double* lower, double* upper, double* optimum, double* objective);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HGQP_H */

@@ -1,0 +1,123 @@
+"""HIP engine against the oracle (CPU restatement of the reference path).
+
+Bit-exact where the arithmetic is the reference's (ORDERED reduction, FP64,
+reference momentum); within the stated tolerances otherwise. Every call goes
+through the C ABI of libhgqp.so.
+"""
+import numpy as np
+import pytest
+
+from helpers import planted_qp, random_csr_qp, tridiagonal_qp
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances of the box-QP contract: KKT (stationarity) <= 1e-6 and objective
+# within 1e-8 relative of an FP64 CPU solve.
+KKT_TOL = 1e-6
+OBJ_REL = 1e-8
+
+
+def engine_inputs(oracle, q):
+    """The sign-normalized mirrored CSR, c and L exactly as the reference assembles them."""
+    start, index, value = oracle.build_csr(q)
+    sign = -1.0 if q.maximize else 1.0
+    return start, index, value, sign * q.cost, q.lower, q.upper, oracle.lipschitz(start, value)
+
+
+def cases(hg):
+    return {
+        "tridiagonal": tridiagonal_qp(hg, 2000)[0],
+        "planted": planted_qp(hg, 3000, 8, 11)[0],
+        "long_rows": random_csr_qp(1500, 5, long_rows=(3, 700, 1499), density=0.004, empty_rows=(10, 11)),
+    }
+
+
+def test_device_is_gfx950(hg):
+    assert "gfx950" in hg.device_name()
+
+
+@pytest.mark.parametrize("case", ["tridiagonal", "planted", "long_rows"])
+def test_steps_bit_exact(hg, oracle, case):
+    q = cases(hg)[case]
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    betas = oracle.betas(40)
+    x0 = np.clip(np.zeros(q.n), lo, up)
+    x_cpu, _ = oracle.fista_steps(s, i, v, c, lo, up, L, betas, x0, x0)
+    print(f"case={case} n={q.n} nnz={len(v)}", flush=True)
+    with hg.Box(s, i, v, c, lo, up, L) as box:
+        for b in betas:
+            box.step(b)
+        x_gpu = box.primal()
+    assert np.array_equal(x_gpu, x_cpu), np.abs(x_gpu - x_cpu).max()
+    with hg.Box(s, i, v, c, lo, up, L, reduce=hg.REDUCE_TREE) as box:
+        for b in betas:
+            box.step(b)
+        x_tree = box.primal()
+    assert np.abs(x_tree - x_cpu).max() <= 1e-12 * (1 + np.abs(x_cpu).max())
+
+
+@pytest.mark.parametrize("case", ["tridiagonal", "planted", "long_rows"])
+def test_check_matches_oracle(hg, oracle, case):
+    q = cases(hg)[case]
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    betas = oracle.betas(30)
+    with hg.Box(s, i, v, c, lo, up, L) as box:
+        for b in betas:
+            box.step(b)
+        x = box.primal()
+        dev = box.check(1e-7)
+    ref = oracle.check(q, x, 1e-7)
+    print(case, dev, ref, flush=True)
+    assert dev["optimality_verified"] == ref["optimality_verified"]
+    assert abs(dev["objective"] - ref["objective"]) <= 1e-13 * (1 + abs(ref["objective"]))
+    assert abs(dev["stationarity_residual"] - ref["stationarity_residual"]) <= 1e-12 * (1 + ref["stationarity_residual"])
+    assert dev["primal_residual"] == ref["primal_residual"] == 0.0
+    assert abs(dev["dual_bound"] - ref["dual_bound"]) <= 1e-12 * (1 + abs(ref["dual_bound"]))
+    assert abs(dev["relative_gap"] - ref["relative_gap"]) <= 1e-12
+
+
+def test_resident_solve_bit_exact(hg, oracle):
+    """Device-resident loop with the reference schedule equals the CPU path bit for bit."""
+    q, d = planted_qp(hg, 10000, 8, 1)
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    ref = oracle.solve_box(q, tolerance=1e-7, iteration_limit=20000, time_limit=120)
+    for graph in (1, 0):
+        with hg.Box(s, i, v, c, lo, up, L, use_graph=graph) as box:
+            rep = box.solve(tolerance=1e-7, iteration_limit=20000, time_limit=120)
+            x = box.primal()
+        print(rep, flush=True)
+        assert rep["status"] == ref["status"] == "optimal"
+        assert rep["iterations"] == ref["iterations"]
+        assert np.array_equal(x, ref["primal"])
+        assert abs(rep["objective"] - ref["objective"]) <= 1e-13 * (1 + abs(ref["objective"]))
+
+
+def test_tridiagonal_golden(hg):
+    """The reference's recorded run (kaggle-gpu-03 results.json, n=10000): 1700 iterations,
+    objective -947.1248894454228, stationarity 7.093582844702874e-08, max error 5.3349e-06."""
+    d = hg.gen_tridiagonal(10000)
+    r = hg.solve_qp(10000, d["hrow"], d["hcol"], d["hval"], d["cost"], d["lower"], d["upper"],
+                    tolerance=1e-7, iteration_limit=20000, time_limit=120)
+    print(r["status"], r["iterations"], repr(r["objective"]), r["stationarity_residual"], flush=True)
+    assert r["status"] == "optimal" and r["optimality_verified"]
+    assert r["iterations"] == 1700
+    assert abs(r["objective"] - -947.1248894454228) <= 1e-13 * 947
+    assert abs(r["stationarity_residual"] - 7.093582844702874e-08) <= 1e-12
+    assert abs(np.abs(r["primal"] - d["optimum"]).max() - 5.334931689526812e-06) <= 1e-12
+
+
+@pytest.mark.parametrize("momentum,precision", [(1, 0), (1, 1), (0, 1)])
+def test_fast_modes_meet_contract(hg, oracle, momentum, precision):
+    """Restart and FP32-with-FP64-polish: KKT <= 1e-6, objective within 1e-8 of the FP64 oracle."""
+    q, d = planted_qp(hg, 10000, 8, 3)
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    ref = oracle.solve_box(q, tolerance=1e-9, iteration_limit=20000, time_limit=120)
+    with hg.Box(s, i, v, c, lo, up, L, momentum=momentum, precision=precision, reduce=hg.REDUCE_TREE) as box:
+        rep = box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=120)
+        x = box.primal()
+    print(rep, flush=True)
+    assert rep["status"] == "optimal"
+    chk = oracle.check(q, x, KKT_TOL)
+    assert chk["optimality_verified"] and chk["stationarity_residual"] <= KKT_TOL
+    assert abs(chk["objective"] - ref["objective"]) <= OBJ_REL * (1 + abs(ref["objective"]))
+    assert np.abs(x - d["optimum"]).max() < 1e-4

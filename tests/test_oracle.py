@@ -1,0 +1,255 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+
+Each function of oracle/boxqp_oracle.cpp is fixed three ways where possible:
+  1. against the compiled reference sources (oracle/_ref), bit for bit;
+  2. against results the reference recorded (tests/golden/reference_records.json);
+  3. against closed forms and brute force, chosen so that a dropped term, a
+     wrong sign or a transposed operand fails.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import BoxQP, OracleError
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def records():
+    with open(os.path.join(GOLDEN, "reference_records.json")) as f:
+        return json.load(f)
+
+
+def two_var(maximize=False):
+    """test_solver.cpp:85-89: H = diag(2,2), c = (-2,-4), x in [-10,10]^2; optimum (1,2), objective -5."""
+    s = -1.0 if maximize else 1.0
+    return BoxQP(2, [0, 1], [0, 1], [2 * s, 2 * s], [-2 * s, -4 * s], [-10, -10], [10, 10], maximize=maximize)
+
+
+def block_psd(n=300):
+    """test_v2.cpp:18-24: 2x2 blocks [[1,2],[2,5]] (PSD, not diagonally dominant), c = -1, x in [0,10]."""
+    hr, hc, hv = [], [], []
+    for j in range(n):
+        hr.append(j); hc.append(j); hv.append(5.0 if j % 2 else 1.0)
+        if j % 2:
+            hr.append(j); hc.append(j - 1); hv.append(2.0)
+    return BoxQP(n, hr, hc, hv, -np.ones(n), np.zeros(n), 10 * np.ones(n))
+
+
+def random_triplets(seed, n=40, m=120):
+    rng = np.random.default_rng(seed)
+    r = rng.integers(0, n, m)
+    c = rng.integers(0, n, m)
+    hr, hc = np.maximum(r, c), np.minimum(r, c)
+    hv = rng.normal(size=m)
+    hv[::7] = 0.0                      # explicit zeros are dropped
+    hr = np.concatenate([hr, hr[:10]])  # duplicates are summed
+    hc = np.concatenate([hc, hc[:10]])
+    hv = np.concatenate([hv, -hv[:10] * 0.5])
+    return BoxQP(n, hr, hc, hv, rng.normal(size=n), -np.ones(n), np.ones(n))
+
+
+def dense_of(q):
+    sign = -1.0 if q.maximize else 1.0
+    H = np.zeros((q.n, q.n))
+    for r, c, v in zip(q.hrow, q.hcol, q.hval):
+        H[r, c] += sign * v
+        if r != c:
+            H[c, r] += sign * v
+    return H
+
+
+# ---- 1. the compiled reference ---------------------------------------------------
+
+def test_build_csr_matches_reference(oracle, reference):
+    for seed in range(5):
+        q = random_triplets(seed)
+        a, b = oracle.build_csr(q), reference.build_csr(q)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_solve_matches_reference(oracle, reference):
+    from oracle import Oracle  # noqa: F401
+    probs = [two_var(), two_var(True), block_psd(), random_triplets(3), oracle.gen_tridiagonal(1000)[0]]
+    for q in probs:
+        a = oracle.solve_box(q, tolerance=1e-7, iteration_limit=5000, time_limit=60)
+        b = reference.solve_box(q, tolerance=1e-7, iteration_limit=5000, time_limit=60)
+        for k in ("status", "iterations", "objective", "stationarity_residual", "dual_bound", "relative_gap",
+                  "optimality_verified", "primal_residual"):
+            assert a[k] == b[k] or (isinstance(a[k], float) and math.isnan(a[k]) and math.isnan(b[k])), k
+        assert np.array_equal(a["primal"], b["primal"])
+
+
+def test_check_matches_reference(oracle, reference):
+    rng = np.random.default_rng(7)
+    for q in (two_var(), two_var(True), block_psd(40), random_triplets(1)):
+        for x in (rng.uniform(-1, 1, q.n), np.zeros(q.n), np.full(q.n, 20.0)):
+            a, b = oracle.check(q, x, 1e-7), reference.check(q, x, 1e-7)
+            for k in ("objective", "primal_residual", "stationarity_residual", "dual_bound", "relative_gap",
+                      "optimality_verified"):
+                assert a[k] == b[k], (k, a[k], b[k])
+        bad = np.zeros(q.n)
+        bad[0] = np.nan
+        assert not oracle.check(q, bad)["optimality_verified"]
+        assert not reference.check(q, bad)["optimality_verified"]
+
+
+def test_convexity_matches_reference(oracle, reference):
+    q = block_psd()
+    assert oracle.check_convex(q) and reference.check_convex(q)
+    q.hval[0] = -1.0  # test_v2.cpp:25
+    assert not oracle.check_convex(q) and not reference.check_convex(q)
+    q = BoxQP(2, [0, 1, 1], [0, 1, 0], [1.0, 1.0, 1.5], [0, 0], [-1, -1], [1, 1])  # indefinite 2x2
+    assert not oracle.check_convex(q) and not reference.check_convex(q)
+
+
+def test_invalid_input_matches_reference(oracle, reference):
+    q = two_var()
+    for kind in (oracle, reference):
+        with pytest.raises(OracleError) as e:
+            kind.solve_box(q, threads=0)  # test_v2.cpp:59
+        assert e.value.code == 1
+        bad = two_var()
+        bad.cost[0] = np.nan
+        with pytest.raises(OracleError):
+            kind.solve_box(bad)
+    inf = two_var()
+    inf.lower[1], inf.upper[1] = 3.0, 2.0
+    assert oracle.solve_box(inf)["status"] == reference.solve_box(inf)["status"] == "infeasible_bounds"
+
+
+# ---- 2. the reference's recorded results --------------------------------------------
+
+@pytest.mark.parametrize("n", [10000, 50000])
+def test_tridiagonal_recorded_runs(oracle, n):
+    rec = [r for r in records()["tridiagonal"] if r["columns"] == n and r["backend"] == "cpu" and r["threads"] == 1][0]
+    q, opt, known = oracle.gen_tridiagonal(n)
+    assert known == rec["known_objective"]
+    r = oracle.solve_box(q, tolerance=1e-7, iteration_limit=20000, time_limit=120)
+    assert r["status"] == "optimal" and r["iterations"] == rec["iterations"] == 1700
+    assert r["objective"] == rec["objective"]
+    assert r["stationarity_residual"] == rec["stationarity_residual"]
+    assert np.abs(r["primal"] - opt).max() == rec["max_solution_error"]
+
+
+def test_tridiagonal_threads_do_not_change_result(oracle):
+    q, _, _ = oracle.gen_tridiagonal(20000)
+    a = oracle.solve_box(q, iteration_limit=400, threads=1)
+    b = oracle.solve_box(q, iteration_limit=400, threads=4)
+    assert np.array_equal(a["primal"], b["primal"]) and a["objective"] == b["objective"]
+
+
+# ---- 3. closed forms and brute force ---------------------------------------------
+
+def test_two_variable_closed_form(oracle):
+    r = oracle.solve_box(two_var())
+    assert r["status"] == "optimal" and abs(r["objective"] + 5) < 1e-7
+    assert np.allclose(r["primal"], [1, 2], atol=1e-6)
+    c = oracle.check(two_var(), np.array([1.0, 2.0]))
+    assert c["objective"] == -5 and c["stationarity_residual"] == 0 and c["dual_bound"] == -5
+    assert c["relative_gap"] == 0 and c["optimality_verified"] and c["positive_diagonal"]
+    # x = 0: gradient (-2,-4) gives stationarity 4; separable bound -5; gap 5 / 1
+    c = oracle.check(two_var(), np.zeros(2))
+    assert c["stationarity_residual"] == 4 and c["dual_bound"] == -5 and c["relative_gap"] == 5
+    # maximization reports the original sense and the bound in minimization sense
+    c = oracle.check(two_var(True), np.array([1.0, 2.0]))
+    assert c["objective"] == 5 and c["dual_bound"] == -5 and c["optimality_verified"]
+
+
+def test_block_psd_closed_form(oracle):
+    """Per block the optimum is (1, 0) (y at its bound with gradient 1), objective -1/2."""
+    r = oracle.solve_box(block_psd(), time_limit=10)
+    assert r["status"] == "optimal" and r["optimality_verified"]
+    assert abs(r["objective"] + 75) < 1e-6
+    expect = np.tile([1.0, 0.0], 150)
+    assert np.abs(r["primal"] - expect).max() < 1e-6
+
+
+def test_tiny_curvature_needs_the_gap(oracle):
+    """H = diag(1e-16, 1): the projected gradient at (1e8, 0) is tiny, but the
+    separable bound is 0 while the value is 1/2, so the point is not verified
+    (the box analogue of test_v2.cpp:53-56 and test_qp_certificate.py:44-49)."""
+    q = BoxQP(2, [0, 1], [0, 1], [1e-16, 1.0], [0, 0], [-1e12, -1], [1e12, 1])
+    c = oracle.check(q, np.array([1e8, 0.0]), 1e-7)
+    assert c["stationarity_residual"] < 1e-7
+    assert c["dual_bound"] == 0 and abs(c["objective"] - 0.5) < 1e-15
+    assert c["relative_gap"] > 0.1 and not c["optimality_verified"]
+
+
+def test_tangent_bound_closed_form(oracle):
+    """Non-diagonal H: bound = value + sum_j g_j (target_j - x_j), target = lower if g > 0 else upper;
+    an infinite target makes the bound -inf."""
+    q = BoxQP(2, [0, 1, 1], [0, 1, 0], [2.0, 2.0, 0.5], [1.0, -1.0], [-1, -2], [3, 4])
+    x = np.array([0.5, 0.25])
+    H = dense_of(q)
+    g = H @ x + q.cost
+    value = 0.5 * x @ H @ x + q.cost @ x
+    target = np.where(g > 0, q.lower, q.upper)
+    c = oracle.check(q, x)
+    assert abs(c["objective"] - value) < 1e-15
+    assert abs(c["dual_bound"] - (value + g @ (target - x))) < 1e-14
+    stat = np.max(np.abs(x - np.clip(x - g, q.lower, q.upper)) / (1 + np.abs(x)))
+    assert abs(c["stationarity_residual"] - stat) < 1e-16
+    q.upper[1] = np.inf
+    if g[1] < 0:
+        assert oracle.check(q, x)["dual_bound"] == -np.inf
+
+
+def test_scaled_violation_closed_form(oracle):
+    q = two_var()
+    c = oracle.check(q, np.array([11.0, -12.0]))
+    assert abs(c["primal_residual"] - max(1 / (1 + 10 + 11), 2 / (1 + 10 + 12))) < 1e-16
+    assert not c["optimality_verified"]
+
+
+def test_csr_and_spmv_brute_force(oracle):
+    for seed in range(4):
+        q = random_triplets(seed)
+        start, index, value = oracle.build_csr(q)
+        dense = np.zeros((q.n, q.n))
+        for i in range(q.n):
+            assert np.all(np.diff(index[start[i]:start[i + 1]]) > 0)  # sorted, unique
+            dense[i, index[start[i]:start[i + 1]]] = value[start[i]:start[i + 1]]
+        assert np.allclose(dense, dense_of(q), rtol=0, atol=1e-14)
+        assert np.all(value != 0)
+        x = np.random.default_rng(seed).normal(size=q.n)
+        assert np.allclose(oracle.spmv(start, index, value, x), dense_of(q) @ x, rtol=1e-13, atol=1e-13)
+        assert oracle.lipschitz(start, value) == pytest.approx(np.abs(dense).sum(axis=1).max(), rel=1e-14)
+
+
+def test_momentum_closed_form(oracle):
+    b = oracle.betas(4)
+    t1 = (1 + math.sqrt(5)) / 2
+    t2 = (1 + math.sqrt(1 + 4 * t1 * t1)) / 2
+    assert b[0] == 0.0 and b[1] == (t1 - 1) / t2
+    assert np.all(np.diff(oracle.betas(200)) > 0) and oracle.betas(200)[-1] < 1
+
+
+def test_fista_step_textbook(oracle):
+    """One step from w: x+ = P(w - (Hw + c)/L), w+ = x+ + beta (x+ - x) (Beck & Teboulle 2009)."""
+    q = random_triplets(2)
+    s, i, v = oracle.build_csr(q)
+    L = oracle.lipschitz(s, v)
+    H = dense_of(q)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, q.n)
+    w = rng.uniform(-1, 1, q.n)
+    xn, wn = oracle.fista_steps(s, i, v, q.cost, q.lower, q.upper, L, np.array([0.3]), x, w)
+    expect = np.clip(w - (H @ w + q.cost) / L, q.lower, q.upper)
+    assert np.allclose(xn, expect, atol=1e-14)
+    assert np.allclose(wn, expect + 0.3 * (expect - x), atol=1e-14)
+
+
+def test_planted_optimum_is_stationary(oracle, hg):
+    """The generator's x* satisfies the KKT conditions of its model and its objective."""
+    from helpers import planted_qp
+    q, d = planted_qp(hg, 3000, 8, 5)
+    c = oracle.check(q, d["optimum"], 1e-12)
+    assert c["stationarity_residual"] < 1e-14 and c["primal_residual"] == 0
+    assert abs(c["objective"] - d["objective"]) <= 1e-13 * (1 + abs(d["objective"]))
+    r = oracle.solve_box(q, tolerance=1e-9, iteration_limit=5000)
+    assert r["status"] == "optimal" and np.abs(r["primal"] - d["optimum"]).max() < 1e-6
