@@ -14,8 +14,10 @@
 //            or the residual / objective / dual-bound terms of
 //            check_qp_solution (qp_admm.cpp:63-117) for a check launch.
 // The gradient therefore never reaches HBM. Grid-wide reductions (restart test,
-// check report) write one partial per workgroup; the last workgroup to arrive
-// combines the partials in workgroup order, so results are deterministic.
+// check report) write one partial per workgroup; a one-workgroup finalize
+// launch combines them in workgroup order after the kernel boundary, so results
+// are deterministic and the hot kernels carry no device-scope fence (on a
+// multi-XCD part such a fence writes back the XCD's L2 for every workgroup).
 //
 // This is the CSR-stream scheme of Greathouse & Daga ("Efficient sparse
 // matrix-vector multiplication on GPUs using the CSR storage format", SC 2014),
@@ -122,7 +124,16 @@ __device__ inline void row_block(const Matrix<T>& m, const T* __restrict__ gat, 
             for (int r = r0 + lane; r < r1; r += kWave) {
                 const int a = m.start[r] - k0, e = m.start[r + 1] - k0;
                 T s = 0;
-                for (int q = a; q < e; ++q) s += lds[q];
+                int q = a;
+                // eight LDS reads in flight, then the adds in stored order
+                for (; q + 8 <= e; q += 8) {
+                    T v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = lds[q + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) s += v[u];
+                }
+                for (; q < e; ++q) s += lds[q];
                 epi(r, s);
             }
         } else {
@@ -208,7 +219,6 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
                                                       double* partials, int parity, double beta_imm) {
     __shared__ T stage[kWavesPerGroup][kCap];
     __shared__ double wave_dot[kWavesPerGroup];
-    __shared__ int last;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     double beta = beta_imm, next_t = 0;
     int use_x = 0;
@@ -237,6 +247,7 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
             ctrl->use_x[parity ^ 1] = 0;
         }
     } else if (MOM == kMomentumRestart) {
+        // one partial per workgroup; finalize_restart combines them after this launch
         double d = wave_sum(epi.dot);
         if (lane == 0) wave_dot[wave] = d;
         __syncthreads();
@@ -244,28 +255,45 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
             double s = 0;
             for (int w = 0; w < kWavesPerGroup; ++w) s += wave_dot[w];
             partials[blockIdx.x] = s;
-            __threadfence();
-            last = atomicAdd(&ctrl->arrive, 1u) == gridDim.x - 1;
         }
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            double s = 0;
-            for (int g = threadIdx.x; g < (int)gridDim.x; g += kGroup) s += __builtin_nontemporal_load(partials + g);
-            s = wave_sum(s);
-            if (lane == 0) wave_dot[wave] = s;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double tot = 0;
-                for (int w = 0; w < kWavesPerGroup; ++w) tot += wave_dot[w];
-                const int restart = tot > 0;
-                ctrl->t[parity ^ 1] = restart ? 1.0 : next_t;
-                ctrl->use_x[parity ^ 1] = restart;
-                ctrl->restarts += restart;
-                ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
-                ctrl->arrive = 0;
-            }
+    }
+}
+
+// Restart decision of a step launch: the sum of the step's workgroup partials in
+// workgroup order (deterministic), then the t / extrapolation state of the next
+// step. One workgroup; runs after the step launch on the same stream.
+constexpr int kFinal = 1024;  // threads of a finalize launch
+
+__global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const double* partials, int groups,
+                                                           int parity) {
+    __shared__ double wave_dot[kFinal / kWave];
+    if (ctrl->done) return;  // the step returned early
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // thread t owns partials t, t + kFinal, ...; 8 loads in flight per batch
+    double s = 0;
+    for (int g0 = threadIdx.x; g0 < groups; g0 += 8 * kFinal) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int g = g0 + u * kFinal;
+            v[u] = g < groups ? partials[g] : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    s = wave_sum(s);
+    if (lane == 0) wave_dot[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0;
+        for (int w = 0; w < kFinal / kWave; ++w) tot += wave_dot[w];
+        double next_t, beta;
+        momentum_of(ctrl->t[parity], &next_t, &beta);
+        const int restart = tot > 0;
+        ctrl->t[parity ^ 1] = restart ? 1.0 : next_t;
+        ctrl->use_x[parity ^ 1] = restart;
+        ctrl->restarts += restart;
+        ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
     }
 }
 
@@ -278,6 +306,7 @@ struct CheckEpilogue {
     const int* __restrict__ start;
     const int* __restrict__ col;
     const double* __restrict__ val;
+    double* __restrict__ gout;  // optional: the FP64 gradient Hx + c per row
     int diagonal;
     double stat, viol;
     DD value, tangent, separable;
@@ -293,6 +322,7 @@ struct CheckEpilogue {
         value = dd_add(value, dd_add(t, dd_from(cv * xv)));
         // stationarity (qp_admm.cpp:94-97)
         const double g = hx + cv;
+        if (gout) gout[r] = g;
         double res = fabs(xv - clip(xv - g, l, u)) / (1 + fabs(xv));
         if (!isfinite(g) || !isfinite(res)) res = INFINITY;
         stat = fmax(stat, res);
@@ -324,7 +354,6 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
                                                        double* partials, int parity, CheckParams prm) {
     __shared__ double stage[kWavesPerGroup][kCap];
     __shared__ double red[kWavesPerGroup][kPartial];
-    __shared__ int last;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (prm.set_done && ctrl->done) return;  // grid-uniform
     CheckEpilogue epi;
@@ -336,6 +365,7 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
     epi.col = m.col;
     epi.val = m.val;
     epi.diagonal = prm.diagonal;
+    epi.gout = prm.gout;
     epi.stat = 0;
     epi.viol = 0;
     epi.value = epi.tangent = epi.separable = DD{0, 0};
@@ -367,24 +397,37 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
             acc[8] = double(int(acc[8]) | int(red[w][8]));
         }
         for (int q = 0; q < kPartial; ++q) partials[blockIdx.x * kPartial + q] = acc[q];
-        __threadfence();
-        last = atomicAdd(&ctrl->check_arrive, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // last workgroup: fixed-order combination of all partials
+}
+
+// Report of a check launch: the workgroup partials combined in workgroup order,
+// then the verdict rules of check_qp_solution (qp_admm.cpp:79-116). One
+// workgroup; runs after the check launch on the same stream.
+__global__ __launch_bounds__(kFinal) void finalize_check(Control* ctrl, const double* partials, int groups, int parity,
+                                                         CheckParams prm) {
+    __shared__ double red[kFinal / kWave][kPartial];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (prm.set_done && ctrl->done) return;  // the check returned early
     double stat = 0, viol = 0;
     DD v{0, 0}, tg{0, 0}, sp{0, 0};
     int flags = 0;
-    for (int g = threadIdx.x; g < (int)gridDim.x; g += kGroup) {
-        const double* p = partials + g * kPartial;
-        stat = fmax(stat, __builtin_nontemporal_load(p + 0));
-        viol = fmax(viol, __builtin_nontemporal_load(p + 1));
-        v = dd_add(v, DD{__builtin_nontemporal_load(p + 2), __builtin_nontemporal_load(p + 3)});
-        tg = dd_add(tg, DD{__builtin_nontemporal_load(p + 4), __builtin_nontemporal_load(p + 5)});
-        sp = dd_add(sp, DD{__builtin_nontemporal_load(p + 6), __builtin_nontemporal_load(p + 7)});
-        flags |= int(__builtin_nontemporal_load(p + 8));
+    for (int g0 = threadIdx.x; g0 < groups; g0 += 4 * kFinal) {
+        double q[4][9];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int g = g0 + u * kFinal;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) q[u][k] = g < groups ? partials[g * kPartial + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            stat = fmax(stat, q[u][0]);
+            viol = fmax(viol, q[u][1]);
+            v = dd_add(v, DD{q[u][2], q[u][3]});
+            tg = dd_add(tg, DD{q[u][4], q[u][5]});
+            sp = dd_add(sp, DD{q[u][6], q[u][7]});
+            flags |= int(q[u][8]);
+        }
     }
     stat = wave_max(stat);
     viol = wave_max(viol);
@@ -392,7 +435,6 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
     tg = wave_dd(tg);
     sp = wave_dd(sp);
     flags = wave_or(flags);
-    __syncthreads();
     if (lane == 0) {
         double* o = red[wave];
         o[0] = stat; o[1] = viol; o[2] = v.hi; o[3] = v.lo; o[4] = tg.hi; o[5] = tg.lo;
@@ -400,7 +442,7 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int w = 1; w < kWavesPerGroup; ++w) {
+    for (int w = 1; w < kFinal / kWave; ++w) {
         red[0][0] = fmax(red[0][0], red[w][0]);
         red[0][1] = fmax(red[0][1], red[w][1]);
         for (int q = 2; q < 8; q += 2) {
@@ -449,7 +491,6 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
         if (verified) ctrl->done = kVerified;
         else if (!finite) ctrl->done = kNumerical;
     }
-    ctrl->check_arrive = 0;
 }
 
 // ---- check of the FP32 phase: stationarity only, decides the switch to FP64 ----
@@ -473,7 +514,6 @@ __global__ __launch_bounds__(kGroup) void check32_kernel(Matrix<float> m, StepBu
                                                          double* partials, int parity, CheckParams prm) {
     __shared__ float stage[kWavesPerGroup][kCap];
     __shared__ double red[kWavesPerGroup];
-    __shared__ int last;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (ctrl->done) return;
     Check32Epilogue epi{bufs.x[parity], bufs.c, bufs.lo, bufs.up, 0.0};
@@ -486,20 +526,30 @@ __global__ __launch_bounds__(kGroup) void check32_kernel(Matrix<float> m, StepBu
         double a = red[0];
         for (int w = 1; w < kWavesPerGroup; ++w) a = fmax(a, red[w]);
         partials[blockIdx.x] = a;
-        __threadfence();
-        last = atomicAdd(&ctrl->check_arrive, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+__global__ __launch_bounds__(kFinal) void finalize_check32(Control* ctrl, const double* partials, int groups,
+                                                           int parity, CheckParams prm) {
+    __shared__ double red[kFinal / kWave];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (ctrl->done) return;
     double a = 0;
-    for (int g = threadIdx.x; g < (int)gridDim.x; g += kGroup) a = fmax(a, __builtin_nontemporal_load(partials + g));
+    for (int g0 = threadIdx.x; g0 < groups; g0 += 8 * kFinal) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int g = g0 + u * kFinal;
+            v[u] = g < groups ? partials[g] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a = fmax(a, v[u]);
+    }
     a = wave_max(a);
-    __syncthreads();
     if (lane == 0) red[wave] = a;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int w = 1; w < kWavesPerGroup; ++w) red[0] = fmax(red[0], red[w]);
+    for (int w = 1; w < kFinal / kWave; ++w) red[0] = fmax(red[0], red[w]);
     const double stat = red[0];
     ctrl->stationarity = stat;
     ctrl->check_iter = ctrl->iter[parity];
@@ -511,7 +561,30 @@ __global__ __launch_bounds__(kGroup) void check32_kernel(Matrix<float> m, StepBu
     // The FP32 phase ends at the switch threshold, on stagnation (FP32 rounding
     // floor reached) or on a non-finite residual; the FP64 check then decides.
     if (!(stat > prm.switch_stat) || ctrl->stall >= 4) ctrl->done = kPhaseEnd;
-    ctrl->check_arrive = 0;
+}
+
+// Anchors an FP32 correction cycle at the FP64 iterate xa: the correction
+// d = x - xa has gradient g(xa) + H d, so the FP32 step kernel runs unchanged
+// with c := g(xa) and the box shifted by -xa, starting from d = 0.
+__global__ void anchor_kernel(int64_t n, const double* g, const double* xa, const double* lo, const double* up,
+                              float* c32, float* lo32, float* up32, float* d, float* w) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        c32[j] = float(g[j]);
+        lo32[j] = float(lo[j] - xa[j]);
+        up32[j] = float(up[j] - xa[j]);
+        d[j] = 0.0f;
+        w[j] = 0.0f;
+    }
+}
+
+// x = clip(xa + d, lower, upper) in FP64; the extrapolated point restarts at x.
+__global__ void apply_kernel(int64_t n, const double* xa, const float* d, const double* lo, const double* up,
+                             double* x, double* w) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const double v = clip(xa[j] + double(d[j]), lo[j], up[j]);
+        x[j] = v;
+        w[j] = v;
+    }
 }
 
 template <class T>
@@ -553,6 +626,8 @@ void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, dou
         else HGQP_STEP(false, kMomentumRestart);
     }
 #undef HGQP_STEP
+    if (momentum == kMomentumRestart)
+        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity);
 }
 
 template void launch_step<double>(const Matrix<double>&, const StepBuffers<double>&, Control*, double*, int, int,
@@ -565,6 +640,7 @@ void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
     if (ordered) hipLaunchKernelGGL(check_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
     else hipLaunchKernelGGL(check_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+    hipLaunchKernelGGL(finalize_check, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity, params);
 }
 
 void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
@@ -572,6 +648,18 @@ void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
     if (ordered) hipLaunchKernelGGL(check32_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
     else hipLaunchKernelGGL(check32_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+    hipLaunchKernelGGL(finalize_check32, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity, params);
+}
+
+void launch_anchor(int64_t n, const double* g, const double* xa, const double* lo, const double* up, float* c32,
+                   float* lo32, float* up32, float* d, float* w, hipStream_t stream) {
+    hipLaunchKernelGGL(anchor_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, g, xa, lo, up, c32, lo32, up32,
+                       d, w);
+}
+
+void launch_apply(int64_t n, const double* xa, const float* d, const double* lo, const double* up, double* x,
+                  double* w, hipStream_t stream) {
+    hipLaunchKernelGGL(apply_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, xa, d, lo, up, x, w);
 }
 
 template <class T>

@@ -35,8 +35,8 @@ struct Control {
     int32_t use_x[2];  // 1: the step gathers x instead of the extrapolated point (after a restart)
     int32_t done;
     int32_t restarts;
-    uint32_t arrive;        // last-group counter of restart steps
-    uint32_t check_arrive;  // last-group counter of checks
+    uint32_t reserved0;
+    uint32_t reserved1;
     int64_t check_iter;     // iteration of the last evaluated check
     int32_t check_parity;   // buffer parity of the last evaluated check
     int32_t checks;
@@ -74,6 +74,7 @@ struct CheckParams {
     int diagonal;       // 1: Hessian is positive diagonal (separable dual bound)
     int set_done;       // 1: a verified / non-finite check stops the loop
     double switch_stat; // FP32 phase: stationarity that ends the phase
+    double* gout;       // FP64 check: optional per-row gradient output
 };
 
 // Launchers. parity selects x[parity], w[parity] as inputs; step launches
@@ -87,6 +88,13 @@ void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control
 
 void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
                     int parity, bool ordered, const CheckParams& params, hipStream_t stream);
+
+// FP32 correction cycle anchored at xa with FP64 gradient g (see anchor_kernel)
+void launch_anchor(int64_t n, const double* g, const double* xa, const double* lo, const double* up, float* c32,
+                   float* lo32, float* up32, float* d, float* w, hipStream_t stream);
+// x = w = clip(xa + d, lo, up)
+void launch_apply(int64_t n, const double* xa, const float* d, const double* lo, const double* up, double* x,
+                  double* w, hipStream_t stream);
 
 // x[0] = w[0] = clip(0, lower, upper)
 template <class T>

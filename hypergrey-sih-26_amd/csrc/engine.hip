@@ -105,6 +105,7 @@ struct hgqp_box {
     double* val = nullptr;
     float* val32 = nullptr;
     double *c = nullptr, *lo = nullptr, *up = nullptr;
+    double* g = nullptr;  // FP64 gradient of the last check (FP32_REFINE)
     float *c32 = nullptr, *lo32 = nullptr, *up32 = nullptr;
     double* x[2] = {nullptr, nullptr};
     double* w[2] = {nullptr, nullptr};
@@ -184,6 +185,7 @@ CheckParams check_params(const hgqp_box* b, double tol, int set_done, double swi
     p.diagonal = b->diagonal;
     p.set_done = set_done;
     p.switch_stat = switch_stat;
+    p.gout = nullptr;
     return p;
 }
 
@@ -214,25 +216,45 @@ void enqueue_period(hgqp_box* b, int precision, int steps, int momentum, const C
         launch_check32(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
 }
 
-// Timed variant: an event pair around every step launch.
+// Timed variant: one event pair around the period's back-to-back step launches
+// (the check is outside the pair), so the pair measures `steps` step launches.
+struct TimedPeriod {
+    size_t event;       // index of the start event; end event is event + 1
+    uint64_t first;     // iteration index of the first step
+    int steps;
+    int precision;
+    bool worked;        // false for periods enqueued after their phase had ended
+};
+
+// Marks the periods appended since `from` that ran before the phase's deciding
+// check at iteration `end` (later ones returned early without work).
+void settle_periods(std::vector<TimedPeriod>* timed, size_t from, uint64_t end) {
+    if (!timed) return;
+    for (size_t i = from; i < timed->size(); ++i) {
+        TimedPeriod& p = (*timed)[i];
+        p.worked = p.first + uint64_t(p.steps) <= end;
+    }
+}
+
 void enqueue_period_timed(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp,
-                          size_t* next_event) {
+                          uint64_t first, std::vector<TimedPeriod>* timed) {
     const bool ord = b->ordered();
+    const size_t e = timed->empty() ? 0 : timed->back().event + 2;
+    while (b->events.size() < e + 2) {
+        hipEvent_t ev;
+        HIPCHECK(hipEventCreate(&ev));
+        b->events.push_back(ev);
+    }
+    HIPCHECK(hipEventRecord(b->events[e], b->stream));
     for (int s = 0; s < steps; ++s) {
-        while (b->events.size() < *next_event + 2) {
-            hipEvent_t e;
-            HIPCHECK(hipEventCreate(&e));
-            b->events.push_back(e);
-        }
-        HIPCHECK(hipEventRecord(b->events[*next_event], b->stream));
         if (precision == HGQP_PRECISION_FP64)
             launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
         else
             launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
-        HIPCHECK(hipEventRecord(b->events[*next_event + 1], b->stream));
-        *next_event += 2;
         b->parity ^= 1;
     }
+    HIPCHECK(hipEventRecord(b->events[e + 1], b->stream));
+    timed->push_back({e, first, steps, precision, true});
     if (precision == HGQP_PRECISION_FP64)
         launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
     else
@@ -266,7 +288,7 @@ struct LoopOutcome {
 // or the time limit. `it` counts iterations of the whole solve.
 LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, uint64_t limit, int check_every,
                       const CheckParams& cp, std::chrono::steady_clock::time_point t0, double time_limit,
-                      size_t* timed_events) {
+                      std::vector<TimedPeriod>* timed) {
     LoopOutcome out;
     const int lag = 2;
     std::vector<int> pending;  // slot indices of enqueued read-backs, oldest first
@@ -274,7 +296,7 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
     bool stop = false;
     while (it < limit && !stop) {
         const int steps = static_cast<int>(std::min<uint64_t>(check_every, limit - it));
-        if (timed_events) enqueue_period_timed(b, precision, steps, momentum, cp, timed_events);
+        if (timed) enqueue_period_timed(b, precision, steps, momentum, cp, it, timed);
         else if (b->opt.use_graph && steps == check_every) {
             hipGraphExec_t g = period_graph(b, precision, steps, momentum, cp);
             HIPCHECK(hipGraphLaunch(g, b->stream));
@@ -351,7 +373,8 @@ int hgqp_device_name(char* buffer, size_t length) {
         HIPCHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, dev));
-        std::string name = std::string(prop.name) + " (" + prop.gcnArchName + ", " +
+        std::string label = prop.name[0] ? std::string(prop.name) : std::string("AMD GPU");
+        std::string name = label + " (" + prop.gcnArchName + ", " +
                            std::to_string(prop.multiProcessorCount) + " CUs, " +
                            std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
         std::strncpy(buffer, name.c_str(), length - 1);
@@ -392,7 +415,8 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         hgqp_box_options opt;
         if (options) opt = *options;
         else hgqp_box_options_default(&opt);
-        if (opt.precision != HGQP_PRECISION_FP64 && opt.precision != HGQP_PRECISION_FP32_POLISH)
+        if (opt.precision != HGQP_PRECISION_FP64 && opt.precision != HGQP_PRECISION_FP32_POLISH &&
+            opt.precision != HGQP_PRECISION_FP32_REFINE)
             throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown precision");
         if (opt.momentum != HGQP_MOMENTUM_REFERENCE && opt.momentum != HGQP_MOMENTUM_RESTART)
             throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown momentum schedule");
@@ -408,12 +432,14 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         std::vector<int> blk = row_blocks(n, start);
         b->nblocks = static_cast<int>(blk.size()) - 1;
         b->groups = groups_for(b->nblocks);
-        const bool f32 = opt.precision == HGQP_PRECISION_FP32_POLISH;
+        const bool f32 = opt.precision != HGQP_PRECISION_FP64;
+        const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
         const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
         size_t bytes = Arena::span(blk.size(), 4) + Arena::span(un + 1, 4) + Arena::span(unnz, 4) +
                        Arena::span(unnz, 8) + 3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) +
                        Arena::span(1, sizeof(Control)) + Arena::span(size_t(b->groups) * 10 + 16, 8);
         if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
+        if (refine) bytes += Arena::span(un, 8);
         HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         b->arena.reserve(bytes);
         b->blk = b->arena.take<int>(blk.size());
@@ -439,6 +465,7 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
                 b->w32[p] = b->arena.take<float>(un);
             }
         }
+        if (refine) b->g = b->arena.take<double>(un);
         HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(Control) * hgqp_box::kSlots,
                                hipHostMallocDefault));
         for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -452,7 +479,7 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
         HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
         if (f32) {
-            if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);
+            if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
             launch_convert<float, double>(n, b->c, b->c32, b->stream);
             launch_convert<float, double>(n, b->lo, b->lo32, b->stream);
             launch_convert<float, double>(n, b->up, b->up32, b->stream);
@@ -513,71 +540,100 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         if (options) so = *options;
         else hgqp_solve_options_default(&so);
         if (!(so.tolerance > 0) || !std::isfinite(so.tolerance) || so.time_limit < 0 || !std::isfinite(so.time_limit) ||
-            so.check_every < 1)
+            so.check_every < 1 || so.polish_every < 0)
             throw Error(HGQP_INVALID_ARGUMENT, "Invalid box QP options");
         std::memset(report, 0, sizeof(*report));
         const auto t0 = std::chrono::steady_clock::now();
         const int momentum = b->opt.momentum == HGQP_MOMENTUM_RESTART ? kMomentumRestart : kMomentumReference;
-        const bool f32 = b->opt.precision == HGQP_PRECISION_FP32_POLISH;
+        const int precision = b->opt.precision;
+        const bool f32 = precision != HGQP_PRECISION_FP64;
+        const bool refine = precision == HGQP_PRECISION_FP32_REFINE;
+        const int polish_every = so.polish_every > 0 ? so.polish_every : so.check_every;
         const double switch_stat = so.fp32_switch > 0 ? so.fp32_switch : 10 * so.tolerance;
-        const CheckParams cp64 = check_params(b, so.tolerance, 1, 0.0);
-        const CheckParams cp32 = check_params(b, so.tolerance, 1, switch_stat);
-        size_t timed = 0;
-        size_t* timed_ptr = b->opt.timing ? &timed : nullptr;
-        struct Span { uint64_t first, end; int precision; size_t ev_first, ev_end; };
-        std::vector<Span> spans;
+        CheckParams cp64 = check_params(b, so.tolerance, 1, 0.0);
+        if (refine) cp64.gout = b->g;  // the anchor gradient of the next correction cycle
+        std::vector<TimedPeriod> timed;
+        std::vector<TimedPeriod>* timed_ptr = b->opt.timing ? &timed : nullptr;
 
         // fresh momentum state at the handle's current iterate (accelerated_qp.cpp:37)
-        Control c0 = fresh_control();
-        put_control(b, c0);
+        put_control(b, fresh_control());
         uint64_t it = 0;
         bool time_up = false;
-        uint64_t fp32_its = 0;
         // check at iteration 0 (accelerated_qp.cpp:40)
         launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), cp64, b->stream);
         HIPCHECK(hipGetLastError());
         Control c = get_control(b);
-        if (c.done == kRunning && f32 && so.iteration_limit > 0) {
-            launch_convert<float, double>(b->n, b->x[b->parity], b->x32[b->parity], b->stream);
-            launch_convert<float, double>(b->n, b->w[b->parity], b->w32[b->parity], b->stream);
-            HIPCHECK(hipGetLastError());
-            const size_t ev0 = timed;
-            auto o = run_phase(b, HGQP_PRECISION_FP32_POLISH, momentum, it, so.iteration_limit, so.check_every, cp32, t0,
-                               so.time_limit, timed_ptr);
-            time_up = o.time_up;
-            c = get_control(b);
-            // steps after the deciding FP32 check returned early without flipping buffers
-            if (c.done != kRunning) {
-                it = uint64_t(c.check_iter);
-                b->parity = c.check_parity;
-            }
-            fp32_its = it;
-            spans.push_back({0, it, HGQP_PRECISION_FP32_POLISH, ev0, timed});
-            launch_convert<double, float>(b->n, b->x32[b->parity], b->x[b->parity], b->stream);
-            launch_convert<double, float>(b->n, b->x32[b->parity], b->w[b->parity], b->stream);
+        // Resets the momentum state and the done flag at iteration `at` for a new phase.
+        auto new_phase = [&](uint64_t at) {
             Control cc = c;
             cc.done = kRunning;
             cc.t[b->parity] = 1.0;
             cc.use_x[b->parity] = 0;
-            cc.iter[b->parity] = int64_t(it);
+            cc.iter[b->parity] = int64_t(at);
+            cc.best_stat = std::numeric_limits<double>::infinity();
+            cc.stall = 0;
             put_control(b, cc);
+        };
+        // One FP32 phase from the current parity; on return x32/w32[b->parity]
+        // hold the phase's last checked iterate.
+        auto fp32_phase = [&](double stat_target) {
+            CheckParams cp32 = check_params(b, so.tolerance, 1, stat_target);
+            const size_t from = timed.size();
+            auto o = run_phase(b, HGQP_PRECISION_FP32_POLISH, momentum, it, so.iteration_limit, so.check_every, cp32,
+                               t0, so.time_limit, timed_ptr);
+            time_up = o.time_up;
+            c = get_control(b);
+            if (c.done != kRunning) {  // steps after the deciding check returned early
+                it = uint64_t(c.check_iter);
+                b->parity = c.check_parity;
+            }
+            settle_periods(timed_ptr, from, it);
+        };
+        auto fp64_check = [&] {
+            new_phase(it);
             launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), cp64, b->stream);
             HIPCHECK(hipGetLastError());
             c = get_control(b);
+        };
+        if (c.done == kRunning && refine) {
+            // FP32 correction cycles: the first from x0 ends near the FP32 floor
+            // (switch_stat); later ones start from an FP64 anchor and aim below tol.
+            for (int cycle = 0; cycle < 8 && c.done == kRunning && !time_up && it < so.iteration_limit; ++cycle) {
+                const int pa = b->parity;
+                launch_anchor(b->n, b->g, b->x[pa], b->lo, b->up, b->c32, b->lo32, b->up32, b->x32[pa], b->w32[pa],
+                              b->stream);
+                HIPCHECK(hipGetLastError());
+                new_phase(it);
+                fp32_phase(cycle == 0 ? switch_stat : 0.5 * so.tolerance);
+                launch_apply(b->n, b->x[pa], b->x32[b->parity], b->lo, b->up, b->x[b->parity], b->w[b->parity],
+                             b->stream);
+                fp64_check();
+            }
+            report->fp32_iterations = it;
+        } else if (c.done == kRunning && f32 && so.iteration_limit > 0) {
+            launch_convert<float, double>(b->n, b->x[b->parity], b->x32[b->parity], b->stream);
+            launch_convert<float, double>(b->n, b->w[b->parity], b->w32[b->parity], b->stream);
+            HIPCHECK(hipGetLastError());
+            new_phase(it);
+            fp32_phase(switch_stat);
+            report->fp32_iterations = it;
+            launch_convert<double, float>(b->n, b->x32[b->parity], b->x[b->parity], b->stream);
+            launch_convert<double, float>(b->n, b->x32[b->parity], b->w[b->parity], b->stream);
+            fp64_check();
         }
         if (c.done == kRunning && !time_up && it < so.iteration_limit) {
-            const uint64_t first = it;
-            const size_t ev0 = timed;
-            auto o = run_phase(b, HGQP_PRECISION_FP64, momentum, it, so.iteration_limit, so.check_every, cp64, t0,
+            const int every = f32 ? polish_every : so.check_every;
+            new_phase(it);
+            const size_t from = timed.size();
+            auto o = run_phase(b, HGQP_PRECISION_FP64, momentum, it, so.iteration_limit, every, cp64, t0,
                                so.time_limit, timed_ptr);
             time_up = o.time_up;
             c = get_control(b);
             if (c.done != kRunning) b->parity = c.check_parity;
-            spans.push_back({first, uint64_t(c.check_iter), HGQP_PRECISION_FP64, ev0, timed});
+            settle_periods(timed_ptr, from, uint64_t(c.check_iter));
         }
         HIPCHECK(hipStreamSynchronize(b->stream));
         report->iterations = uint64_t(c.check_iter);
-        report->fp32_iterations = fp32_its;
         report->restarts = c.restarts;
         report->checks = uint64_t(c.checks);
         fill_report(c, &report->check);
@@ -586,18 +642,12 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         else if (time_up) report->status = HGQP_STATUS_TIME_LIMIT;
         else report->status = HGQP_STATUS_ITERATION_LIMIT;
         report->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        // event pairs of steps that did work: iteration indices [first, end) of each span
-        for (const Span& s : spans) {
-            const uint64_t work = s.end > s.first ? s.end - s.first : 0;
-            double ms = 0;
-            uint64_t cnt = 0;
-            for (size_t e = s.ev_first; e + 1 < s.ev_end && cnt < work; e += 2, ++cnt) {
-                float t = 0;
-                HIPCHECK(hipEventElapsedTime(&t, b->events[e], b->events[e + 1]));
-                ms += t;
-            }
-            if (s.precision == HGQP_PRECISION_FP64) { report->step_kernel_ms += ms; report->step_launches += cnt; }
-            else { report->step32_kernel_ms += ms; report->step32_launches += cnt; }
+        for (const TimedPeriod& p : timed) {
+            if (!p.worked) continue;
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, b->events[p.event], b->events[p.event + 1]));
+            if (p.precision == HGQP_PRECISION_FP64) { report->step_kernel_ms += ms; report->step_launches += p.steps; }
+            else { report->step32_kernel_ms += ms; report->step32_launches += p.steps; }
         }
     });
 }

@@ -15,7 +15,7 @@ LIBRARY = os.path.join(HERE, "libhgqp.so")
 OK, INVALID_ARGUMENT, NO_DEVICE, OUT_OF_MEMORY, DEVICE_ERROR, BUSY, UNSUPPORTED = range(7)
 STATUS = ["optimal", "iteration_limit", "time_limit", "numerical_error", "infeasible_bounds",
           "unsupported_curvature", "running"]
-PRECISION_FP64, PRECISION_FP32_POLISH = 0, 1
+PRECISION_FP64, PRECISION_FP32_POLISH, PRECISION_FP32_REFINE = 0, 1, 2
 MOMENTUM_REFERENCE, MOMENTUM_RESTART = 0, 1
 REDUCE_ORDERED, REDUCE_TREE = 0, 1
 
@@ -51,7 +51,7 @@ class CheckReport(C.Structure):
 
 class SolveOptions(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("time_limit", C.c_double), ("iteration_limit", C.c_uint64),
-                ("check_every", C.c_int32), ("reserved", C.c_int32), ("fp32_switch", C.c_double)]
+                ("check_every", C.c_int32), ("polish_every", C.c_int32), ("fp32_switch", C.c_double)]
 
 
 class BoxReport(C.Structure):

@@ -44,6 +44,10 @@ extern "C" {
 /* ---- iteration arithmetic ---------------------------------------------- */
 #define HGQP_PRECISION_FP64 0        /* FP64 values and iterates             */
 #define HGQP_PRECISION_FP32_POLISH 1 /* FP32 iterations, FP64 checks, FP64 polish */
+#define HGQP_PRECISION_FP32_REFINE 2 /* FP32 correction cycles anchored at FP64
+                                        iterates and FP64 gradients from the
+                                        device check (defect correction); FP64
+                                        iterations only as a last resort     */
 
 /* ---- momentum schedule --------------------------------------------------- */
 #define HGQP_MOMENTUM_REFERENCE 0 /* t-recurrence, no restart (accelerated_qp.cpp:37,47) */
@@ -92,7 +96,7 @@ typedef struct hgqp_solve_options {
     double time_limit;       /* seconds, default 30                               */
     uint64_t iteration_limit;/* default 200000                                    */
     int32_t check_every;     /* check cadence, 25 in the reference (accelerated_qp.cpp:40) */
-    int32_t reserved;
+    int32_t polish_every;    /* cadence of FP64 checks after an FP32 phase (0 = check_every) */
     double fp32_switch;      /* FP32_POLISH: switch to FP64 once the FP32 check's
                                 stationarity falls below this (0 = 10*tolerance)  */
 } hgqp_solve_options;
