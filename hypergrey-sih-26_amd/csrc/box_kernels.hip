@@ -238,8 +238,8 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     epi.lipschitz = bufs.lipschitz;
     epi.beta = T(beta);
     epi.dot = 0;
-    const int b = blockIdx.x * kWavesPerGroup + wave;
-    if (b < m.nblocks) row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi);
+    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+        row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi);
     if (MOM == kMomentumReference) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctrl->t[parity ^ 1] = next_t;
@@ -370,8 +370,8 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
     epi.viol = 0;
     epi.value = epi.tangent = epi.separable = DD{0, 0};
     epi.bound_inf = epi.bad = 0;
-    const int b = blockIdx.x * kWavesPerGroup + wave;
-    if (b < m.nblocks) row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+        row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
     // workgroup partial
     {
         double stat = wave_max(epi.stat), viol = wave_max(epi.viol);
@@ -517,8 +517,8 @@ __global__ __launch_bounds__(kGroup) void check32_kernel(Matrix<float> m, StepBu
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (ctrl->done) return;
     Check32Epilogue epi{bufs.x[parity], bufs.c, bufs.lo, bufs.up, 0.0};
-    const int b = blockIdx.x * kWavesPerGroup + wave;
-    if (b < m.nblocks) row_block<float, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+        row_block<float, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
     double s = wave_max(epi.stat);
     if (lane == 0) red[wave] = s;
     __syncthreads();
@@ -609,7 +609,13 @@ int elementwise_grid(int64_t n) {
 
 }  // namespace
 
-int groups_for(int nblocks) { return (nblocks + kWavesPerGroup - 1) / kWavesPerGroup; }
+// Workgroups of a sparse launch: one per four row blocks, at most kMaxGroups
+// (8 waves per SIMD on 256 CUs); waves then stride over the remaining blocks,
+// which also bounds the per-workgroup partials a finalize launch reads.
+int groups_for(int nblocks) {
+    int g = (nblocks + kWavesPerGroup - 1) / kWavesPerGroup;
+    return g < kMaxGroups ? (g < 1 ? 1 : g) : kMaxGroups;
+}
 
 template <class T>
 void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,

@@ -15,6 +15,7 @@ constexpr int kGroup = kWave * kWavesPerGroup;
 constexpr int kLoads = 8;
 constexpr int kCap = kWave * kLoads;
 constexpr int kMaxRows = 256;
+constexpr int kMaxGroups = 2048;
 
 // Momentum source of a step launch.
 enum Momentum : int {
