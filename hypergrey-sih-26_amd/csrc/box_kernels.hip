@@ -23,195 +23,20 @@
 // matrix-vector multiplication on GPUs using the CSR storage format", SC 2014),
 // written for wave64 with the solver step fused into the row epilogue.
 #include "box_kernels.hpp"
+#include "sparse_pass.hpp"
 
 #include <cmath>
 #include <limits>
 
 namespace hgqp {
 namespace {
+using namespace dev;
 
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <class T>
-__device__ inline T load_stream(const T* p) {
-    return __builtin_nontemporal_load(p);
-}
-
-// std::max(lo, std::min(up, v)) with the comparisons of <algorithm>
-// (accelerated_qp.cpp:54; qp_admm.cpp:21).
-template <class T>
-__device__ inline T clip(T v, T lo, T up) {
-    T m = (v < up) ? v : up;
-    return (lo < m) ? m : lo;
-}
-
-// ---- double-double accumulation (Dekker / Knuth two-sum, FMA two-product) ----
-struct DD {
-    double hi, lo;
-};
-
-__device__ inline DD dd_add(DD a, DD b) {
-#pragma clang fp contract(off)
-    double s = a.hi + b.hi;
-    if (!isfinite(s)) return {s, 0.0};
-    double bb = s - a.hi;
-    double e = (a.hi - (s - bb)) + (b.hi - bb);
-    e = e + a.lo + b.lo;
-    double h = s + e;
-    return {h, e - (h - s)};
-}
-
-__device__ inline DD dd_from(double a) { return {a, 0.0}; }
-
-__device__ inline DD dd_prod(double a, double b) {
-    double p = a * b;
-    if (!isfinite(p)) return {p, 0.0};
-    return {p, fma(a, b, -p)};
-}
-
-__device__ inline double shfl_xor(double v, int m) { return __shfl_xor(v, m, kWave); }
-__device__ inline float shfl_xor(float v, int m) { return __shfl_xor(v, m, kWave); }
-__device__ inline int shfl_xor(int v, int m) { return __shfl_xor(v, m, kWave); }
-
-__device__ inline double wave_sum(double v) {
-    for (int m = 32; m >= 1; m >>= 1) v += shfl_xor(v, m);
-    return v;
-}
-__device__ inline double wave_max(double v) {
-    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor(v, m));
-    return v;
-}
-__device__ inline int wave_or(int v) {
-    for (int m = 32; m >= 1; m >>= 1) v |= shfl_xor(v, m);
-    return v;
-}
-__device__ inline DD wave_dd(DD v) {
-    for (int m = 32; m >= 1; m >>= 1) {
-        DD o{shfl_xor(v.hi, m), shfl_xor(v.lo, m)};
-        v = dd_add(v, o);
-    }
-    return v;
-}
-
-// ---- the shared sparse pass --------------------------------------------------
-// Calls epi(row, sum) once per row of block b from the lane that owns the row.
+// Calls epi(row, sum) for each row of block b of m.
 template <class T, bool ORDERED, class Epi>
-__device__ inline void row_block(const Matrix<T>& m, const T* __restrict__ gat, int b, T* __restrict__ lds,
-                                 int lane, Epi& epi) {
-    const int r0 = m.blk[b], r1 = m.blk[b + 1];
-    const int k0 = m.start[r0], k1 = m.start[r1];
-    if (k1 - k0 <= kCap) {
-        T p[kLoads];
-        int cidx[kLoads];
-#pragma unroll
-        for (int j = 0; j < kLoads; ++j) {
-            const int k = k0 + j * kWave + lane;
-            cidx[j] = k < k1 ? load_stream(m.col + k) : -1;
-            p[j] = k < k1 ? load_stream(m.val + k) : T(0);
-        }
-#pragma unroll
-        for (int j = 0; j < kLoads; ++j) {
-            T g = cidx[j] >= 0 ? gat[cidx[j]] : T(0);
-            lds[j * kWave + lane] = p[j] * g;  // rounded product, as in the CPU sum
-        }
-        wave_sync();
-        const int nrows = r1 - r0;
-        if (ORDERED || nrows > kWave / 2) {
-            for (int r = r0 + lane; r < r1; r += kWave) {
-                const int a = m.start[r] - k0, e = m.start[r + 1] - k0;
-                T s = 0;
-                int q = a;
-                // eight LDS reads in flight, then the adds in stored order
-                for (; q + 8 <= e; q += 8) {
-                    T v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) v[u] = lds[q + u];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) s += v[u];
-                }
-                for (; q < e; ++q) s += lds[q];
-                epi(r, s);
-            }
-        } else {
-            // 2..64 lanes per row: lpr = 64 / next_pow2(nrows)
-            int lg = 0;
-            while ((1 << lg) < nrows) ++lg;
-            const int lpr = kWave >> lg;
-            const int r = r0 + lane / lpr, sub = lane % lpr;
-            T s = 0;
-            if (r < r1) {
-                const int a = m.start[r] - k0, e = m.start[r + 1] - k0;
-                for (int q = a + sub; q < e; q += lpr) s += lds[q];
-            }
-            for (int msk = lpr >> 1; msk >= 1; msk >>= 1) s += shfl_xor(s, msk);
-            if (sub == 0 && r < r1) epi(r, s);
-        }
-        wave_sync();
-    } else {
-        // one long row, streamed in kCap chunks
-        T s = 0;
-        for (int base = k0; base < k1; base += kCap) {
-            if (ORDERED) {
-#pragma unroll
-                for (int j = 0; j < kLoads; ++j) {
-                    const int k = base + j * kWave + lane;
-                    lds[j * kWave + lane] = k < k1 ? load_stream(m.val + k) * gat[load_stream(m.col + k)] : T(0);
-                }
-                wave_sync();
-                if (lane == 0) {
-                    const int e = min(kCap, k1 - base);
-                    for (int q = 0; q < e; ++q) s += lds[q];
-                }
-                wave_sync();
-            } else {
-#pragma unroll
-                for (int j = 0; j < kLoads; ++j) {
-                    const int k = base + j * kWave + lane;
-                    if (k < k1) s += load_stream(m.val + k) * gat[load_stream(m.col + k)];
-                }
-            }
-        }
-        if (!ORDERED) {
-            for (int msk = 32; msk >= 1; msk >>= 1) s += shfl_xor(s, msk);
-        }
-        if (lane == 0) epi(r0, s);
-    }
-}
-
-// ---- step ---------------------------------------------------------------------
-template <class T>
-struct StepEpilogue {
-    const T* __restrict__ gat;  // point the gradient was taken at
-    const T* __restrict__ xin;
-    const T* __restrict__ c;
-    const T* __restrict__ lo;
-    const T* __restrict__ up;
-    T* __restrict__ xout;
-    T* __restrict__ wout;
-    T lipschitz;
-    T beta;
-    double dot;  // restart test: sum (w - next)(next - x)
-    __device__ inline void operator()(int r, T g) {
-#pragma clang fp contract(off)
-        const T wv = gat[r], xv = xin[r];
-        const T next = clip(wv - (g + c[r]) / lipschitz, lo[r], up[r]);
-        wout[r] = next + beta * (next - xv);
-        xout[r] = next;
-        dot += double(wv - next) * double(next - xv);
-    }
-};
-
-// t-recurrence of accelerated_qp.cpp:47 with IEEE double operations, so the
-// device sequence equals the host's bit for bit.
-__device__ inline void momentum_of(double t, double* next_t, double* beta) {
-#pragma clang fp contract(off)
-    double nt = (1 + sqrt(1 + 4 * t * t)) / 2;
-    *next_t = nt;
-    *beta = (t - 1) / nt;
+__device__ inline void row_block(const Matrix<T>& m, const T* __restrict__ gat, int b, T* __restrict__ lds, int lane,
+                                 Epi& epi) {
+    dev::row_block<T, ORDERED, kLoads>(m.start, m.col, m.val, gat, m.blk[b], m.blk[b + 1], lds, lane, epi);
 }
 
 template <class T, bool ORDERED, int MOM>
@@ -296,58 +121,6 @@ __global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const 
         ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
     }
 }
-
-// ---- check (FP64) ---------------------------------------------------------------
-struct CheckEpilogue {
-    const double* __restrict__ x;
-    const double* __restrict__ c;
-    const double* __restrict__ lo;
-    const double* __restrict__ up;
-    const int* __restrict__ start;
-    const int* __restrict__ col;
-    const double* __restrict__ val;
-    double* __restrict__ gout;  // optional: the FP64 gradient Hx + c per row
-    int diagonal;
-    double stat, viol;
-    DD value, tangent, separable;
-    int bound_inf, bad;
-    __device__ inline void operator()(int r, double hx) {
-#pragma clang fp contract(off)
-        const double xv = x[r], cv = c[r], l = lo[r], u = up[r];
-        if (!isfinite(xv)) { bad = 1; return; }
-        // objective term 1/2 x (Hx) + c x  (qp_admm.cpp:79)
-        DD t = dd_prod(xv, hx);
-        t.hi *= 0.5;
-        t.lo *= 0.5;
-        value = dd_add(value, dd_add(t, dd_from(cv * xv)));
-        // stationarity (qp_admm.cpp:94-97)
-        const double g = hx + cv;
-        if (gout) gout[r] = g;
-        double res = fabs(xv - clip(xv - g, l, u)) / (1 + fabs(xv));
-        if (!isfinite(g) || !isfinite(res)) res = INFINITY;
-        stat = fmax(stat, res);
-        // tangent dual bound (qp_admm.cpp:100)
-        if (g != 0) {
-            const double target = g > 0 ? l : u;
-            if (isfinite(target)) tangent = dd_add(tangent, dd_from(g * (target - xv)));
-            else bound_inf = 1;
-        }
-        // separable bound for a positive diagonal Hessian (qp_admm.cpp:106-112)
-        if (diagonal) {
-            double d = 0;
-            for (int k = start[r]; k < start[r + 1]; ++k)
-                if (col[k] == r) d = val[k];
-            const double mnz = clip(-cv / d, l, u);
-            DD s = dd_prod(0.5 * d * mnz, mnz);
-            separable = dd_add(separable, dd_add(s, dd_prod(cv, mnz)));
-        }
-        // scaled bound violation, original_violation (mps.cpp:284-295)
-        if (isfinite(l)) viol = fmax(viol, (l - xv) / (1 + fabs(l) + fabs(xv)));
-        if (isfinite(u)) viol = fmax(viol, (xv - u) / (1 + fabs(u) + fabs(xv)));
-    }
-};
-
-constexpr int kPartial = 10;  // doubles per workgroup partial of a check
 
 template <bool ORDERED>
 __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuffers<double> bufs, Control* ctrl,
@@ -451,32 +224,10 @@ __global__ __launch_bounds__(kFinal) void finalize_check(Control* ctrl, const do
         }
         red[0][8] = double(int(red[0][8]) | int(red[w][8]));
     }
-    const double* o = red[0];
-    const int fl = int(o[8]);
-    const double nan = __builtin_nan("");
-    double objective = nan, primal = INFINITY, stationarity = INFINITY, bound = -INFINITY, gap = INFINITY;
-    int verified = 0;
-    if (!(fl & 2)) {
-        // value = sign*offset + sum(1/2 x Hx + c x) in the normalized sense (qp_admm.cpp:71,79)
-        DD value = dd_add(dd_from(prm.sign * prm.offset), DD{o[2], o[3]});
-        objective = prm.sign * (value.hi + value.lo);
-        primal = o[1];
-        stationarity = o[0];
-        DD bnd;
-        bool bnd_finite = true;
-        if (prm.diagonal) bnd = dd_add(dd_from(prm.sign * prm.offset), DD{o[6], o[7]});
-        else if (fl & 1) bnd_finite = false;
-        else bnd = dd_add(value, DD{o[4], o[5]});
-        if (bnd_finite) {
-            bound = bnd.hi + bnd.lo;
-            DD diff = dd_add(value, DD{-bnd.hi, -bnd.lo});
-            double dv = diff.hi + diff.lo;
-            gap = isfinite(bound) ? fabs(dv) / (1 + fabs(value.hi + value.lo)) : INFINITY;
-        }
-        verified = isfinite(objective) && fmax(primal, fmax(stationarity, 0.0)) <= prm.tol;
-        if (prm.diagonal) verified = verified && gap <= prm.tol;
-    }
-    const int finite = isfinite(objective) && isfinite(stationarity);
+    const Verdict vd = verdict(red[0], prm.sign, prm.offset, prm.diagonal, prm.tol);
+    const double objective = vd.objective, primal = vd.primal, stationarity = vd.stationarity, bound = vd.bound,
+                 gap = vd.gap;
+    const int verified = vd.verified, finite = vd.finite;
     ctrl->objective = objective;
     ctrl->primal_residual = primal;
     ctrl->stationarity = stationarity;

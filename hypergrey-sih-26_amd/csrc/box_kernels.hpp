@@ -16,6 +16,7 @@ constexpr int kLoads = 8;
 constexpr int kCap = kWave * kLoads;
 constexpr int kMaxRows = 256;
 constexpr int kMaxGroups = 2048;
+constexpr int kPartial = 10;  // doubles per partial of an FP64 check
 
 // Momentum source of a step launch.
 enum Momentum : int {

@@ -1,0 +1,40 @@
+// Host helpers around the HIP runtime shared by the engines.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "internal.hpp"
+
+namespace hgqp {
+
+// Throws Error with an HGQP_* code for a failed HIP call.
+void hip_check(hipError_t e, const char* what);
+#define HIPCHECK(call) ::hgqp::hip_check((call), #call)
+
+// Checks that a HIP device is present and is gfx950.
+void require_device();
+
+// One hipMalloc per handle; sub-buffers are 256-byte aligned slices.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    void reserve(size_t bytes) {
+        size = bytes;
+        HIPCHECK(hipMalloc(reinterpret_cast<void**>(&base), bytes));
+    }
+    template <class T>
+    T* take(size_t count) {
+        size_t off = (used + 255) & ~size_t(255);
+        size_t bytes = count * sizeof(T);
+        if (off + bytes > size) throw Error(HGQP_DEVICE_ERROR, "internal: arena overflow");
+        used = off + bytes;
+        return reinterpret_cast<T*>(base + off);
+    }
+    static size_t span(size_t count, size_t elem) { return ((count * elem + 255) & ~size_t(255)) + 256; }
+    ~Arena() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+}  // namespace hgqp

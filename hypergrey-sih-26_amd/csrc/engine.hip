@@ -3,6 +3,7 @@
 #include "box_kernels.hpp"
 #include "hgqp.h"
 #include "internal.hpp"
+#include "device_util.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -30,7 +31,6 @@ void hip_check(hipError_t e, const char* what) {
              : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? HGQP_NO_DEVICE : HGQP_DEVICE_ERROR;
     throw Error(code, std::string("HIP: ") + what + ": " + hipGetErrorString(e));
 }
-#define HIPCHECK(call) hip_check((call), #call)
 
 // Checks that a HIP device is present and is gfx950; the library carries
 // gfx950 code only.
@@ -46,38 +46,16 @@ void require_device() {
         throw Error(HGQP_NO_DEVICE, std::string("hgqp is built for gfx950; device is ") + prop.gcnArchName);
 }
 
-// One hipMalloc per handle; sub-buffers are 256-byte aligned slices.
-struct Arena {
-    char* base = nullptr;
-    size_t size = 0, used = 0;
-    void reserve(size_t bytes) {
-        size = bytes;
-        HIPCHECK(hipMalloc(reinterpret_cast<void**>(&base), bytes));
-    }
-    template <class T>
-    T* take(size_t count) {
-        size_t off = (used + 255) & ~size_t(255);
-        size_t bytes = count * sizeof(T);
-        if (off + bytes > size) throw Error(HGQP_DEVICE_ERROR, "internal: arena overflow");
-        used = off + bytes;
-        return reinterpret_cast<T*>(base + off);
-    }
-    static size_t span(size_t count, size_t elem) { return ((count * elem + 255) & ~size_t(255)) + 256; }
-    ~Arena() {
-        if (base) (void)hipFree(base);
-    }
-};
-
-std::vector<int> row_blocks(int64_t n, const int32_t* start) {
+std::vector<int> row_blocks(int64_t n, const int32_t* start, int cap, int max_rows) {
     std::vector<int> blk{0};
     int64_t r = 0;
     while (r < n) {
         const int64_t rs = r;
-        if (start[r + 1] - start[r] > kCap) {
+        if (start[r + 1] - start[r] > cap) {
             ++r;
         } else {
             int64_t cnt = 0;
-            while (r < n && r - rs < kMaxRows && cnt + (start[r + 1] - start[r]) <= kCap) {
+            while (r < n && r - rs < max_rows && cnt + (start[r + 1] - start[r]) <= cap) {
                 cnt += start[r + 1] - start[r];
                 ++r;
             }
@@ -429,7 +407,7 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         b->opt = opt;
         b->lipschitz = lipschitz;
         b->diagonal = opt.diagonal_hessian < 0 ? (diag ? 1 : 0) : (opt.diagonal_hessian ? 1 : 0);
-        std::vector<int> blk = row_blocks(n, start);
+        std::vector<int> blk = row_blocks(n, start, kCap, kMaxRows);
         b->nblocks = static_cast<int>(blk.size()) - 1;
         b->groups = groups_for(b->nblocks);
         const bool f32 = opt.precision != HGQP_PRECISION_FP64;

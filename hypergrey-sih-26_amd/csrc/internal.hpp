@@ -7,6 +7,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace hgqp {
 
@@ -44,5 +45,10 @@ int guarded(F&& body) {
         return HGQP_DEVICE_ERROR;
     }
 }
+
+// Row blocks of a CSR (host side): consecutive rows with at most `cap` stored
+// entries and at most `max_rows` rows; a row longer than `cap` is a block of its
+// own. Returns the first row of every block followed by n.
+std::vector<int> row_blocks(int64_t n, const int32_t* start, int cap, int max_rows);
 
 }  // namespace hgqp

@@ -1,0 +1,337 @@
+// Pool kernels: many box QPs resident in HBM as one block-diagonal CSR.
+//
+// Each row block belongs to one instance, so every per-instance quantity (the
+// restart test, the FP32 check, the FP64 check report) is a fixed-order
+// combination of that instance's own block partials: an instance's iterates
+// are the same whether it runs alone or beside others. Work lists of active
+// blocks are rebuilt on the device at every check, so finished instances drop
+// out of later launches and the whole period replays as one captured graph.
+//
+// Iteration scheme per instance (FP32 correction cycles anchored at FP64):
+//   anchor x_a (FP64), g_a = H x_a + c from the FP64 check, d = w = 0;
+//   FP32 FISTA steps on d with gradient g_a + H d over [l - x_a, u - x_a];
+//   FP32 check every period; when it meets the cycle's target, x = clip(x_a + d)
+//   and the FP64 check decides: verified -> done, else re-anchor at x.
+#include "pool_kernels.hpp"
+#include "sparse_pass.hpp"
+
+#include "hgqp.h"
+
+namespace hgqp {
+namespace {
+using namespace dev;
+
+constexpr int kListThreads = 1024;
+
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---- FP32 correction step over the active blocks -------------------------------
+template <int LOADS>
+__global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
+    __shared__ float stage[kWavesPerGroup][kWave * LOADS];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int len = v.counts[0];
+    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
+        const int b = uniform(v.list_active[idx]);
+        const int i = uniform(v.blk_inst[b]);
+        const PoolInst* s = v.inst + i;
+        double next_t, beta;
+        momentum_of(s->t[parity], &next_t, &beta);
+        StepEpilogue<float> epi;
+        epi.gat = s->use_x[parity] ? v.d[parity] : v.w[parity];
+        epi.xin = v.d[parity];
+        epi.c = v.c32;
+        epi.lo = v.lo32;
+        epi.up = v.up32;
+        epi.xout = v.d[parity ^ 1];
+        epi.wout = v.w[parity ^ 1];
+        epi.lipschitz = float(s->lipschitz);
+        epi.beta = float(beta);
+        epi.dot = 0;
+        row_block<float, false, LOADS>(v.start, v.col, v.val32, epi.gat, v.blk_r0[b], v.blk_r1[b], stage[wave], lane,
+                                       epi);
+        const double dot = wave_sum(epi.dot);
+        if (lane == 0) v.partials[b] = dot;
+    }
+}
+
+// Sum over an instance's blocks, in block order, by one workgroup.
+__device__ inline double slot_sum(const double* part, int b0, int nblk, double* red) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    double s = 0;
+    for (int k = threadIdx.x; k < nblk; k += kGroup) s += part[b0 + k];
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    double tot = 0;
+    for (int w = 0; w < kWavesPerGroup; ++w) tot += red[w];
+    return tot;
+}
+
+__device__ inline double slot_max(const double* part, int b0, int nblk, double* red) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    double s = 0;
+    for (int k = threadIdx.x; k < nblk; k += kGroup) s = fmax(s, part[b0 + k]);
+    s = wave_max(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    double tot = 0;
+    for (int w = 0; w < kWavesPerGroup; ++w) tot = fmax(tot, red[w]);
+    return tot;
+}
+
+// Restart test and momentum of every active slot after a step (one workgroup per slot).
+__global__ __launch_bounds__(kGroup) void pool_finalize_step(PoolView v, int parity) {
+    __shared__ double red[kWavesPerGroup];
+    PoolInst* s = v.inst + blockIdx.x;
+    if (s->phase != kSlotActive) return;
+    const double tot = slot_sum(v.partials, s->b0, s->nblk, red);
+    if (threadIdx.x != 0) return;
+    double next_t, beta;
+    momentum_of(s->t[parity], &next_t, &beta);
+    const int restart = v.restart && tot > 0;
+    s->t[parity ^ 1] = restart ? 1.0 : next_t;
+    s->use_x[parity ^ 1] = restart;
+    s->restarts += restart;
+    s->iter += 1;
+}
+
+// ---- FP32 check of the correction problem ---------------------------------------
+struct PoolCheck32 {
+    const float* __restrict__ d;
+    const float* __restrict__ c;
+    const float* __restrict__ lo;
+    const float* __restrict__ up;
+    const double* __restrict__ xa;
+    double stat;
+    __device__ inline void operator()(int r, float hd) {
+        const float dv = d[r];
+        const float g = hd + c[r];
+        const double res = fabs(double(dv - clip(dv - g, lo[r], up[r]))) / (1 + fabs(xa[r] + double(dv)));
+        stat = (isfinite(res) && isfinite(g)) ? fmax(stat, res) : INFINITY;
+    }
+};
+
+template <int LOADS>
+__global__ __launch_bounds__(kGroup) void pool_check32(PoolView v, int parity) {
+    __shared__ float stage[kWavesPerGroup][kWave * LOADS];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int len = v.counts[0];
+    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
+        const int b = uniform(v.list_active[idx]);
+        PoolCheck32 epi{v.d[parity], v.c32, v.lo32, v.up32, v.x, 0.0};
+        row_block<float, false, LOADS>(v.start, v.col, v.val32, v.d[parity], v.blk_r0[b], v.blk_r1[b], stage[wave],
+                                       lane, epi);
+        const double m = wave_max(epi.stat);
+        if (lane == 0) v.partials[b] = m;
+    }
+}
+
+__global__ __launch_bounds__(kGroup) void pool_finalize32(PoolView v) {
+    __shared__ double red[kWavesPerGroup];
+    PoolInst* s = v.inst + blockIdx.x;
+    if (s->phase != kSlotActive) return;
+    const double stat = slot_max(v.partials, s->b0, s->nblk, red);
+    if (threadIdx.x != 0) return;
+    s->stat32 = stat;
+    s->checks += 1;
+    if (!(stat <= s->best_stat * 0.5)) s->stall += 1;
+    else s->stall = 0;
+    if (stat < s->best_stat) s->best_stat = stat;
+    if (!(stat > s->target) || s->stall >= 2 || s->iter >= s->limit) s->phase = kSlotVerify;
+}
+
+// ---- work lists -------------------------------------------------------------------
+__device__ inline bool selected(int phase, int which) {
+    return which == 0 ? phase == kSlotActive : (phase == kSlotInit || phase == kSlotVerify);
+}
+
+// One workgroup: exclusive prefix of the selected slots' block counts in slot order.
+__global__ __launch_bounds__(kListThreads) void pool_list_count(PoolView v, int which) {
+    __shared__ int sums[kListThreads];
+    const int t = threadIdx.x;
+    const int per = (v.max_inst + kListThreads - 1) / kListThreads;
+    const int a = t * per, e = min(v.max_inst, a + per);
+    int local = 0;
+    for (int i = a; i < e; ++i) local += selected(v.inst[i].phase, which) ? v.inst[i].nblk : 0;
+    sums[t] = local;
+    __syncthreads();
+    for (int off = 1; off < kListThreads; off <<= 1) {  // inclusive Hillis-Steele scan
+        int add = t >= off ? sums[t - off] : 0;
+        __syncthreads();
+        sums[t] += add;
+        __syncthreads();
+    }
+    int run = t ? sums[t - 1] : 0;
+    for (int i = a; i < e; ++i) {
+        v.offsets[i] = run;
+        run += selected(v.inst[i].phase, which) ? v.inst[i].nblk : 0;
+    }
+    if (t == kListThreads - 1) v.counts[which] = sums[t];
+}
+
+__global__ __launch_bounds__(kGroup) void pool_list_expand(PoolView v, int which) {
+    const PoolInst* s = v.inst + blockIdx.x;
+    if (!selected(s->phase, which)) return;
+    int* list = which == 0 ? v.list_active : v.list_verify;
+    const int o = v.offsets[blockIdx.x];
+    for (int k = threadIdx.x; k < s->nblk; k += kGroup) list[o + k] = s->b0 + k;
+}
+
+// ---- FP64 verification of new and converging slots ----------------------------------
+// x = clip(0, l, u) for a new slot (accelerated_qp.cpp:34), x = clip(x_a + d) otherwise.
+__global__ __launch_bounds__(kGroup) void pool_apply(PoolView v, int parity) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int len = v.counts[1];
+    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
+        const int b = uniform(v.list_verify[idx]);
+        const int init = v.inst[uniform(v.blk_inst[b])].phase == kSlotInit;
+        for (int r = v.blk_r0[b] + lane; r < v.blk_r1[b]; r += kWave) {
+            const double base = init ? 0.0 : v.x[r] + double(v.d[parity][r]);
+            v.x[r] = clip(base, v.lo[r], v.up[r]);
+        }
+    }
+}
+
+template <int LOADS>
+__global__ __launch_bounds__(kGroup) void pool_check64(PoolView v) {
+    __shared__ double stage[kWavesPerGroup][kWave * LOADS];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int len = v.counts[1];
+    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
+        const int b = uniform(v.list_verify[idx]);
+        const int i = uniform(v.blk_inst[b]);
+        CheckEpilogue epi;
+        epi.x = v.x;
+        epi.c = v.c;
+        epi.lo = v.lo;
+        epi.up = v.up;
+        epi.start = v.start;
+        epi.col = v.col;
+        epi.val = v.val;
+        epi.gout = v.g;
+        epi.diagonal = v.inst[i].diagonal;
+        epi.stat = 0;
+        epi.viol = 0;
+        epi.value = epi.tangent = epi.separable = DD{0, 0};
+        epi.bound_inf = epi.bad = 0;
+        row_block<double, false, LOADS>(v.start, v.col, v.val, v.x, v.blk_r0[b], v.blk_r1[b], stage[wave], lane, epi);
+        const double stat = wave_max(epi.stat), viol = wave_max(epi.viol);
+        const DD val = wave_dd(epi.value), tg = wave_dd(epi.tangent), sp = wave_dd(epi.separable);
+        const int flags = wave_or(epi.bound_inf | (epi.bad << 1));
+        if (lane == 0) {
+            double* o = v.partials + size_t(b) * kPartial;
+            o[0] = stat; o[1] = viol; o[2] = val.hi; o[3] = val.lo; o[4] = tg.hi; o[5] = tg.lo;
+            o[6] = sp.hi; o[7] = sp.lo; o[8] = flags; o[9] = 0;
+        }
+    }
+}
+
+// Verdict per new / converging slot: done (verified, non-finite, limit) or a new
+// FP32 correction cycle anchored at the checked x.
+__global__ __launch_bounds__(kGroup) void pool_finalize64(PoolView v) {
+    __shared__ double red[kWavesPerGroup][kPartial];
+    PoolInst* s = v.inst + blockIdx.x;
+    const int phase = s->phase;
+    if (phase != kSlotInit && phase != kSlotVerify) return;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    double acc[kPartial] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < s->nblk; k += kGroup) combine_partial(acc, v.partials + size_t(s->b0 + k) * kPartial);
+    // fixed-shape combination across lanes and waves
+    for (int m = 32; m >= 1; m >>= 1) {
+        double o[kPartial];
+        for (int q = 0; q < 9; ++q) o[q] = shfl_xor(acc[q], m);
+        combine_partial(acc, o);
+    }
+    if (lane == 0)
+        for (int q = 0; q < 9; ++q) red[wave][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < kWavesPerGroup; ++w) combine_partial(red[0], red[w]);
+    const Verdict vd = verdict(red[0], s->sign, s->offset, s->diagonal, s->tol);
+    s->objective = vd.objective;
+    s->primal_residual = vd.primal;
+    s->stationarity = vd.stationarity;
+    s->dual_bound = vd.bound;
+    s->relative_gap = vd.gap;
+    s->verified = vd.verified;
+    s->finite = vd.finite;
+    s->checks += 1;
+    if (vd.verified) { s->phase = kSlotDone; s->status = HGQP_STATUS_OPTIMAL; return; }
+    if (!vd.finite) { s->phase = kSlotDone; s->status = HGQP_STATUS_NUMERICAL_ERROR; return; }
+    if (s->iter >= s->limit) { s->phase = kSlotDone; s->status = HGQP_STATUS_ITERATION_LIMIT; return; }
+    s->cycle = phase == kSlotInit ? 0 : s->cycle + 1;
+    s->target = s->cycle == 0 ? v.switch_stat * s->tol : 0.5 * s->tol;
+    s->t[0] = s->t[1] = 1.0;
+    s->use_x[0] = s->use_x[1] = 0;
+    s->stall = 0;
+    s->best_stat = INFINITY;
+    s->phase = kSlotActive;
+}
+
+// New correction cycle for the slots that finalize64 re-activated.
+__global__ __launch_bounds__(kGroup) void pool_anchor(PoolView v, int parity) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int len = v.counts[1];
+    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
+        const int b = uniform(v.list_verify[idx]);
+        if (v.inst[uniform(v.blk_inst[b])].phase != kSlotActive) continue;
+        for (int r = v.blk_r0[b] + lane; r < v.blk_r1[b]; r += kWave) {
+            v.c32[r] = float(v.g[r]);
+            v.lo32[r] = float(v.lo[r] - v.x[r]);
+            v.up32[r] = float(v.up[r] - v.x[r]);
+            v.d[parity][r] = 0.0f;
+            v.w[parity][r] = 0.0f;
+        }
+    }
+}
+
+int grid_for(int max_blocks) {
+    int g = (max_blocks + kWavesPerGroup - 1) / kWavesPerGroup;
+    return g < kMaxGroups ? (g < 1 ? 1 : g) : kMaxGroups;
+}
+
+template <int LOADS>
+void period_impl(const PoolView& v, int max_blocks, int steps, hipStream_t stream) {
+    const dim3 grid(grid_for(max_blocks)), block(kGroup), slots(v.max_inst);
+    int parity = 0;
+    for (int k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(pool_step<LOADS>, grid, block, 0, stream, v, parity);
+        hipLaunchKernelGGL(pool_finalize_step, slots, block, 0, stream, v, parity);
+        parity ^= 1;
+    }
+    hipLaunchKernelGGL(pool_check32<LOADS>, grid, block, 0, stream, v, parity);
+    hipLaunchKernelGGL(pool_finalize32, slots, block, 0, stream, v);
+    hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 1);
+    hipLaunchKernelGGL(pool_list_expand, slots, block, 0, stream, v, 1);
+    hipLaunchKernelGGL(pool_apply, grid, block, 0, stream, v, parity);
+    hipLaunchKernelGGL(pool_check64<LOADS>, grid, block, 0, stream, v);
+    hipLaunchKernelGGL(pool_finalize64, slots, block, 0, stream, v);
+    hipLaunchKernelGGL(pool_anchor, grid, block, 0, stream, v, parity);
+    hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 0);
+    hipLaunchKernelGGL(pool_list_expand, slots, block, 0, stream, v, 0);
+}
+
+}  // namespace
+
+void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps, hipStream_t stream) {
+    if (loads == 4) period_impl<4>(v, max_blocks, steps, stream);
+    else if (loads == 16) period_impl<16>(v, max_blocks, steps, stream);
+    else period_impl<8>(v, max_blocks, steps, stream);
+}
+
+void launch_pool_lists(const PoolView& v, hipStream_t stream) {
+    hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 0);
+    hipLaunchKernelGGL(pool_list_expand, dim3(v.max_inst), dim3(kGroup), 0, stream, v, 0);
+}
+
+void launch_pool_steps(const PoolView& v, int loads, int max_blocks, int reps, hipStream_t stream) {
+    const dim3 grid(grid_for(max_blocks)), block(kGroup);
+    for (int k = 0; k < reps; ++k) {
+        if (loads == 4) hipLaunchKernelGGL(pool_step<4>, grid, block, 0, stream, v, k & 1);
+        else if (loads == 16) hipLaunchKernelGGL(pool_step<16>, grid, block, 0, stream, v, k & 1);
+        else hipLaunchKernelGGL(pool_step<8>, grid, block, 0, stream, v, k & 1);
+    }
+}
+
+}  // namespace hgqp

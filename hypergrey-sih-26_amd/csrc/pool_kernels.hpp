@@ -1,0 +1,89 @@
+// Device layout and launchers of the instance pool (many box QPs resident at
+// once; batched and served solves).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "box_kernels.hpp"
+
+namespace hgqp {
+
+// Phases of a pool slot; transitions happen in the finalize kernels.
+enum PoolPhase : int {
+    kSlotFree = 0,
+    kSlotInit = 1,    // admitted: x0 and the iteration-0 FP64 check are pending
+    kSlotActive = 2,  // FP32 correction steps
+    kSlotVerify = 3,  // FP32 criterion met: FP64 check pending
+    kSlotDone = 4,    // finished; report and x valid until the host frees the slot
+};
+
+struct PoolInst {
+    double lipschitz;
+    double tol;
+    double t[2];
+    int32_t use_x[2];
+    int32_t phase;
+    int32_t status;  // HGQP_STATUS_* once done
+    int64_t iter;    // local iterations done
+    int64_t limit;   // local iteration limit
+    int32_t cycle;   // FP32 correction cycles started
+    int32_t restarts;
+    int32_t stall;
+    int32_t checks;
+    double best_stat, target, stat32;
+    double objective, primal_residual, stationarity, dual_bound, relative_gap;
+    int32_t verified, finite;
+    double sign, offset;
+    int32_t diagonal;
+    int32_t b0, nblk;  // block range
+    int32_t r0, n;     // row range
+    int32_t done_iter_pad;
+};
+
+struct PoolView {
+    // matrix (block-diagonal concatenation; columns carry the instance row offset)
+    const int* start;
+    const int* col;
+    const double* val;
+    const float* val32;
+    // blocks
+    const int* blk_r0;
+    const int* blk_r1;
+    const int* blk_inst;
+    // rows
+    const double* c;
+    const double* lo;
+    const double* up;
+    double* x;     // FP64 iterate / anchor
+    double* g;     // FP64 gradient of the last check
+    float* c32;    // anchor gradient
+    float* lo32;   // lower - anchor
+    float* up32;   // upper - anchor
+    float* d[2];   // FP32 correction, ping-pong
+    float* w[2];   // FP32 extrapolated correction, ping-pong
+    // slots and lists
+    PoolInst* inst;
+    int max_inst;
+    int* list_active;
+    int* list_verify;
+    int* counts;    // [0] active length, [1] verify length
+    int* offsets;   // [max_inst] scratch for list building
+    double* partials;  // per block
+    double switch_stat;
+    int restart;       // 1: gradient restart, 0: reference momentum
+};
+
+// One check period: `steps` FP32 correction steps of every active slot, the
+// FP32 check, then the FP64 check of slots that are new or met the FP32
+// criterion, retirement or re-anchoring, and the next active list. Uses
+// `parity` for the first step; `steps` must be even for graph replay.
+void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps, hipStream_t stream);
+
+// Builds the active list from the slot table (after host admission edits).
+void launch_pool_lists(const PoolView& v, hipStream_t stream);
+
+// Times `reps` FP32 step launches over the current active list (autotuning).
+void launch_pool_steps(const PoolView& v, int loads, int max_blocks, int reps, hipStream_t stream);
+
+}  // namespace hgqp

@@ -104,7 +104,37 @@ class Planted(C.Structure):
                 ("lower", C.c_double), ("upper", C.c_double)]
 
 
+class PoolOptions(C.Structure):
+    _fields_ = [("max_rows", C.c_int64), ("max_nnz", C.c_int64), ("max_instances", C.c_int32),
+                ("check_every", C.c_int32), ("block_loads", C.c_int32), ("block_rows", C.c_int32),
+                ("use_graph", C.c_int32), ("momentum", C.c_int32), ("fp32_switch", C.c_double)]
+
+
+class PoolResult(C.Structure):
+    _fields_ = [("tag", C.c_int64), ("slot", C.c_int32), ("status", C.c_int32), ("iterations", C.c_uint64),
+                ("restarts", C.c_int32), ("checks", C.c_int32), ("cycles", C.c_int32), ("reserved", C.c_int32),
+                ("check", CheckReport)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("check", "reserved")}
+        d["status"] = STATUS[self.status]
+        d.update(self.check.as_dict())
+        return d
+
+
+class Served(C.Structure):
+    _fields_ = [("result", PoolResult), ("queued_seconds", C.c_double), ("latency_seconds", C.c_double)]
+
+    def as_dict(self):
+        d = self.result.as_dict()
+        d["queued_seconds"] = self.queued_seconds
+        d["latency_seconds"] = self.latency_seconds
+        return d
+
+
 P_I32, P_I64, P_F64 = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+_ADMIT = [C.c_int64, C.c_int64, P_I32, P_I32, P_F64, P_F64, P_F64, P_F64, C.c_double, C.c_int32, C.c_double,
+          C.c_double, C.c_uint64, C.c_int64, P_F64]
 
 # Exported symbols and their argument types; tests check that every symbol
 # declared in include/hgqp.h is exported.
@@ -125,6 +155,19 @@ SIGNATURES = {
     "hgqp_box_info_get": ([C.c_void_p, C.POINTER(BoxInfo)], C.c_int),
     "hgqp_solve_qp": ([C.POINTER(Problem), C.POINTER(QpOptions), C.POINTER(QpResult), P_F64], C.c_int),
     "hgqp_csr_lipschitz": ([C.c_int64, P_I32, P_F64], C.c_double),
+    "hgqp_pool_options_default": ([C.POINTER(PoolOptions)], None),
+    "hgqp_pool_bytes": ([C.POINTER(PoolOptions), P_I64], C.c_int),
+    "hgqp_pool_create": ([C.POINTER(PoolOptions), C.POINTER(C.c_void_p)], C.c_int),
+    "hgqp_pool_admit": ([C.c_void_p] + _ADMIT + [P_I32], C.c_int),
+    "hgqp_pool_run": ([C.c_void_p, C.c_int32], C.c_int),
+    "hgqp_pool_collect": ([C.c_void_p, C.POINTER(PoolResult), C.c_int32, P_I32, P_I32], C.c_int),
+    "hgqp_pool_autotune": ([C.c_void_p, P_I32, P_I32], C.c_int),
+    "hgqp_pool_destroy": ([C.c_void_p], C.c_int),
+    "hgqp_server_start": ([C.POINTER(PoolOptions), C.POINTER(C.c_void_p)], C.c_int),
+    "hgqp_server_submit": ([C.c_void_p] + _ADMIT[0:], C.c_int),
+    "hgqp_server_poll": ([C.c_void_p, C.POINTER(Served), C.c_int32, P_I32], C.c_int),
+    "hgqp_server_pending": ([C.c_void_p, P_I64], C.c_int),
+    "hgqp_server_stop": ([C.c_void_p], C.c_int),
     "hgqp_gen_tridiagonal": ([C.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64], C.c_int),
     "hgqp_gen_planted": ([C.POINTER(Planted), P_I64, P_I32, P_I32, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64],
                          C.c_int),
@@ -300,6 +343,24 @@ def gen_planted(n, draws, seed, lower=-1.0, upper=1.0):
                 objective=f.value)
 
 
+def gen_batch(count, seed, n_min=1000, n_max=100000, draws_min=4, draws_max=32, lower=(-2.0, -0.5),
+              upper=(0.5, 2.0)):
+    """Uneven planted batch: n log-uniform in [n_min, n_max], draws per row uniform
+    in {draws_min..draws_max}, per-instance scalar bounds lower ~ U(lower),
+    upper ~ U(upper); instance k uses generator seed (seed, k)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        draws = int(rng.integers(draws_min, draws_max + 1))
+        lo = float(rng.uniform(*lower))
+        up = float(rng.uniform(*upper))
+        d = gen_planted(n, min(draws, n - 1), (int(seed) << 20) + k, lo, up)
+        d["draws"] = draws
+        out.append(d)
+    return out
+
+
 def lower_triplets(start, index, value):
     """Lower-triangle triplets (row, col, value), col <= row, of a symmetric CSR."""
     n = len(start) - 1
@@ -313,3 +374,150 @@ def lipschitz(start, value):
     start = np.ascontiguousarray(start, dtype=np.int32)
     value = np.ascontiguousarray(value, dtype=np.float64)
     return lib().hgqp_csr_lipschitz(len(start) - 1, _ptr(start, C.c_int32), _ptr(value, C.c_double))
+
+
+def pool_options(**kw):
+    o = PoolOptions()
+    lib().hgqp_pool_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def pool_bytes(**kw):
+    b = C.c_int64()
+    _raise(lib().hgqp_pool_bytes(C.byref(pool_options(**kw)), C.byref(b)))
+    return b.value
+
+
+def _instance_args(inst, lipschitz_value, maximize, offset, tolerance, iteration_limit, tag, x_out):
+    """Arguments of hgqp_pool_admit / hgqp_server_submit for an instance dict with
+    start, index, value (sign-normalized mirrored CSR), cost, lower, upper."""
+    keep = [np.ascontiguousarray(inst["start"], dtype=np.int32), np.ascontiguousarray(inst["index"], dtype=np.int32),
+            np.ascontiguousarray(inst["value"], dtype=np.float64), np.ascontiguousarray(inst["cost"], dtype=np.float64),
+            np.ascontiguousarray(inst["lower"], dtype=np.float64), np.ascontiguousarray(inst["upper"], dtype=np.float64)]
+    s, i, v, c, lo, up = keep
+    n = len(c)
+    args = [n, len(v), _ptr(s, C.c_int32), _ptr(i, C.c_int32), _ptr(v, C.c_double), _ptr(c, C.c_double),
+            _ptr(lo, C.c_double), _ptr(up, C.c_double), float(lipschitz_value), int(bool(maximize)), float(offset),
+            float(tolerance), int(iteration_limit), int(tag), _ptr(x_out, C.c_double)]
+    return args, keep
+
+
+class Pool:
+    """An instance pool (hgqp_pool_*): many box QPs advanced together."""
+
+    def __init__(self, **options):
+        self.handle = C.c_void_p()
+        _raise(lib().hgqp_pool_create(C.byref(pool_options(**options)), C.byref(self.handle)))
+        self.out = {}  # tag -> x buffer (kept alive until collected)
+
+    def admit(self, inst, tag, tolerance=1e-6, iteration_limit=20000, lipschitz_value=None, maximize=False,
+              offset=0.0):
+        """Returns the slot, or None when the pool is full (HGQP_BUSY)."""
+        L = lipschitz_value if lipschitz_value is not None else lipschitz(inst["start"], inst["value"])
+        x = np.zeros(len(inst["cost"]), dtype=np.float64)
+        args, keep = _instance_args(inst, L, maximize, offset, tolerance, iteration_limit, tag, x)
+        slot = C.c_int32(-1)
+        rc = lib().hgqp_pool_admit(self.handle, *args, C.byref(slot))
+        if rc == BUSY:
+            return None
+        _raise(rc)
+        self.out[int(tag)] = x
+        return slot.value
+
+    def run(self, periods=1):
+        _raise(lib().hgqp_pool_run(self.handle, int(periods)))
+
+    def collect(self):
+        """Finished instances as dicts with 'primal'; also returns the resident count."""
+        out = []
+        buf = (PoolResult * 256)()
+        while True:
+            cnt, active = C.c_int32(), C.c_int32()
+            _raise(lib().hgqp_pool_collect(self.handle, buf, 256, C.byref(cnt), C.byref(active)))
+            for k in range(cnt.value):
+                d = buf[k].as_dict()
+                d["primal"] = self.out.pop(d["tag"])
+                out.append(d)
+            if cnt.value < 256:
+                return out, active.value
+
+    def autotune(self):
+        l, r = C.c_int32(), C.c_int32()
+        _raise(lib().hgqp_pool_autotune(self.handle, C.byref(l), C.byref(r)))
+        return l.value, r.value
+
+    def solve_all(self, max_periods=100000):
+        """Runs periods until nothing is resident; returns all results."""
+        results = []
+        for _ in range(max_periods):
+            self.run(1)
+            got, active = self.collect()
+            results += got
+            if active == 0:
+                return results
+        raise RuntimeError("pool did not finish")
+
+    def close(self):
+        if self.handle:
+            _raise(lib().hgqp_pool_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (hgqp_server_*): submit from any thread, poll results."""
+
+    def __init__(self, **options):
+        self.handle = C.c_void_p()
+        _raise(lib().hgqp_server_start(C.byref(pool_options(**options)), C.byref(self.handle)))
+        self.out = {}
+
+    def submit(self, inst, tag, tolerance=1e-6, iteration_limit=20000, lipschitz_value=None, maximize=False,
+               offset=0.0):
+        L = lipschitz_value if lipschitz_value is not None else lipschitz(inst["start"], inst["value"])
+        x = np.zeros(len(inst["cost"]), dtype=np.float64)
+        args, keep = _instance_args(inst, L, maximize, offset, tolerance, iteration_limit, tag, x)
+        self.out[int(tag)] = x
+        _raise(lib().hgqp_server_submit(self.handle, *args))
+
+    def poll(self):
+        out = []
+        buf = (Served * 256)()
+        while True:
+            cnt = C.c_int32()
+            _raise(lib().hgqp_server_poll(self.handle, buf, 256, C.byref(cnt)))
+            for k in range(cnt.value):
+                d = buf[k].as_dict()
+                d["primal"] = self.out.pop(d["tag"])
+                out.append(d)
+            if cnt.value < 256:
+                return out
+
+    def pending(self):
+        p = C.c_int64()
+        _raise(lib().hgqp_server_pending(self.handle, C.byref(p)))
+        return p.value
+
+    def stop(self):
+        if self.handle:
+            _raise(lib().hgqp_server_stop(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -230,6 +230,100 @@ int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, h
  * (accelerated_qp.cpp:25-26), for callers that assemble the CSR themselves. */
 double hgqp_csr_lipschitz(int64_t n, const int32_t* start, const double* value);
 
+/* ---- instance pool: many box QPs resident at once ---------------------- */
+/* No counterpart in the reference, which solves one QP per process
+ * (main.cpp:146; backend/limits.py:13). A pool keeps up to max_instances box
+ * QPs resident in one HBM budget (max_rows, max_nnz) as a block-diagonal CSR
+ * and advances all of them together: FP32 correction cycles anchored at FP64
+ * iterates, an FP64 check (qp_admm.cpp:63-117 restated) deciding each one, and
+ * finished instances retiring from later launches. An instance's result does
+ * not depend on what else is resident. Not thread-safe (see hgqp_server). */
+typedef struct hgqp_pool hgqp_pool;
+
+typedef struct hgqp_pool_options {
+    int64_t max_rows;        /* rows of all resident instances (+1 per instance) */
+    int64_t max_nnz;         /* stored entries of all resident instances         */
+    int32_t max_instances;   /* slots                                            */
+    int32_t check_every;     /* steps per check period, even (default 10)        */
+    int32_t block_loads;     /* row-block entries per lane: 4, 8 or 16; 0 = tune */
+    int32_t block_rows;      /* rows per row block: 64 or 256; 0 = tune          */
+    int32_t use_graph;       /* 1: replay each period as a captured HIP graph    */
+    int32_t momentum;        /* HGQP_MOMENTUM_RESTART (default) or _REFERENCE    */
+    double fp32_switch;      /* first FP32 cycle ends at this multiple of the
+                                tolerance (default 10)                           */
+} hgqp_pool_options;
+
+void hgqp_pool_options_default(hgqp_pool_options* options);
+
+/* Device bytes a pool with these options reserves (the HBM budget it uses). */
+int hgqp_pool_bytes(const hgqp_pool_options* options, int64_t* bytes);
+
+int hgqp_pool_create(const hgqp_pool_options* options, hgqp_pool** out);
+
+/* Admits one instance in the form of hgqp_box_create (mirrored, sign-
+ * normalized CSR; sign-normalized cost) plus the original sense and offset for
+ * the report. x_out (length n) receives the solution when the instance is
+ * collected and must stay valid until then. Returns HGQP_BUSY when no slot or
+ * no contiguous room in the budget is free. The instance starts at the next
+ * period boundary. */
+int hgqp_pool_admit(hgqp_pool* pool, int64_t n, int64_t nnz, const int32_t* start, const int32_t* index,
+                    const double* value, const double* cost, const double* lower, const double* upper,
+                    double lipschitz, int32_t maximize, double offset, double tolerance, uint64_t iteration_limit,
+                    int64_t tag, double* x_out, int32_t* slot);
+
+/* Enqueues `periods` check periods for every resident instance (asynchronous). */
+int hgqp_pool_run(hgqp_pool* pool, int32_t periods);
+
+typedef struct hgqp_pool_result {
+    int64_t tag;
+    int32_t slot;
+    int32_t status;          /* HGQP_STATUS_*                                      */
+    uint64_t iterations;     /* FP32 correction steps run by this instance         */
+    int32_t restarts;
+    int32_t checks;          /* FP32 and FP64 checks                               */
+    int32_t cycles;          /* FP32 correction cycles                             */
+    int32_t reserved;
+    hgqp_check_report check; /* the deciding FP64 check                            */
+} hgqp_pool_result;
+
+/* Waits for the enqueued periods, copies the solutions of finished instances
+ * to their x_out buffers, frees their slots and returns up to `capacity`
+ * results (more are returned by later calls). *active receives the number of
+ * instances still resident. */
+int hgqp_pool_collect(hgqp_pool* pool, hgqp_pool_result* results, int32_t capacity, int32_t* count,
+                      int32_t* active);
+
+/* Chooses block_loads and block_rows by timing FP32 steps over the resident
+ * instances (when either was 0 at creation); a no-op otherwise. */
+int hgqp_pool_autotune(hgqp_pool* pool, int32_t* block_loads, int32_t* block_rows);
+
+int hgqp_pool_destroy(hgqp_pool* pool);
+
+/* ---- serving loop: a pool driven by its own thread ---------------------- */
+/* No counterpart in the reference (its service admits one request at a time
+ * and answers 429 otherwise, backend/limits.py:24-26). Requests are copied at
+ * submission; the loop admits them in arrival order as slots and budget free
+ * up (continuous batching), runs periods while anything is resident and hands
+ * finished results to hgqp_server_poll. submit and poll are thread-safe. */
+typedef struct hgqp_server hgqp_server;
+
+typedef struct hgqp_served {
+    hgqp_pool_result result;
+    double queued_seconds;   /* submission to admission                          */
+    double latency_seconds;  /* submission to result                             */
+} hgqp_served;
+
+int hgqp_server_start(const hgqp_pool_options* options, hgqp_server** out);
+int hgqp_server_submit(hgqp_server* server, int64_t n, int64_t nnz, const int32_t* start, const int32_t* index,
+                       const double* value, const double* cost, const double* lower, const double* upper,
+                       double lipschitz, int32_t maximize, double offset, double tolerance,
+                       uint64_t iteration_limit, int64_t tag, double* x_out);
+int hgqp_server_poll(hgqp_server* server, hgqp_served* results, int32_t capacity, int32_t* count);
+/* Requests submitted and not yet returned by poll. */
+int hgqp_server_pending(hgqp_server* server, int64_t* pending);
+/* Finishes resident and queued work, then stops the thread. */
+int hgqp_server_stop(hgqp_server* server);
+
 /* ---- planted instance generators (inputs for tests and benchmarks) ------ */
 
 /* The reference's seeded tridiagonal family (scale_benchmark.cpp:13-18):

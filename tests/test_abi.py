@@ -38,7 +38,8 @@ def test_struct_layouts_match_header(hg, tmp_path):
     pairs = {"hgqp_box_options": hg.BoxOptions, "hgqp_check_report": hg.CheckReport,
              "hgqp_solve_options": hg.SolveOptions, "hgqp_box_report": hg.BoxReport, "hgqp_box_info": hg.BoxInfo,
              "hgqp_problem": hg.Problem, "hgqp_qp_options": hg.QpOptions, "hgqp_qp_result": hg.QpResult,
-             "hgqp_planted": hg.Planted}
+             "hgqp_planted": hg.Planted, "hgqp_pool_options": hg.PoolOptions, "hgqp_pool_result": hg.PoolResult,
+             "hgqp_served": hg.Served}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "hgqp.h"', "int main(void){"]
     for cname, cls in pairs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
