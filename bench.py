@@ -103,6 +103,154 @@ def cpu_baseline(inst, iterations_needed, threads):
             "setup_seconds": t_setup, "ms_per_iteration": per_iter * 1e3}
 
 
+def batch_workload(args):
+    """512 uneven planted instances solved together in one pool (FP32 correction
+    cycles, FP64 verification), against the plain FP64 single-instance loop."""
+    batch = hgqp.gen_batch(args.instances, args.seed, 1000, 100000, 4, 32)
+    rows = sum(d["n"] + 1 for d in batch)
+    nnz = sum(len(d["value"]) for d in batch)
+    Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in batch]
+    print(f"# batch: {len(batch)} instances, rows {rows}, nnz {nnz}", file=sys.stderr, flush=True)
+    pool = hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
+                     check_every=args.check_every)
+    times, results = [], None
+    tuned = None
+    try:
+        for rep in range(args.warmup + args.steps):
+            for k, d in enumerate(batch):
+                assert pool.admit(d, tag=k, tolerance=KKT_TOL, lipschitz_value=Ls[k]) is not None
+            if tuned is None:
+                tuned = pool.autotune()
+            t0 = time.perf_counter()
+            res = pool.solve_all()
+            dt = time.perf_counter() - t0
+            if rep >= args.warmup:
+                times.append(dt)
+                results = res
+    finally:
+        pool.close()
+    res = {r["tag"]: r for r in results}
+    ok = all(r["status"] == "optimal" and r["optimality_verified"] for r in res.values())
+    worst_x = max(float(np.abs(res[k]["primal"] - d["optimum"]).max()) for k, d in enumerate(batch))
+    worst_obj = max(abs(res[k]["objective"] - d["objective"]) / (1 + abs(d["objective"])) for k, d in enumerate(batch))
+    iters = np.array([res[k]["iterations"] for k in range(len(batch))])
+    wall = float(np.median(times))
+    # plain FP64 single-instance form, one instance at a time, same instances
+    plain_n = min(len(batch), args.plain_sample)
+    boxes = [hgqp.Box(d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"], Ls[k])
+             for k, d in enumerate(batch[:plain_n])]
+    try:
+        for b in boxes[:4]:
+            b.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60)
+            b.reset()
+        t0 = time.perf_counter()
+        plain = [b.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60) for b in boxes]
+        plain_wall = time.perf_counter() - t0
+    finally:
+        for b in boxes:
+            b.close()
+    plain_rate = plain_n / plain_wall
+    return {
+        "metric": METRIC, "value": round(len(batch) / wall, 2), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded planted batch, admitted to HBM before timing)",
+        "config": {"workload": f"batched: {len(batch)} planted box QPs, n log-uniform in [1e3, 1e5], "
+                               f"draws/row uniform in 4..32, bounds l~U(-2,-0.5), u~U(0.5,2), KKT<=1e-6, "
+                               f"one pool, graph-replayed check periods; one step = the whole batch",
+                   "rows": rows, "nnz": nnz, "autotuned_block_loads": tuned[0], "autotuned_block_rows": tuned[1],
+                   "check_every": args.check_every},
+        "all_verified": ok, "worst_max_abs_err_vs_planted_x": worst_x, "worst_objective_rel_err_vs_planted": worst_obj,
+        "iterations": {"min": int(iters.min()), "median": float(np.median(iters)), "max": int(iters.max())},
+        "plain_fp64_single_instance": {"solves_per_s": round(plain_rate, 2), "instances_timed": plain_n,
+                                       "all_verified": all(p["status"] == "optimal" for p in plain)},
+        "speedup_vs_plain_fp64": round(len(batch) / wall / plain_rate, 2),
+        "device": hgqp.device_name(),
+    }
+
+
+def serve_workload(args):
+    """Poisson arrival trace served by the continuous-batching loop, against the
+    plain one-at-a-time FP64 loop on the same trace (both in real time)."""
+    import threading
+    rng = np.random.default_rng(args.seed)
+    distinct = [hgqp.gen_planted(int(round(np.exp(rng.uniform(np.log(1000), np.log(200000))))), 8,
+                                 (args.seed << 20) + k, -1.0, 1.0) for k in range(args.distinct)]
+    Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in distinct]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
+    pick = rng.integers(0, len(distinct), size=args.requests)
+    print(f"# serve: {args.requests} requests at {args.rate}/s over {arrivals[-1]:.1f}s; "
+          f"{len(distinct)} distinct instances", file=sys.stderr, flush=True)
+
+    def pool_server():
+        srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
+                          check_every=args.check_every)
+        done = {}
+        t0 = time.perf_counter()
+        try:
+            def producer():
+                for j in range(args.requests):
+                    delay = t0 + arrivals[j] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    k = int(pick[j])
+                    srv.submit(distinct[k], tag=j, tolerance=KKT_TOL, lipschitz_value=Ls[k])
+            th = threading.Thread(target=producer)
+            th.start()
+            while len(done) < args.requests:
+                for r in srv.poll():
+                    done[r["tag"]] = r
+                time.sleep(0.0005)
+            th.join()
+            wall = time.perf_counter() - t0
+        finally:
+            srv.stop()
+        lat = np.array([done[j]["latency_seconds"] for j in range(args.requests)]) * 1e3
+        ok = all(done[j]["status"] == "optimal" for j in range(args.requests))
+        err = max(float(np.abs(done[j]["primal"] - distinct[int(pick[j])]["optimum"]).max())
+                  for j in range(0, args.requests, max(1, args.requests // 200)))
+        return {"solves_per_s": args.requests / wall, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "all_verified": ok, "wall_s": wall,
+                "sampled_max_abs_err_vs_planted_x": err}
+
+    def plain_loop():
+        """One request at a time in arrival order: upload, FP64 reference-schedule solve, read x."""
+        lat = []
+        ok = True
+        t0 = time.perf_counter()
+        for j in range(args.requests):
+            now = time.perf_counter() - t0
+            if now < arrivals[j]:
+                time.sleep(arrivals[j] - now)
+            d = distinct[int(pick[j])]
+            with hgqp.Box(d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"],
+                          Ls[int(pick[j])]) as b:
+                r = b.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60)
+                b.primal()
+            ok &= r["status"] == "optimal"
+            lat.append(time.perf_counter() - t0 - arrivals[j])
+        wall = time.perf_counter() - t0
+        lat = np.array(lat) * 1e3
+        return {"solves_per_s": args.requests / wall, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "all_verified": bool(ok), "wall_s": wall}
+
+    served = pool_server()
+    plain = plain_loop() if not args.no_plain else None
+    print(f"# served: {served}; plain: {plain}", file=sys.stderr, flush=True)
+    return {
+        "metric": METRIC, "value": round(served["solves_per_s"], 2), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.requests, "warmup": 0, "ms_per_step": round(served["wall_s"] / args.requests * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded planted instances; Poisson arrival trace replayed in real time)",
+        "config": {"workload": f"serving: {args.requests} requests, Poisson lambda={args.rate}/s, n log-uniform in "
+                               f"[1e3, 2e5], 8 draws/row, bounds [-1,1], KKT<=1e-6, continuous batching in one pool",
+                   "slots": args.slots, "budget_rows": args.budget_rows, "budget_nnz": args.budget_nnz,
+                   "distinct_instances": args.distinct, "check_every": args.check_every},
+        "p50_ms": round(served["p50_ms"], 3), "p99_ms": round(served["p99_ms"], 3),
+        "served": served, "plain_fp64_one_at_a_time": plain, "device": hgqp.device_name(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,11 +264,27 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--check-every", type=int, default=10, help="check cadence of the non-reference modes")
     ap.add_argument("--polish-every", type=int, default=5, help="FP64 check cadence after an FP32 phase")
+    ap.add_argument("--workload", choices=["single", "batch", "serve"], default="single")
+    ap.add_argument("--instances", type=int, default=512, help="batch: instances")
+    ap.add_argument("--plain-sample", type=int, default=512, help="batch: instances timed one at a time in FP64")
+    ap.add_argument("--requests", type=int, default=20000, help="serve: requests in the trace")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate per second")
+    ap.add_argument("--distinct", type=int, default=64, help="serve: distinct instances the trace draws from")
+    ap.add_argument("--slots", type=int, default=256, help="serve: pool slots")
+    ap.add_argument("--budget-rows", type=int, default=8_000_000, help="serve: pool row budget")
+    ap.add_argument("--budget-nnz", type=int, default=160_000_000, help="serve: pool entry budget")
+    ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-at-a-time FP64 loop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (instances would shard "
               f"across GPUs with no exchange; see the design notes).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        print(json.dumps(batch_workload(args)))
+        return
+    if args.workload == "serve":
+        print(json.dumps(serve_workload(args)))
+        return
 
     device = hgqp.device_name()
     t = time.perf_counter()

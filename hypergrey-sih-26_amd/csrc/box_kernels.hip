@@ -329,10 +329,24 @@ __global__ void anchor_kernel(int64_t n, const double* g, const double* xa, cons
 }
 
 // x = clip(xa + d, lower, upper) in FP64; the extrapolated point restarts at x.
-__global__ void apply_kernel(int64_t n, const double* xa, const float* d, const double* lo, const double* up,
-                             double* x, double* w) {
+// A correction at its FP32 bound (lo32 = fl(lower - xa)) means the variable is
+// at its bound: x takes the exact FP64 bound, not xa + fl(lower - xa).
+__global__ void apply_kernel(int64_t n, const double* xa, const float* d, const float* lo32, const float* up32,
+                             const double* lo, const double* up, double* x, double* w) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        const double v = clip(xa[j] + double(d[j]), lo[j], up[j]);
+        const float dj = d[j];
+        const double v = dj <= lo32[j] ? lo[j] : dj >= up32[j] ? up[j] : clip(xa[j] + double(dj), lo[j], up[j]);
+        x[j] = v;
+        w[j] = v;
+    }
+}
+
+// FP32 iterate back to FP64 with the same snapping to exact bounds.
+__global__ void widen_kernel(int64_t n, const float* x32, const float* lo32, const float* up32, const double* lo,
+                             const double* up, double* x, double* w) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const float v32 = x32[j];
+        const double v = v32 <= lo32[j] ? lo[j] : v32 >= up32[j] ? up[j] : clip(double(v32), lo[j], up[j]);
         x[j] = v;
         w[j] = v;
     }
@@ -414,9 +428,15 @@ void launch_anchor(int64_t n, const double* g, const double* xa, const double* l
                        d, w);
 }
 
-void launch_apply(int64_t n, const double* xa, const float* d, const double* lo, const double* up, double* x,
-                  double* w, hipStream_t stream) {
-    hipLaunchKernelGGL(apply_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, xa, d, lo, up, x, w);
+void launch_apply(int64_t n, const double* xa, const float* d, const float* lo32, const float* up32, const double* lo,
+                  const double* up, double* x, double* w, hipStream_t stream) {
+    hipLaunchKernelGGL(apply_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, xa, d, lo32, up32, lo, up, x,
+                       w);
+}
+
+void launch_widen(int64_t n, const float* x32, const float* lo32, const float* up32, const double* lo,
+                  const double* up, double* x, double* w, hipStream_t stream) {
+    hipLaunchKernelGGL(widen_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, x32, lo32, up32, lo, up, x, w);
 }
 
 template <class T>

@@ -94,9 +94,12 @@ void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control
 // FP32 correction cycle anchored at xa with FP64 gradient g (see anchor_kernel)
 void launch_anchor(int64_t n, const double* g, const double* xa, const double* lo, const double* up, float* c32,
                    float* lo32, float* up32, float* d, float* w, hipStream_t stream);
-// x = w = clip(xa + d, lo, up)
-void launch_apply(int64_t n, const double* xa, const float* d, const double* lo, const double* up, double* x,
-                  double* w, hipStream_t stream);
+// x = w = clip(xa + d, lo, up), exact bounds where d sits at its FP32 bound
+void launch_apply(int64_t n, const double* xa, const float* d, const float* lo32, const float* up32, const double* lo,
+                  const double* up, double* x, double* w, hipStream_t stream);
+// x = w = FP64 of the FP32 iterate, exact bounds where it sits at an FP32 bound
+void launch_widen(int64_t n, const float* x32, const float* lo32, const float* up32, const double* lo,
+                  const double* up, double* x, double* w, hipStream_t stream);
 
 // x[0] = w[0] = clip(0, lower, upper)
 template <class T>

@@ -583,8 +583,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
                 HIPCHECK(hipGetLastError());
                 new_phase(it);
                 fp32_phase(cycle == 0 ? switch_stat : 0.5 * so.tolerance);
-                launch_apply(b->n, b->x[pa], b->x32[b->parity], b->lo, b->up, b->x[b->parity], b->w[b->parity],
-                             b->stream);
+                launch_apply(b->n, b->x[pa], b->x32[b->parity], b->lo32, b->up32, b->lo, b->up, b->x[b->parity],
+                             b->w[b->parity], b->stream);
                 fp64_check();
             }
             report->fp32_iterations = it;
@@ -595,8 +595,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             new_phase(it);
             fp32_phase(switch_stat);
             report->fp32_iterations = it;
-            launch_convert<double, float>(b->n, b->x32[b->parity], b->x[b->parity], b->stream);
-            launch_convert<double, float>(b->n, b->x32[b->parity], b->w[b->parity], b->stream);
+            launch_widen(b->n, b->x32[b->parity], b->lo32, b->up32, b->lo, b->up, b->x[b->parity], b->w[b->parity],
+                         b->stream);
             fp64_check();
         }
         if (c.done == kRunning && !time_up && it < so.iteration_limit) {

@@ -187,8 +187,13 @@ __global__ __launch_bounds__(kGroup) void pool_apply(PoolView v, int parity) {
         const int b = uniform(v.list_verify[idx]);
         const int init = v.inst[uniform(v.blk_inst[b])].phase == kSlotInit;
         for (int r = v.blk_r0[b] + lane; r < v.blk_r1[b]; r += kWave) {
-            const double base = init ? 0.0 : v.x[r] + double(v.d[parity][r]);
-            v.x[r] = clip(base, v.lo[r], v.up[r]);
+            if (init) {
+                v.x[r] = clip(0.0, v.lo[r], v.up[r]);
+                continue;
+            }
+            // a correction at its FP32 bound puts the variable exactly on its bound
+            const float dr = v.d[parity][r];
+            v.x[r] = dr <= v.lo32[r] ? v.lo[r] : dr >= v.up32[r] ? v.up[r] : clip(v.x[r] + double(dr), v.lo[r], v.up[r]);
         }
     }
 }
