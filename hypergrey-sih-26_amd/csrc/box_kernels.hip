@@ -44,6 +44,7 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
                                                       double* partials, int parity, double beta_imm) {
     __shared__ T stage[kWavesPerGroup][kCap];
     __shared__ double wave_dot[kWavesPerGroup];
+    __shared__ double wave_res[kWavesPerGroup];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     double beta = beta_imm, next_t = 0;
     int use_x = 0;
@@ -63,6 +64,7 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     epi.lipschitz = bufs.lipschitz;
     epi.beta = T(beta);
     epi.dot = 0;
+    epi.resid = 0;
     for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
         row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi);
     if (MOM == kMomentumReference) {
@@ -72,14 +74,22 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
             ctrl->use_x[parity ^ 1] = 0;
         }
     } else if (MOM == kMomentumRestart) {
-        // one partial per workgroup; finalize_restart combines them after this launch
-        double d = wave_sum(epi.dot);
-        if (lane == 0) wave_dot[wave] = d;
+        // per workgroup: the restart dot product and the residual maximum;
+        // finalize_restart combines them after this launch
+        double d = wave_sum(epi.dot), rr = wave_max(epi.resid);
+        if (lane == 0) {
+            wave_dot[wave] = d;
+            wave_res[wave] = rr;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double s = 0;
-            for (int w = 0; w < kWavesPerGroup; ++w) s += wave_dot[w];
+            double s = 0, r = 0;
+            for (int w = 0; w < kWavesPerGroup; ++w) {
+                s += wave_dot[w];
+                r = fmax(r, wave_res[w]);
+            }
             partials[blockIdx.x] = s;
+            partials[gridDim.x + blockIdx.x] = r;
         }
     }
 }
@@ -90,28 +100,40 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
 constexpr int kFinal = 1024;  // threads of a finalize launch
 
 __global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const double* partials, int groups,
-                                                           int parity) {
+                                                           int parity, double switch_stat) {
     __shared__ double wave_dot[kFinal / kWave];
+    __shared__ double wave_res[kFinal / kWave];
     if (ctrl->done) return;  // the step returned early
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // thread t owns partials t, t + kFinal, ...; 8 loads in flight per batch
-    double s = 0;
+    double s = 0, res = 0;
     for (int g0 = threadIdx.x; g0 < groups; g0 += 8 * kFinal) {
-        double v[8];
+        double v[8], r[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int g = g0 + u * kFinal;
             v[u] = g < groups ? partials[g] : 0.0;
+            r[u] = g < groups ? partials[groups + g] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
+        for (int u = 0; u < 8; ++u) {
+            s += v[u];
+            res = fmax(res, r[u]);
+        }
     }
     s = wave_sum(s);
-    if (lane == 0) wave_dot[wave] = s;
+    res = wave_max(res);
+    if (lane == 0) {
+        wave_dot[wave] = s;
+        wave_res[wave] = res;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double tot = 0;
-        for (int w = 0; w < kFinal / kWave; ++w) tot += wave_dot[w];
+        double tot = 0, rmax = 0;
+        for (int w = 0; w < kFinal / kWave; ++w) {
+            tot += wave_dot[w];
+            rmax = fmax(rmax, wave_res[w]);
+        }
         double next_t, beta;
         momentum_of(ctrl->t[parity], &next_t, &beta);
         const int restart = tot > 0;
@@ -119,6 +141,19 @@ __global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const 
         ctrl->use_x[parity ^ 1] = restart;
         ctrl->restarts += restart;
         ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
+        if (switch_stat > 0) {
+            // FP32 phase: the step's residual ends the phase at the switch
+            // threshold, on stagnation (FP32 floor) or when it is not finite
+            if (!(rmax <= ctrl->best_stat * 0.5)) ctrl->stall += 1;
+            else ctrl->stall = 0;
+            if (rmax < ctrl->best_stat) ctrl->best_stat = rmax;
+            if (!(rmax > switch_stat) || ctrl->stall >= 20) {
+                ctrl->stationarity = rmax;
+                ctrl->check_iter = ctrl->iter[parity] + 1;
+                ctrl->check_parity = parity ^ 1;
+                ctrl->done = kPhaseEnd;
+            }
+        }
     }
 }
 
@@ -384,7 +419,7 @@ int groups_for(int nblocks) {
 
 template <class T>
 void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
-                 int momentum, bool ordered, double beta, hipStream_t stream) {
+                 int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat) {
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
 #define HGQP_STEP(ORD, MOM) hipLaunchKernelGGL((step_kernel<T, ORD, MOM>), grid, block, 0, stream, m, b, ctrl, partials, parity, beta)
     if (ordered) {
@@ -398,13 +433,14 @@ void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, dou
     }
 #undef HGQP_STEP
     if (momentum == kMomentumRestart)
-        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity);
+        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity,
+                           switch_stat);
 }
 
 template void launch_step<double>(const Matrix<double>&, const StepBuffers<double>&, Control*, double*, int, int,
-                                  bool, double, hipStream_t);
+                                  bool, double, hipStream_t, double);
 template void launch_step<float>(const Matrix<float>&, const StepBuffers<float>&, Control*, double*, int, int, bool,
-                                 double, hipStream_t);
+                                 double, hipStream_t, double);
 
 void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream) {

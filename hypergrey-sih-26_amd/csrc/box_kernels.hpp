@@ -81,9 +81,12 @@ struct CheckParams {
 
 // Launchers. parity selects x[parity], w[parity] as inputs; step launches
 // write x[parity^1], w[parity^1]. Every launch goes to `stream`.
+// With the restart schedule and switch_stat > 0, the step's own residual
+// max L|w - x+| / (1 + |x+|) ends an FP32 phase (done = kPhaseEnd) once it is at
+// or below switch_stat, on stagnation, or when it is not finite.
 template <class T>
 void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
-                 int momentum, bool ordered, double beta, hipStream_t stream);
+                 int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat = 0.0);
 
 void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream);

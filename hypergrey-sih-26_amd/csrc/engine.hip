@@ -98,6 +98,7 @@ struct hgqp_box {
     // captured check periods, keyed by (precision, start parity, steps)
     struct Graph {
         int precision, parity, steps, momentum;
+        double switch_stat;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -178,20 +179,34 @@ void fill_report(const Control& c, hgqp_check_report* r) {
     r->finite = c.finite;
 }
 
-// Enqueues one check period: `steps` steps from parity b->parity, then a check.
-void enqueue_period(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
+// An FP32 phase under the restart schedule ends on the steps' own residual, so
+// its periods carry no separate FP32 check launch.
+bool residual_phase(int precision, int momentum) {
+    return precision != HGQP_PRECISION_FP64 && momentum == kMomentumRestart;
+}
+
+void launch_one_step(hgqp_box* b, int precision, int momentum, const CheckParams& cp) {
     const bool ord = b->ordered();
-    for (int s = 0; s < steps; ++s) {
-        if (precision == HGQP_PRECISION_FP64)
-            launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
-        else
-            launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
-        b->parity ^= 1;
-    }
+    if (precision == HGQP_PRECISION_FP64)
+        launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+    else
+        launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream,
+                           residual_phase(precision, momentum) ? cp.switch_stat : 0.0);
+    b->parity ^= 1;
+}
+
+void launch_period_check(hgqp_box* b, int precision, int momentum, const CheckParams& cp) {
+    const bool ord = b->ordered();
     if (precision == HGQP_PRECISION_FP64)
         launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
-    else
+    else if (!residual_phase(precision, momentum))
         launch_check32(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
+}
+
+// Enqueues one check period: `steps` steps from parity b->parity, then a check.
+void enqueue_period(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
+    for (int s = 0; s < steps; ++s) launch_one_step(b, precision, momentum, cp);
+    launch_period_check(b, precision, momentum, cp);
 }
 
 // Timed variant: one event pair around the period's back-to-back step launches
@@ -216,7 +231,6 @@ void settle_periods(std::vector<TimedPeriod>* timed, size_t from, uint64_t end) 
 
 void enqueue_period_timed(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp,
                           uint64_t first, std::vector<TimedPeriod>* timed) {
-    const bool ord = b->ordered();
     const size_t e = timed->empty() ? 0 : timed->back().event + 2;
     while (b->events.size() < e + 2) {
         hipEvent_t ev;
@@ -224,24 +238,16 @@ void enqueue_period_timed(hgqp_box* b, int precision, int steps, int momentum, c
         b->events.push_back(ev);
     }
     HIPCHECK(hipEventRecord(b->events[e], b->stream));
-    for (int s = 0; s < steps; ++s) {
-        if (precision == HGQP_PRECISION_FP64)
-            launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
-        else
-            launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
-        b->parity ^= 1;
-    }
+    for (int s = 0; s < steps; ++s) launch_one_step(b, precision, momentum, cp);
     HIPCHECK(hipEventRecord(b->events[e + 1], b->stream));
     timed->push_back({e, first, steps, precision, true});
-    if (precision == HGQP_PRECISION_FP64)
-        launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
-    else
-        launch_check32(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, ord, cp, b->stream);
+    launch_period_check(b, precision, momentum, cp);
 }
 
 hipGraphExec_t period_graph(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
     for (auto& g : b->graphs)
-        if (g.precision == precision && g.parity == b->parity && g.steps == steps && g.momentum == momentum)
+        if (g.precision == precision && g.parity == b->parity && g.steps == steps && g.momentum == momentum &&
+            g.switch_stat == cp.switch_stat)
             return g.exec;
     const int p0 = b->parity;
     hipGraph_t graph;
@@ -253,7 +259,7 @@ hipGraphExec_t period_graph(hgqp_box* b, int precision, int steps, int momentum,
     (void)hipGraphDestroy(graph);
     HIPCHECK(e);
     b->parity = p0;
-    b->graphs.push_back({precision, p0, steps, momentum, exec});
+    b->graphs.push_back({precision, p0, steps, momentum, cp.switch_stat, exec});
     return exec;
 }
 

@@ -121,7 +121,7 @@ size_t pool_bytes(const hgqp_pool_options& o) {
            5 * Arena::span(R, 8) +                                                            // c lo up x g
            7 * Arena::span(R, 4) +                                                            // c32 lo32 up32 d[2] w[2]
            Arena::span(S, sizeof(PoolInst)) + 2 * Arena::span(B, 4) + Arena::span(4, 4) + Arena::span(S, 4) +
-           Arena::span(B * kPartial, 8);
+           Arena::span(B * kPartial, 8) + Arena::span(B, 8);
 }
 
 void validate_options(const hgqp_pool_options& o) {
@@ -319,6 +319,7 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     v.counts = p->arena.take<int>(4);
     v.offsets = p->arena.take<int>(S);
     v.partials = p->arena.take<double>(B * kPartial);
+    v.resid = p->arena.take<double>(B);
     v.switch_stat = o.fp32_switch;
     v.restart = o.momentum == HGQP_MOMENTUM_RESTART ? 1 : 0;
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->table_host), sizeof(PoolInst) * S, hipHostMallocDefault));

@@ -10,8 +10,9 @@
 // Iteration scheme per instance (FP32 correction cycles anchored at FP64):
 //   anchor x_a (FP64), g_a = H x_a + c from the FP64 check, d = w = 0;
 //   FP32 FISTA steps on d with gradient g_a + H d over [l - x_a, u - x_a];
-//   FP32 check every period; when it meets the cycle's target, x = clip(x_a + d)
-//   and the FP64 check decides: verified -> done, else re-anchor at x.
+//   the step's own residual ends the cycle at its target (10 tol from x0, then
+//   tol / 2); x = clip(x_a + d) and the FP64 check decides: verified -> done,
+//   else re-anchor at x.
 #include "pool_kernels.hpp"
 #include "sparse_pass.hpp"
 
@@ -35,6 +36,7 @@ __global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
         const int b = uniform(v.list_active[idx]);
         const int i = uniform(v.blk_inst[b]);
         const PoolInst* s = v.inst + i;
+        if (s->phase != kSlotActive) continue;  // met its target earlier in this period
         double next_t, beta;
         momentum_of(s->t[parity], &next_t, &beta);
         StepEpilogue<float> epi;
@@ -48,10 +50,14 @@ __global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
         epi.lipschitz = float(s->lipschitz);
         epi.beta = float(beta);
         epi.dot = 0;
+        epi.resid = 0;
         row_block<float, false, LOADS>(v.start, v.col, v.val32, epi.gat, v.blk_r0[b], v.blk_r1[b], stage[wave], lane,
                                        epi);
-        const double dot = wave_sum(epi.dot);
-        if (lane == 0) v.partials[b] = dot;
+        const double dot = wave_sum(epi.dot), res = wave_max(epi.resid);
+        if (lane == 0) {
+            v.partials[b] = dot;
+            v.resid[b] = res;
+        }
     }
 }
 
@@ -80,12 +86,17 @@ __device__ inline double slot_max(const double* part, int b0, int nblk, double* 
     return tot;
 }
 
-// Restart test and momentum of every active slot after a step (one workgroup per slot).
+// Restart test, momentum and the cycle's stopping test of every active slot
+// after a step (one workgroup per slot). A slot whose step residual
+// max L|w - d+| / (1 + |x_a + d+|) meets its target, stagnates or reaches its
+// iteration limit waits for the FP64 check with d+ frozen.
 __global__ __launch_bounds__(kGroup) void pool_finalize_step(PoolView v, int parity) {
     __shared__ double red[kWavesPerGroup];
+    __shared__ double red2[kWavesPerGroup];
     PoolInst* s = v.inst + blockIdx.x;
     if (s->phase != kSlotActive) return;
     const double tot = slot_sum(v.partials, s->b0, s->nblk, red);
+    const double res = slot_max(v.resid, s->b0, s->nblk, red2);
     if (threadIdx.x != 0) return;
     double next_t, beta;
     momentum_of(s->t[parity], &next_t, &beta);
@@ -94,51 +105,14 @@ __global__ __launch_bounds__(kGroup) void pool_finalize_step(PoolView v, int par
     s->use_x[parity ^ 1] = restart;
     s->restarts += restart;
     s->iter += 1;
-}
-
-// ---- FP32 check of the correction problem ---------------------------------------
-struct PoolCheck32 {
-    const float* __restrict__ d;
-    const float* __restrict__ c;
-    const float* __restrict__ lo;
-    const float* __restrict__ up;
-    const double* __restrict__ xa;
-    double stat;
-    __device__ inline void operator()(int r, float hd) {
-        const float dv = d[r];
-        const float g = hd + c[r];
-        const double res = fabs(double(dv - clip(dv - g, lo[r], up[r]))) / (1 + fabs(xa[r] + double(dv)));
-        stat = (isfinite(res) && isfinite(g)) ? fmax(stat, res) : INFINITY;
-    }
-};
-
-template <int LOADS>
-__global__ __launch_bounds__(kGroup) void pool_check32(PoolView v, int parity) {
-    __shared__ float stage[kWavesPerGroup][kWave * LOADS];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int len = v.counts[0];
-    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
-        const int b = uniform(v.list_active[idx]);
-        PoolCheck32 epi{v.d[parity], v.c32, v.lo32, v.up32, v.x, 0.0};
-        row_block<float, false, LOADS>(v.start, v.col, v.val32, v.d[parity], v.blk_r0[b], v.blk_r1[b], stage[wave],
-                                       lane, epi);
-        const double m = wave_max(epi.stat);
-        if (lane == 0) v.partials[b] = m;
-    }
-}
-
-__global__ __launch_bounds__(kGroup) void pool_finalize32(PoolView v) {
-    __shared__ double red[kWavesPerGroup];
-    PoolInst* s = v.inst + blockIdx.x;
-    if (s->phase != kSlotActive) return;
-    const double stat = slot_max(v.partials, s->b0, s->nblk, red);
-    if (threadIdx.x != 0) return;
-    s->stat32 = stat;
-    s->checks += 1;
-    if (!(stat <= s->best_stat * 0.5)) s->stall += 1;
+    s->stat32 = res;
+    if (!(res <= s->best_stat * 0.5)) s->stall += 1;
     else s->stall = 0;
-    if (stat < s->best_stat) s->best_stat = stat;
-    if (!(stat > s->target) || s->stall >= 2 || s->iter >= s->limit) s->phase = kSlotVerify;
+    if (res < s->best_stat) s->best_stat = res;
+    if (!(res > s->target) || s->stall >= 20 || s->iter >= s->limit) {
+        s->phase = kSlotVerify;
+        s->frozen = parity ^ 1;
+    }
 }
 
 // ---- work lists -------------------------------------------------------------------
@@ -185,14 +159,16 @@ __global__ __launch_bounds__(kGroup) void pool_apply(PoolView v, int parity) {
     const int len = v.counts[1];
     for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
         const int b = uniform(v.list_verify[idx]);
-        const int init = v.inst[uniform(v.blk_inst[b])].phase == kSlotInit;
+        const PoolInst* s = v.inst + uniform(v.blk_inst[b]);
+        const int init = s->phase == kSlotInit;
+        const float* d = v.d[s->frozen];
         for (int r = v.blk_r0[b] + lane; r < v.blk_r1[b]; r += kWave) {
             if (init) {
                 v.x[r] = clip(0.0, v.lo[r], v.up[r]);
                 continue;
             }
             // a correction at its FP32 bound puts the variable exactly on its bound
-            const float dr = v.d[parity][r];
+            const float dr = d[r];
             v.x[r] = dr <= v.lo32[r] ? v.lo[r] : dr >= v.up32[r] ? v.up[r] : clip(v.x[r] + double(dr), v.lo[r], v.up[r]);
         }
     }
@@ -305,8 +281,6 @@ void period_impl(const PoolView& v, int max_blocks, int steps, hipStream_t strea
         hipLaunchKernelGGL(pool_finalize_step, slots, block, 0, stream, v, parity);
         parity ^= 1;
     }
-    hipLaunchKernelGGL(pool_check32<LOADS>, grid, block, 0, stream, v, parity);
-    hipLaunchKernelGGL(pool_finalize32, slots, block, 0, stream, v);
     hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 1);
     hipLaunchKernelGGL(pool_list_expand, slots, block, 0, stream, v, 1);
     hipLaunchKernelGGL(pool_apply, grid, block, 0, stream, v, parity);

@@ -38,7 +38,7 @@ struct PoolInst {
     int32_t diagonal;
     int32_t b0, nblk;  // block range
     int32_t r0, n;     // row range
-    int32_t done_iter_pad;
+    int32_t frozen;    // parity of the correction a waiting slot stopped at
 };
 
 struct PoolView {
@@ -70,14 +70,16 @@ struct PoolView {
     int* counts;    // [0] active length, [1] verify length
     int* offsets;   // [max_inst] scratch for list building
     double* partials;  // per block
+    double* resid;     // per block: the step's residual maximum
     double switch_stat;
     int restart;       // 1: gradient restart, 0: reference momentum
 };
 
-// One check period: `steps` FP32 correction steps of every active slot, the
-// FP32 check, then the FP64 check of slots that are new or met the FP32
-// criterion, retirement or re-anchoring, and the next active list. Uses
-// `parity` for the first step; `steps` must be even for graph replay.
+// One check period: `steps` FP32 correction steps of every active slot (a slot
+// whose step residual meets its cycle target stops and waits for
+// verification), then the FP64 check of slots that are new or waiting,
+// retirement or re-anchoring, and the next active list. Starts at parity 0;
+// `steps` must be even so every period starts alike (graph replay).
 void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps, hipStream_t stream);
 
 // Builds the active list from the slot table (after host admission edits).

@@ -185,7 +185,8 @@ struct StepEpilogue {
     T* __restrict__ wout;
     T lipschitz;
     T beta;
-    double dot;  // restart test: sum (w - next)(next - x)
+    double dot;    // restart test: sum (w - next)(next - x)
+    double resid;  // max L |w - next| / (1 + |next|): the step's own gradient-mapping residual
     __device__ inline void operator()(int r, T g) {
 #pragma clang fp contract(off)
         const T wv = gat[r], xv = xin[r];
@@ -193,6 +194,8 @@ struct StepEpilogue {
         wout[r] = next + beta * (next - xv);
         xout[r] = next;
         dot += double(wv - next) * double(next - xv);
+        const double rr = double(lipschitz) * fabs(double(wv) - double(next)) / (1 + fabs(double(next)));
+        resid = isfinite(rr) ? fmax(resid, rr) : INFINITY;
     }
 };
 
