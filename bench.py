@@ -48,6 +48,20 @@ def step_bytes(n, nnz, value_bytes, vector_bytes):
     return nnz * (value_bytes + 4) + 4 * (n + 1) + 7 * n * vector_bytes
 
 
+def measured_traffic(use32, n, draws, nnz):
+    """HBM bytes per launch of the FP32 step from the committed rocprofv3 PMC summary
+    (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on plain streaming kernels), when this
+    run's kernel and instance are the ones profiled; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "traffic_planted_n1e6.json")
+    if not use32 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}":
+        return None
+    return t["traffic_bytes_per_launch"]
+
+
 def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
     precision, momentum, reduce = MODES[name]
     # the reference schedule keeps the reference's check cadence of 25 iterations
@@ -344,7 +358,7 @@ def main():
     nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8)
     achieved = nbytes / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(use32, n, args.draws, nnz),
                 "kernel": f"step_kernel<{'float' if use32 else 'double'}>", "algorithmic_bytes": nbytes,
                 "avg_launch_us": round(us, 2)}
     # the FP64 reference-schedule step beside it
