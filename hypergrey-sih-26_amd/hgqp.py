@@ -155,6 +155,9 @@ SIGNATURES = {
     "hgqp_box_info_get": ([C.c_void_p, C.POINTER(BoxInfo)], C.c_int),
     "hgqp_solve_qp": ([C.POINTER(Problem), C.POINTER(QpOptions), C.POINTER(QpResult), P_F64], C.c_int),
     "hgqp_csr_lipschitz": ([C.c_int64, P_I32, P_F64], C.c_double),
+    "hgqp_qplib_parse": ([C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
+    "hgqp_qplib_problem": ([C.c_void_p, C.POINTER(Problem), P_I64], C.c_int),
+    "hgqp_qplib_free": ([C.c_void_p], C.c_int),
     "hgqp_pool_options_default": ([C.POINTER(PoolOptions)], None),
     "hgqp_pool_bytes": ([C.POINTER(PoolOptions), P_I64], C.c_int),
     "hgqp_pool_create": ([C.POINTER(PoolOptions), C.POINTER(C.c_void_p)], C.c_int),
@@ -309,6 +312,26 @@ def solve_qp(n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False
     d = r.as_dict()
     d["primal"] = x
     return d
+
+
+def read_qplib(text):
+    """Parses QPLIB text (hgqp_qplib_parse) into a dict of numpy arrays in the
+    triplet form of solve_qp, plus 'rows' (linear rows; the box engine needs 0)."""
+    if isinstance(text, str):
+        text = text.encode()
+    h = C.c_void_p()
+    _raise(lib().hgqp_qplib_parse(text, len(text), C.byref(h)))
+    try:
+        view, rows = Problem(), C.c_int64()
+        _raise(lib().hgqp_qplib_problem(h, C.byref(view), C.byref(rows)))
+        n, m = view.n, view.nterms
+        arr = lambda p, k, t: np.ctypeslib.as_array(p, shape=(k,)).astype(t, copy=True) if k else np.zeros(0, t)
+        return dict(n=n, hrow=arr(view.hrow, m, np.int64), hcol=arr(view.hcol, m, np.int64),
+                    hval=arr(view.hval, m, np.float64), cost=arr(view.cost, n, np.float64),
+                    lower=arr(view.lower, n, np.float64), upper=arr(view.upper, n, np.float64),
+                    offset=view.offset, maximize=bool(view.maximize), rows=rows.value)
+    finally:
+        lib().hgqp_qplib_free(h)
 
 
 def gen_tridiagonal(n):

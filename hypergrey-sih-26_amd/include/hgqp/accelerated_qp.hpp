@@ -10,6 +10,7 @@
 #include "hgqp.h"
 
 #include <cstddef>
+#include <istream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -86,6 +87,9 @@ struct CsrMatrix {
 struct UnsupportedModel : std::invalid_argument {
     using std::invalid_argument::invalid_argument;
 };
+
+// include/qp.hpp:31 and src/qp.cpp:67-117 (throws std::runtime_error on malformed input)
+QuadraticProblem read_qplib(std::istream&);
 
 // include/mps.hpp:31 and src/mps.cpp:47-60
 void validate_linear_problem(const LinearProblem&);

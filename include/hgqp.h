@@ -230,6 +230,17 @@ int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, h
  * (accelerated_qp.cpp:25-26), for callers that assemble the CSR themselves. */
 double hgqp_csr_lipschitz(int64_t n, const int32_t* start, const double* value);
 
+/* ---- QPLIB input (qp.cpp:12-117) ---------------------------------------- */
+/* Parses a QPLIB model (continuous types (C|D|Q) C (N|B|L), n <= 100000) with
+ * the reference reader's rules and messages (read_qplib, qp.cpp:67-117); a
+ * malformed model returns HGQP_INVALID_ARGUMENT. hgqp_qplib_problem fills a
+ * view whose arrays live until hgqp_qplib_free; *rows receives the number of
+ * linear rows (the box engine requires 0). */
+typedef struct hgqp_qplib hgqp_qplib;
+int hgqp_qplib_parse(const char* text, size_t length, hgqp_qplib** out);
+int hgqp_qplib_problem(const hgqp_qplib* model, hgqp_problem* view, int64_t* rows);
+int hgqp_qplib_free(hgqp_qplib* model);
+
 /* ---- instance pool: many box QPs resident at once ---------------------- */
 /* No counterpart in the reference, which solves one QP per process
  * (main.cpp:146; backend/limits.py:13). A pool keeps up to max_instances box
