@@ -132,6 +132,17 @@ class Served(C.Structure):
         return d
 
 
+class DenseOptions(C.Structure):
+    _fields_ = [("lipschitz", C.c_double), ("power_iterations", C.c_int32), ("check_every", C.c_int32),
+                ("lipschitz_margin", C.c_double), ("fp32_switch", C.c_double)]
+
+
+class DenseReport(C.Structure):
+    _fields_ = [("verified", C.c_int64), ("max_iterations", C.c_uint64), ("bf16_gemms", C.c_uint64),
+                ("fp64_gemms", C.c_uint64), ("worst_stationarity", C.c_double), ("lipschitz", C.c_double),
+                ("solve_seconds", C.c_double)]
+
+
 P_I32, P_I64, P_F64 = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
 _ADMIT = [C.c_int64, C.c_int64, P_I32, P_I32, P_F64, P_F64, P_F64, P_F64, C.c_double, C.c_int32, C.c_double,
           C.c_double, C.c_uint64, C.c_int64, P_F64]
@@ -155,6 +166,13 @@ SIGNATURES = {
     "hgqp_box_info_get": ([C.c_void_p, C.POINTER(BoxInfo)], C.c_int),
     "hgqp_solve_qp": ([C.POINTER(Problem), C.POINTER(QpOptions), C.POINTER(QpResult), P_F64], C.c_int),
     "hgqp_csr_lipschitz": ([C.c_int64, P_I32, P_F64], C.c_double),
+    "hgqp_dense_options_default": ([C.POINTER(DenseOptions)], None),
+    "hgqp_dense_create": ([C.c_int64, C.c_int64, P_F64, P_F64, P_F64, P_F64, C.POINTER(DenseOptions),
+                           C.POINTER(C.c_void_p)], C.c_int),
+    "hgqp_dense_solve": ([C.c_void_p, C.c_double, C.c_uint64, C.POINTER(DenseReport)], C.c_int),
+    "hgqp_dense_results": ([C.c_void_p, P_F64, C.POINTER(CheckReport), P_I32, C.POINTER(C.c_uint64)], C.c_int),
+    "hgqp_dense_time_gemm": ([C.c_void_p, C.c_int32, P_F64, P_F64], C.c_int),
+    "hgqp_dense_destroy": ([C.c_void_p], C.c_int),
     "hgqp_qplib_parse": ([C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
     "hgqp_qplib_problem": ([C.c_void_p, C.POINTER(Problem), P_I64], C.c_int),
     "hgqp_qplib_free": ([C.c_void_p], C.c_int),
@@ -544,3 +562,62 @@ class Server:
             self.stop()
         except Exception:
             pass
+
+
+class Dense:
+    """Dense multi-RHS engine (hgqp_dense_*): one dense Q (n x n), K cost vectors (K x n)."""
+
+    def __init__(self, Q, C_rhs, lower, upper, **options):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        C_rhs = np.ascontiguousarray(C_rhs, dtype=np.float64)
+        lower = np.ascontiguousarray(lower, dtype=np.float64)
+        upper = np.ascontiguousarray(upper, dtype=np.float64)
+        self.n, self.k = Q.shape[0], C_rhs.shape[0]
+        o = DenseOptions()
+        lib().hgqp_dense_options_default(C.byref(o))
+        for key, val in options.items():
+            setattr(o, key, val)
+        self.handle = C.c_void_p()
+        _raise(lib().hgqp_dense_create(self.n, self.k, _ptr(Q, C.c_double), _ptr(C_rhs, C.c_double),
+                                       _ptr(lower, C.c_double), _ptr(upper, C.c_double), C.byref(o),
+                                       C.byref(self.handle)))
+
+    def solve(self, tolerance=1e-6, iteration_limit=20000):
+        r = DenseReport()
+        _raise(lib().hgqp_dense_solve(self.handle, tolerance, iteration_limit, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
+    def results(self):
+        x = np.zeros((self.k, self.n))
+        checks = (CheckReport * self.k)()
+        status = np.zeros(self.k, dtype=np.int32)
+        iters = np.zeros(self.k, dtype=np.uint64)
+        _raise(lib().hgqp_dense_results(self.handle, _ptr(x, C.c_double), checks, _ptr(status, C.c_int32),
+                                        iters.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return x, [c.as_dict() for c in checks], [STATUS[s] for s in status], iters
+
+    def time_gemm(self, reps=50):
+        us, tf = C.c_double(), C.c_double()
+        _raise(lib().hgqp_dense_time_gemm(self.handle, reps, C.byref(us), C.byref(tf)))
+        return us.value, tf.value
+
+    def close(self):
+        if self.handle:
+            _raise(lib().hgqp_dense_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def gen_dense(n, k, seed):
+    """Q = M'M/n + 0.1 I with M an n x n N(0,1) matrix, K cost vectors N(0,1), bounds [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n))
+    Q = M.T @ M / n + 0.1 * np.eye(n)
+    Q = 0.5 * (Q + Q.T)
+    C_rhs = rng.standard_normal((k, n))
+    return Q, C_rhs, -np.ones(n), np.ones(n)

@@ -230,6 +230,46 @@ int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, h
  * (accelerated_qp.cpp:25-26), for callers that assemble the CSR themselves. */
 double hgqp_csr_lipschitz(int64_t n, const int32_t* start, const double* value);
 
+/* ---- dense multi-RHS: one dense Q, many cost vectors -------------------- */
+/* No counterpart in the reference (one sparse instance, one cost vector per
+ * process). K box QPs min 1/2 x'Qx + c_j'x over the same box share a dense
+ * symmetric positive definite Q (n x n, row-major); c is RHS-major (K x n).
+ * Q X runs on the matrix cores: BF16 operands with FP32 accumulation for
+ * FP32 correction cycles, FP64 for every gradient the verdict uses; each RHS
+ * is decided by the FP64 check of qp_admm.cpp:63-117 restated on the device. */
+typedef struct hgqp_dense hgqp_dense;
+
+typedef struct hgqp_dense_options {
+    double lipschitz;         /* step 1/L; 0 = power-iteration estimate x margin  */
+    int32_t power_iterations; /* default 60                                       */
+    int32_t check_every;      /* steps between host read-backs, default 10        */
+    double lipschitz_margin;  /* default 1.05                                     */
+    double fp32_switch;       /* first cycle ends at this multiple of tol (10)    */
+} hgqp_dense_options;
+
+typedef struct hgqp_dense_report {
+    int64_t verified;            /* RHS verified at the tolerance                 */
+    uint64_t max_iterations;     /* correction steps of the slowest RHS           */
+    uint64_t bf16_gemms, fp64_gemms;
+    double worst_stationarity;
+    double lipschitz;
+    double solve_seconds;
+} hgqp_dense_report;
+
+void hgqp_dense_options_default(hgqp_dense_options* options);
+int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, const double* lower,
+                      const double* upper, const hgqp_dense_options* options, hgqp_dense** out);
+/* Solves every RHS from x0 = clip(0, l, u). */
+int hgqp_dense_solve(hgqp_dense* dense, double tolerance, uint64_t iteration_limit, hgqp_dense_report* report);
+/* x (K x n, RHS-major), per-RHS check reports, HGQP_STATUS_* and iterations;
+ * any output may be NULL. */
+int hgqp_dense_results(hgqp_dense* dense, double* x, hgqp_check_report* checks, int32_t* status,
+                       uint64_t* iterations);
+/* Average time of `reps` back-to-back BF16 GEMM launches (HIP events on the
+ * engine's stream) and the FLOP rate 2 n^2 K / time, for the roofline. */
+int hgqp_dense_time_gemm(hgqp_dense* dense, int32_t reps, double* avg_us, double* tflops);
+int hgqp_dense_destroy(hgqp_dense* dense);
+
 /* ---- QPLIB input (qp.cpp:12-117) ---------------------------------------- */
 /* Parses a QPLIB model (continuous types (C|D|Q) C (N|B|L), n <= 100000) with
  * the reference reader's rules and messages (read_qplib, qp.cpp:67-117); a
