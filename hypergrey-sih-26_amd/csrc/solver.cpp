@@ -131,9 +131,18 @@ struct BoxHandle {
 
 }  // namespace
 
-QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, const std::string& backend) {
-    const auto t0 = std::chrono::steady_clock::now();
-    if (backend != "hip") throw std::invalid_argument("Backend must be hip");
+struct PreparedBox {
+    CsrMatrix h;
+    std::vector<double> cost;
+    double lipschitz = 0;
+    bool positive_diagonal = false;
+};
+
+// The host side of accelerated_qp.cpp:14-35 shared by the single-instance
+// solve and the served path: validation, the convexity gate, the
+// sign-normalized mirrored CSR, L, and the early statuses. Returns false with
+// r.status set when the model ends before any iteration.
+bool prepare_box(const QuadraticProblem& q, const QpOptions& opt, QpResult& r, PreparedBox& out) {
     validate_linear_problem(q.linear);
     if (!q.linear.rows.empty()) throw std::invalid_argument("Accelerated box QP requires box constraints only");
     for (bool v : q.linear.integer)
@@ -141,19 +150,20 @@ QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, 
     if (opt.tolerance <= 0 || !std::isfinite(opt.tolerance) || opt.time_limit < 0 || !std::isfinite(opt.time_limit) ||
         opt.threads < 1)
         throw std::invalid_argument("Invalid box QP options");
-    QpResult r;
-    r.algorithm = opt.precision == HGQP_PRECISION_FP32_POLISH ? "hip_fista_box_fp32_polish" : "hip_fista_box";
+    r.algorithm = opt.precision == HGQP_PRECISION_FP32_POLISH   ? "hip_fista_box_fp32_polish"
+                  : opt.precision == HGQP_PRECISION_FP32_REFINE ? "hip_fista_box_fp32_refine"
+                                                                : "hip_fista_box";
     if (opt.momentum == HGQP_MOMENTUM_RESTART) r.algorithm += "_restart";
     if (!check_convex_hessian(q)) {
         r.status = "unsupported_curvature";
-        return r;
+        return false;
     }
     const auto& p = q.linear;
     const std::size_t n = p.cost.size();
     const double sign = p.maximize ? -1 : 1;
     std::vector<std::vector<std::pair<std::size_t, double> > > rows(n);
-    std::vector<double> c = p.cost;
-    for (double& v : c) v *= sign;
+    out.cost = p.cost;
+    for (double& v : out.cost) v *= sign;
     bool positive_diagonal = true;
     for (const auto& h : q.hessian) {
         rows[h.row].push_back({h.col, sign * h.value});
@@ -169,21 +179,38 @@ QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, 
         for (long double d : diagonal)
             if (!(d > 0)) positive_diagonal = false;
     }
-    CsrMatrix h(n, rows);
+    out.positive_diagonal = positive_diagonal;
+    out.h = CsrMatrix(n, rows);
     rows.clear();
     rows.shrink_to_fit();
     double lipschitz = 0;
     for (std::size_t i = 0; i < n; ++i) {
         double sum = 0;
-        for (int k = h.start[i]; k < h.start[i + 1]; ++k) sum += std::abs(h.value[k]);
+        for (int k = out.h.start[i]; k < out.h.start[i + 1]; ++k) sum += std::abs(out.h.value[k]);
         lipschitz = std::max(lipschitz, sum);
     }
     if (lipschitz == 0) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
+    out.lipschitz = lipschitz;
     for (std::size_t j = 0; j < n; ++j)
         if (p.lower[j] > p.upper[j]) {
             r.status = "infeasible_bounds";
-            return r;
+            return false;
         }
+    return true;
+}
+
+QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, const std::string& backend) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (backend != "hip") throw std::invalid_argument("Backend must be hip");
+    QpResult r;
+    PreparedBox pb;
+    if (!prepare_box(q, opt, r, pb)) return r;
+    const auto& p = q.linear;
+    const std::size_t n = p.cost.size();
+    const CsrMatrix& h = pb.h;
+    const std::vector<double>& c = pb.cost;
+    const double lipschitz = pb.lipschitz;
+    const bool positive_diagonal = pb.positive_diagonal;
     hgqp_box_options bo;
     hgqp_box_options_default(&bo);
     bo.precision = opt.precision;
@@ -236,31 +263,78 @@ extern "C" void hgqp_qp_options_default(hgqp_qp_options* o) {
     o->reduce = HGQP_REDUCE_ORDERED;
 }
 
+namespace {
+
+// The C problem view as the reference's QuadraticProblem (qp.hpp:5-10), with
+// names and integrality filled so validation sees a complete model
+// (mps.cpp:49-51).
+QuadraticProblem to_quadratic_problem(const hgqp_problem* problem, const char* who) {
+    if (!problem) throw std::invalid_argument(std::string(who) + ": null problem");
+    if (problem->n <= 0 || problem->nterms < 0) throw std::invalid_argument("Invalid linear problem dimensions or offset");
+    const std::size_t n = static_cast<std::size_t>(problem->n);
+    if (!problem->cost || !problem->lower || !problem->upper ||
+        (problem->nterms && (!problem->hrow || !problem->hcol || !problem->hval)))
+        throw std::invalid_argument(std::string(who) + ": null array");
+    QuadraticProblem q;
+    auto& lp = q.linear;
+    lp.names.resize(n);
+    lp.integer.assign(n, false);
+    lp.cost.assign(problem->cost, problem->cost + n);
+    lp.lower.assign(problem->lower, problem->lower + n);
+    lp.upper.assign(problem->upper, problem->upper + n);
+    lp.offset = problem->offset;
+    lp.maximize = problem->maximize != 0;
+    q.hessian.resize(static_cast<std::size_t>(problem->nterms));
+    for (int64_t k = 0; k < problem->nterms; ++k) {
+        if (problem->hrow[k] < 0 || problem->hcol[k] < 0) throw std::invalid_argument("Invalid Hessian");
+        q.hessian[k] = {static_cast<std::size_t>(problem->hrow[k]), static_cast<std::size_t>(problem->hcol[k]),
+                        problem->hval[k]};
+    }
+    return q;
+}
+
+}  // namespace
+
+extern "C" int hgqp_server_submit_problem(hgqp_server* server, const hgqp_problem* problem, double tolerance,
+                                          uint64_t iteration_limit, int64_t tag, double* x_out, int32_t* status) {
+    int unsupported = 0;
+    int rc = guarded([&] {
+        if (!server || !x_out || !status) throw std::invalid_argument("hgqp_server_submit_problem: null argument");
+        *status = HGQP_STATUS_RUNNING;
+        const QuadraticProblem q = to_quadratic_problem(problem, "hgqp_server_submit_problem");
+        QpOptions opt;
+        opt.tolerance = tolerance;
+        QpResult r;
+        PreparedBox pb;
+        bool ready;
+        try {
+            ready = prepare_box(q, opt, r, pb);
+        } catch (const UnsupportedModel&) {
+            unsupported = 1;
+            throw;
+        }
+        if (!ready) {
+            *status = r.status == "infeasible_bounds" ? HGQP_STATUS_INFEASIBLE_BOUNDS
+                                                      : HGQP_STATUS_UNSUPPORTED_CURVATURE;
+            return;
+        }
+        const auto& p = q.linear;
+        const int code = hgqp_server_submit(server, problem->n, static_cast<int64_t>(pb.h.value.size()),
+                                            pb.h.start.data(), pb.h.index.data(), pb.h.value.data(), pb.cost.data(),
+                                            p.lower.data(), p.upper.data(), pb.lipschitz, p.maximize ? 1 : 0,
+                                            p.offset, tolerance, iteration_limit, tag, x_out);
+        if (code != HGQP_OK) throw Error(code, hgqp_last_error());
+    });
+    return unsupported ? HGQP_UNSUPPORTED : rc;
+}
+
 extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, hgqp_qp_result* result,
                              double* primal_out) {
     int unsupported = 0;
     int rc = guarded([&] {
         if (!problem || !result) throw std::invalid_argument("hgqp_solve_qp: null problem or result");
-        if (problem->n <= 0 || problem->nterms < 0) throw std::invalid_argument("Invalid linear problem dimensions or offset");
+        const QuadraticProblem q = to_quadratic_problem(problem, "hgqp_solve_qp");
         const std::size_t n = static_cast<std::size_t>(problem->n);
-        if (!problem->cost || !problem->lower || !problem->upper ||
-            (problem->nterms && (!problem->hrow || !problem->hcol || !problem->hval)))
-            throw std::invalid_argument("hgqp_solve_qp: null array");
-        QuadraticProblem q;
-        auto& lp = q.linear;
-        lp.names.resize(n);
-        lp.integer.assign(n, false);
-        lp.cost.assign(problem->cost, problem->cost + n);
-        lp.lower.assign(problem->lower, problem->lower + n);
-        lp.upper.assign(problem->upper, problem->upper + n);
-        lp.offset = problem->offset;
-        lp.maximize = problem->maximize != 0;
-        q.hessian.resize(static_cast<std::size_t>(problem->nterms));
-        for (int64_t k = 0; k < problem->nterms; ++k) {
-            if (problem->hrow[k] < 0 || problem->hcol[k] < 0) throw std::invalid_argument("Invalid Hessian");
-            q.hessian[k] = {static_cast<std::size_t>(problem->hrow[k]), static_cast<std::size_t>(problem->hcol[k]),
-                            problem->hval[k]};
-        }
         hgqp_qp_options o;
         if (options) o = *options;
         else hgqp_qp_options_default(&o);

@@ -30,6 +30,10 @@ class InvalidArgument(HgqpError, ValueError):
     pass
 
 
+class Unsupported(HgqpError, ValueError):
+    """A model the GPU engine does not take (a zero Hessian, accelerated_qp.cpp:28)."""
+
+
 class BoxOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
                 ("use_graph", C.c_int32), ("timing", C.c_int32), ("maximize", C.c_int32),
@@ -187,6 +191,8 @@ SIGNATURES = {
     "hgqp_server_start": ([C.POINTER(PoolOptions), C.POINTER(C.c_void_p)], C.c_int),
     "hgqp_server_submit": ([C.c_void_p] + _ADMIT[0:], C.c_int),
     "hgqp_server_poll": ([C.c_void_p, C.POINTER(Served), C.c_int32, P_I32], C.c_int),
+    "hgqp_server_submit_problem": ([C.c_void_p, C.POINTER(Problem), C.c_double, C.c_uint64, C.c_int64, P_F64,
+                                    P_I32], C.c_int),
     "hgqp_server_pending": ([C.c_void_p, P_I64], C.c_int),
     "hgqp_server_stop": ([C.c_void_p], C.c_int),
     "hgqp_gen_tridiagonal": ([C.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64], C.c_int),
@@ -221,6 +227,8 @@ def _raise(rc):
         msg = lib().hgqp_last_error().decode()
         if rc == INVALID_ARGUMENT:
             raise InvalidArgument(rc, msg)
+        if rc == UNSUPPORTED:
+            raise Unsupported(rc, msg)
         raise HgqpError(rc, msg)
 
 
@@ -315,14 +323,22 @@ def qp_options(**kw):
     return o
 
 
-def solve_qp(n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False, **options):
-    """hgqp_solve_qp: the whole-model entry (solve_accelerated_box with the HIP engine)."""
+def _problem(n, hrow, hcol, hval, cost, lower, upper, offset, maximize):
+    """hgqp_problem over contiguous copies; returns (view, arrays kept alive)."""
     arrs = [np.ascontiguousarray(hrow, dtype=np.int64), np.ascontiguousarray(hcol, dtype=np.int64),
             np.ascontiguousarray(hval, dtype=np.float64), np.ascontiguousarray(cost, dtype=np.float64),
             np.ascontiguousarray(lower, dtype=np.float64), np.ascontiguousarray(upper, dtype=np.float64)]
     hr, hc, hv, c, l, u = arrs
+    if not (len(hr) == len(hc) == len(hv)) or not (len(c) == len(l) == len(u) == int(n)):
+        raise InvalidArgument(INVALID_ARGUMENT, "problem arrays disagree in length")
     p = Problem(int(n), len(hv), _ptr(hr, C.c_int64), _ptr(hc, C.c_int64), _ptr(hv, C.c_double),
                 _ptr(c, C.c_double), _ptr(l, C.c_double), _ptr(u, C.c_double), float(offset), int(bool(maximize)), 0)
+    return p, arrs
+
+
+def solve_qp(n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False, **options):
+    """hgqp_solve_qp: the whole-model entry (solve_accelerated_box with the HIP engine)."""
+    p, keep = _problem(n, hrow, hcol, hval, cost, lower, upper, offset, maximize)
     o = qp_options(**options)
     r = QpResult()
     x = np.zeros(int(n), dtype=np.float64)
@@ -533,6 +549,25 @@ class Server:
         args, keep = _instance_args(inst, L, maximize, offset, tolerance, iteration_limit, tag, x)
         self.out[int(tag)] = x
         _raise(lib().hgqp_server_submit(self.handle, *args))
+
+    def submit_problem(self, n, hrow, hcol, hval, cost, lower, upper, tag, offset=0.0, maximize=False,
+                       tolerance=1e-6, iteration_limit=20000):
+        """hgqp_server_submit_problem: a model in triplet form. Returns the
+        status name: "running" when submitted (the result comes from poll),
+        else the status the model ended with before any iteration."""
+        p, keep = _problem(n, hrow, hcol, hval, cost, lower, upper, offset, maximize)
+        x = np.zeros(int(n), dtype=np.float64)
+        st = C.c_int32()
+        self.out[int(tag)] = x
+        try:
+            _raise(lib().hgqp_server_submit_problem(self.handle, C.byref(p), float(tolerance), int(iteration_limit),
+                                                    int(tag), _ptr(x, C.c_double), C.byref(st)))
+        except Exception:
+            self.out.pop(int(tag), None)
+            raise
+        if STATUS[st.value] != "running":
+            self.out.pop(int(tag), None)
+        return STATUS[st.value]
 
     def poll(self):
         out = []

@@ -375,6 +375,20 @@ int hgqp_server_pending(hgqp_server* server, int64_t* pending);
 /* Finishes resident and queued work, then stops the thread. */
 int hgqp_server_stop(hgqp_server* server);
 
+/* Submits a model in the reference's own form (hgqp_problem: lower-triangle
+ * triplets with full values, original sense) to the serving loop. Runs the host
+ * side of solve_accelerated_box first (accelerated_qp.cpp:14-35): validation,
+ * the convexity gate, sign normalization, the mirrored CSR and L. A model that
+ * ends there is not submitted; *status then holds
+ * HGQP_STATUS_UNSUPPORTED_CURVATURE or HGQP_STATUS_INFEASIBLE_BOUNDS. Otherwise
+ * *status is HGQP_STATUS_RUNNING and the result arrives through
+ * hgqp_server_poll under `tag`. Invalid models return HGQP_INVALID_ARGUMENT, a
+ * zero Hessian HGQP_UNSUPPORTED (accelerated_qp.cpp:28), as hgqp_solve_qp. This
+ * is the entry the reference's FastAPI service would bind for box QPs
+ * (02-software/backend/solve.py:172-183 routes every model to one engine). */
+int hgqp_server_submit_problem(hgqp_server* server, const hgqp_problem* problem, double tolerance,
+                               uint64_t iteration_limit, int64_t tag, double* x_out, int32_t* status);
+
 /* ---- planted instance generators (inputs for tests and benchmarks) ------ */
 
 /* The reference's seeded tridiagonal family (scale_benchmark.cpp:13-18):
