@@ -196,14 +196,14 @@ def serve_workload(args):
     print(f"# serve: {args.requests} requests at {args.rate}/s over {arrivals[-1]:.1f}s; "
           f"{len(distinct)} distinct instances", file=sys.stderr, flush=True)
 
-    def pool_server():
+    def pool_server(arrivals, count):
         srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
                           check_every=args.check_every)
         done = {}
         t0 = time.perf_counter()
         try:
             def producer():
-                for j in range(args.requests):
+                for j in range(count):
                     delay = t0 + arrivals[j] - time.perf_counter()
                     if delay > 0:
                         time.sleep(delay)
@@ -211,7 +211,7 @@ def serve_workload(args):
                     srv.submit(distinct[k], tag=j, tolerance=KKT_TOL, lipschitz_value=Ls[k])
             th = threading.Thread(target=producer)
             th.start()
-            while len(done) < args.requests:
+            while len(done) < count:
                 for r in srv.poll():
                     done[r["tag"]] = r
                 time.sleep(0.0005)
@@ -219,20 +219,20 @@ def serve_workload(args):
             wall = time.perf_counter() - t0
         finally:
             srv.stop()
-        lat = np.array([done[j]["latency_seconds"] for j in range(args.requests)]) * 1e3
-        ok = all(done[j]["status"] == "optimal" for j in range(args.requests))
+        lat = np.array([done[j]["latency_seconds"] for j in range(count)]) * 1e3
+        ok = all(done[j]["status"] == "optimal" for j in range(count))
         err = max(float(np.abs(done[j]["primal"] - distinct[int(pick[j])]["optimum"]).max())
-                  for j in range(0, args.requests, max(1, args.requests // 200)))
-        return {"solves_per_s": args.requests / wall, "p50_ms": float(np.percentile(lat, 50)),
+                  for j in range(0, count, max(1, count // 200)))
+        return {"requests": count, "solves_per_s": count / wall, "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "all_verified": ok, "wall_s": wall,
                 "sampled_max_abs_err_vs_planted_x": err}
 
-    def plain_loop():
+    def plain_loop(arrivals, count):
         """One request at a time in arrival order: upload, FP64 reference-schedule solve, read x."""
         lat = []
         ok = True
         t0 = time.perf_counter()
-        for j in range(args.requests):
+        for j in range(count):
             now = time.perf_counter() - t0
             if now < arrivals[j]:
                 time.sleep(arrivals[j] - now)
@@ -245,12 +245,24 @@ def serve_workload(args):
             lat.append(time.perf_counter() - t0 - arrivals[j])
         wall = time.perf_counter() - t0
         lat = np.array(lat) * 1e3
-        return {"solves_per_s": args.requests / wall, "p50_ms": float(np.percentile(lat, 50)),
+        return {"requests": count, "solves_per_s": count / wall, "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "all_verified": bool(ok), "wall_s": wall}
 
-    served = pool_server()
-    plain = plain_loop() if not args.no_plain else None
-    print(f"# served: {served}; plain: {plain}", file=sys.stderr, flush=True)
+    served = pool_server(arrivals, args.requests)
+    print(f"# served (real time): {served}", file=sys.stderr, flush=True)
+    plain = plain_loop(arrivals, args.requests) if not args.no_plain else None
+    print(f"# plain (real time): {plain}", file=sys.stderr, flush=True)
+    # capacity: the first requests of the trace, all queued at t=0
+    saturated = None
+    if args.saturate:
+        at_once = np.zeros(args.saturate)
+        sat_served = pool_server(at_once, args.saturate)
+        print(f"# served (saturated): {sat_served}", file=sys.stderr, flush=True)
+        sat_plain = plain_loop(at_once, args.saturate) if not args.no_plain else None
+        print(f"# plain (saturated): {sat_plain}", file=sys.stderr, flush=True)
+        saturated = {"requests": args.saturate, "served": sat_served, "plain_fp64_one_at_a_time": sat_plain,
+                     "speedup_solves_per_s": (sat_served["solves_per_s"] / sat_plain["solves_per_s"]
+                                              if sat_plain else None)}
     return {
         "metric": METRIC, "value": round(served["solves_per_s"], 2), "unit": "solves/s", "n_gpus": 1,
         "steps": args.requests, "warmup": 0, "ms_per_step": round(served["wall_s"] / args.requests * 1e3, 3),
@@ -261,7 +273,8 @@ def serve_workload(args):
                    "slots": args.slots, "budget_rows": args.budget_rows, "budget_nnz": args.budget_nnz,
                    "distinct_instances": args.distinct, "check_every": args.check_every},
         "p50_ms": round(served["p50_ms"], 3), "p99_ms": round(served["p99_ms"], 3),
-        "served": served, "plain_fp64_one_at_a_time": plain, "device": hgqp.device_name(),
+        "served": served, "plain_fp64_one_at_a_time": plain, "saturated": saturated,
+        "device": hgqp.device_name(),
     }
 
 
@@ -340,6 +353,8 @@ def main():
     ap.add_argument("--budget-rows", type=int, default=8_000_000, help="serve: pool row budget")
     ap.add_argument("--budget-nnz", type=int, default=160_000_000, help="serve: pool entry budget")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-at-a-time FP64 loop")
+    ap.add_argument("--saturate", type=int, default=4000,
+                    help="serve: also replay this many requests queued at once (capacity); 0 = skip")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (instances would shard "

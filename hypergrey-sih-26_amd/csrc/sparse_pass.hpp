@@ -96,18 +96,24 @@ __device__ inline void row_block(const int* __restrict__ start, const int* __res
     constexpr int CAP = kWave * LOADS;
     const int k0 = start[r0], k1 = start[r1];
     if (k1 - k0 <= CAP) {
-        T p[LOADS];
-        int cidx[LOADS];
+        // Every load is unconditional so that all LOADS gathers of a lane are in
+        // flight together (a guarded gather becomes a branch with a wait in it).
+        // Lanes past the block re-read its last entry; their products land in
+        // stage slots no row sums. A block of empty rows loads nothing.
+        if (k1 > k0) {
+            const int last = k1 - 1;
+            T p[LOADS], g[LOADS];
+            int cidx[LOADS];
 #pragma unroll
-        for (int j = 0; j < LOADS; ++j) {
-            const int k = k0 + j * kWave + lane;
-            cidx[j] = k < k1 ? load_stream(col + k) : -1;
-            p[j] = k < k1 ? load_stream(val + k) : T(0);
-        }
+            for (int j = 0; j < LOADS; ++j) {
+                const int k = min(k0 + j * kWave + lane, last);
+                cidx[j] = load_stream(col + k);
+                p[j] = load_stream(val + k);
+            }
 #pragma unroll
-        for (int j = 0; j < LOADS; ++j) {
-            T g = cidx[j] >= 0 ? gat[cidx[j]] : T(0);
-            lds[j * kWave + lane] = p[j] * g;  // rounded product, as in the CPU sum
+            for (int j = 0; j < LOADS; ++j) g[j] = gat[cidx[j]];
+#pragma unroll
+            for (int j = 0; j < LOADS; ++j) lds[j * kWave + lane] = p[j] * g[j];  // rounded product, as in the CPU sum
         }
         wave_sync();
         const int nrows = r1 - r0;
@@ -147,11 +153,19 @@ __device__ inline void row_block(const int* __restrict__ start, const int* __res
         T s = 0;
         for (int base = k0; base < k1; base += CAP) {
             if (ORDERED) {
+                const int last = k1 - 1;
+                T p[LOADS], g[LOADS];
+                int cidx[LOADS];
 #pragma unroll
                 for (int j = 0; j < LOADS; ++j) {
-                    const int k = base + j * kWave + lane;
-                    lds[j * kWave + lane] = k < k1 ? load_stream(val + k) * gat[load_stream(col + k)] : T(0);
+                    const int k = min(base + j * kWave + lane, last);
+                    cidx[j] = load_stream(col + k);
+                    p[j] = load_stream(val + k);
                 }
+#pragma unroll
+                for (int j = 0; j < LOADS; ++j) g[j] = gat[cidx[j]];
+#pragma unroll
+                for (int j = 0; j < LOADS; ++j) lds[j * kWave + lane] = p[j] * g[j];
                 wave_sync();
                 if (lane == 0) {
                     const int e = min(CAP, k1 - base);
@@ -159,11 +173,20 @@ __device__ inline void row_block(const int* __restrict__ start, const int* __res
                 }
                 wave_sync();
             } else {
+                const int last = k1 - 1;
+                T p[LOADS], g[LOADS];
+                int cidx[LOADS];
 #pragma unroll
                 for (int j = 0; j < LOADS; ++j) {
-                    const int k = base + j * kWave + lane;
-                    if (k < k1) s += load_stream(val + k) * gat[load_stream(col + k)];
+                    const int k = min(base + j * kWave + lane, last);
+                    cidx[j] = load_stream(col + k);
+                    p[j] = load_stream(val + k);
                 }
+#pragma unroll
+                for (int j = 0; j < LOADS; ++j) g[j] = gat[cidx[j]];
+#pragma unroll
+                for (int j = 0; j < LOADS; ++j)
+                    if (base + j * kWave + lane < k1) s += p[j] * g[j];
             }
         }
         if (!ORDERED) {
