@@ -42,9 +42,12 @@ typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTile = 32;     // output tile: 32 rows x 32 RHS per workgroup
-constexpr int kRowPad = 128;  // n is padded to a multiple (4 waves x 32 k)
-constexpr int kRhsPad = 32;
+constexpr int kTile = 32;     // FP64 GEMM output tile: 32 rows x 32 RHS per workgroup
+constexpr int kRowPad = 256;  // n is padded to a multiple (4 k-splits x 64 k)
+constexpr int kRhsPad = 64;
+constexpr int kBM = 64, kBN = 64, kBK = 64;  // BF16 GEMM: rows x RHS tile, k per LDS stage
+constexpr int kLdsRow = kBK + 8;             // LDS row stride in bf16 (144 bytes: 16-byte aligned, bank-spread)
+constexpr int kMaxSplits = 4;
 constexpr int kSeg = 256;     // rows per reduction segment of the elementwise kernels
 constexpr int kXcds = 8;      // accelerator complex dies; workgroups go round-robin over them
 
@@ -77,72 +80,77 @@ struct Rhs {
     int32_t verified, finite;
 };
 
-// Gt[j][r] = sum_k Q[r][k] W[j][k] with BF16 operands, FP32 accumulation. The
-// four waves of a workgroup split k in quarters and meet in LDS.
-// Row j of W is the extrapolated correction of RHS j, or its correction after a
-// restart (the gradient is taken where the step will take it).
-__global__ __launch_bounds__(256) void gemm_bf16(int n, const bf16* __restrict__ Q, const bf16* __restrict__ Wt,
-                                                 const bf16* __restrict__ Dt, const Rhs* __restrict__ rhs, int parity,
-                                                 float* __restrict__ Gt) {
-    __shared__ float red[4][kTile][kTile + 1];
-    const int wave = threadIdx.x / 64, l = threadIdx.x % 64;
-    int rtile, ctile;
-    tile_of(blockIdx.x, n / kTile, gridDim.x / (n / kTile), &rtile, &ctile);
-    const int r0 = rtile * kTile, c0 = ctile * kTile;
-    const int quarter = n / 4, kb = wave * quarter, ke = kb + quarter;
+// G[split][j][r] = sum over the split's k range of Q[r][k] W[j][k], BF16
+// operands, FP32 accumulation. Row j of W is the extrapolated correction of
+// RHS j, or its correction after a restart (the gradient is taken where the
+// step will take it). A workgroup computes a 64-row x 64-RHS tile: each k stage
+// of 64 moves through LDS (every wave loads eight full 128-byte row segments),
+// the next stage is held in registers meanwhile, and each wave multiplies a
+// 32 x 32 quarter of the tile with 2 x 2 MFMA blocks. Launch: grid
+// (n/64, K/64, splits), 256 threads; n/splits must be a multiple of 64.
+__global__ __launch_bounds__(256) void gemm_bf16(int n, int K, const bf16* __restrict__ Q,
+                                                 const bf16* __restrict__ Wt, const bf16* __restrict__ Dt,
+                                                 const Rhs* __restrict__ rhs, int parity, float* __restrict__ G) {
+    __shared__ __attribute__((aligned(16))) bf16 As[2][kBM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][kBN * kLdsRow];
+    const int t = threadIdx.x, wave = t / 64, l = t % 64;
+    const int r0 = blockIdx.x * kBM, c0 = blockIdx.y * kBN;
+    const int span = n / gridDim.z, kb = blockIdx.z * span, stages = span / kBK;
+    // staging: thread t moves rows t/8 and t/8 + 32 of each tile, 16 bytes at k offset 8 (t % 8)
+    const int lr = t / 8, lc = (t % 8) * 8;
+    const bf16* qa0 = Q + size_t(r0 + lr) * n + kb + lc;
+    const bf16* qa1 = qa0 + size_t(32) * n;
+    const int j0 = c0 + lr, j1 = j0 + 32;
+    const bf16* wb0 = (rhs[j0].use_x[parity] ? Dt : Wt) + size_t(j0) * n + kb + lc;
+    const bf16* wb1 = (rhs[j1].use_x[parity] ? Dt : Wt) + size_t(j1) * n + kb + lc;
+    bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(qa0), ra1 = *reinterpret_cast<const bf16x8*>(qa1);
+    bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(wb0), rb1 = *reinterpret_cast<const bf16x8*>(wb1);
+    auto stash = [&](int buf) {
+        *reinterpret_cast<bf16x8*>(&As[buf][lr * kLdsRow + lc]) = ra0;
+        *reinterpret_cast<bf16x8*>(&As[buf][(lr + 32) * kLdsRow + lc]) = ra1;
+        *reinterpret_cast<bf16x8*>(&Bs[buf][lr * kLdsRow + lc]) = rb0;
+        *reinterpret_cast<bf16x8*>(&Bs[buf][(lr + 32) * kLdsRow + lc]) = rb1;
+    };
+    stash(0);
+    __syncthreads();
+    const int wr = (wave / 2) * 32, wc = (wave % 2) * 32;  // this wave's quarter of the tile
     f32x4 acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
-    const bf16* qa0 = Q + size_t(r0 + l % 16) * n + (l / 16) * 8;
-    const bf16* qa1 = qa0 + size_t(16) * n;
-    const int j0 = c0 + l % 16, j1 = j0 + 16;
-    const bf16* wb0 = (rhs[j0].use_x[parity] ? Dt : Wt) + size_t(j0) * n + (l / 16) * 8;
-    const bf16* wb1 = (rhs[j1].use_x[parity] ? Dt : Wt) + size_t(j1) * n + (l / 16) * 8;
-    // four k-chunks of operands per batch; the next batch is in flight while
-    // the current one feeds the matrix cores. The quarter is a multiple of 32
-    // only, so chunks past it load zeros (their products add nothing).
-    constexpr int B = 4;
-    bf16x8 a[2][B], b[2][B], na[2][B], nb[2][B];
-    auto load = [&](int k, bf16x8 (&A)[2][B], bf16x8 (&Bv)[2][B]) {
+    for (int st = 0; st < stages; ++st) {
+        const int cur = st & 1;
+        const bool more = st + 1 < stages;
+        if (more) {
+            const int off = (st + 1) * kBK;
+            ra0 = *reinterpret_cast<const bf16x8*>(qa0 + off);
+            ra1 = *reinterpret_cast<const bf16x8*>(qa1 + off);
+            rb0 = *reinterpret_cast<const bf16x8*>(wb0 + off);
+            rb1 = *reinterpret_cast<const bf16x8*>(wb1 + off);
+        }
 #pragma unroll
-        for (int u = 0; u < B; ++u) {
-            if (k + 32 * u < ke) {  // wave-uniform
-                A[0][u] = *reinterpret_cast<const bf16x8*>(qa0 + k + 32 * u);
-                A[1][u] = *reinterpret_cast<const bf16x8*>(qa1 + k + 32 * u);
-                Bv[0][u] = *reinterpret_cast<const bf16x8*>(wb0 + k + 32 * u);
-                Bv[1][u] = *reinterpret_cast<const bf16x8*>(wb1 + k + 32 * u);
-            } else {
-                A[0][u] = A[1][u] = Bv[0][u] = Bv[1][u] = bf16x8{};
+        for (int ks = 0; ks < kBK; ks += 32) {
+            bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                fa[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wr + i * 16 + l % 16) * kLdsRow + ks + 8 * (l / 16)]);
+                fb[i] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wc + i * 16 + l % 16) * kLdsRow + ks + 8 * (l / 16)]);
             }
-        }
-    };
-    load(kb, a, b);
-    for (int k = kb; k < ke; k += 32 * B) {
-        if (k + 32 * B < ke) load(k + 32 * B, na, nb);
 #pragma unroll
-        for (int u = 0; u < B; ++u) {
-            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][u], b[0][u], acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][u], b[1][u], acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][u], b[0][u], acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][u], b[1][u], acc[1][1], 0, 0, 0);
-        }
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int u = 0; u < B; ++u) {
-            a[0][u] = na[0][u];
-            a[1][u] = na[1][u];
-            b[0][u] = nb[0][u];
-            b[1][u] = nb[1][u];
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
+        if (more) stash(cur ^ 1);
+        __syncthreads();
     }
+    // lane l holds rows 4(l/16) .. +4 of RHS l % 16 in each 16 x 16 block
+    float* out = G + size_t(blockIdx.z) * K * n;
     for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            for (int e = 0; e < 4; ++e) red[wave][i * 16 + (l / 16) * 4 + e][j * 16 + l % 16] = acc[i][j][e];
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < kTile * kTile; idx += 256) {
-        const int c = idx / kTile, r = idx % kTile;  // consecutive threads: consecutive rows of one RHS
-        const float s = (red[0][r][c] + red[1][r][c]) + (red[2][r][c] + red[3][r][c]);
-        Gt[size_t(c0 + c) * n + r0 + r] = s;
-    }
+        for (int j = 0; j < 2; ++j) {
+            const int r = r0 + wr + i * 16 + 4 * (l / 16), jj = c0 + wc + j * 16 + l % 16;
+            *reinterpret_cast<f32x4*>(&out[size_t(jj) * n + r]) = acc[i][j];
+        }
 }
 
 // Gt[j][r] = sum_k Q[r][k] X[j][k] in FP64 on the matrix cores. A lane takes
@@ -224,13 +232,14 @@ __global__ __launch_bounds__(256) void gemv_f64(int n, const double* __restrict_
 
 struct DenseView {
     int n, K, nseg;
+    int splits;        // k splits of the BF16 GEMM: g32 holds that many partial slices
     double lipschitz;
     const double* c;   // [K][n]
     const double* lo;  // [n]
     const double* up;  // [n]
     double* x;         // [K][n] FP64 anchor / iterate
     double* g64;       // [K][n] Q x (FP64)
-    float* g32;        // [K][n] Q w (BF16 GEMM)
+    float* g32;        // [splits][K][n] Q w (BF16 GEMM partials over k ranges)
     float* ga;         // [K][n] anchor gradient
     float* lo32;       // [K][n] lower - anchor
     float* up32;       // [K][n] upper - anchor
@@ -258,7 +267,9 @@ __global__ __launch_bounds__(256) void dense_step(DenseView v, int parity) {
         if (r < v.n) {
             const float* gat = s->use_x[parity] ? v.d[parity] : v.w[parity];
             const float wv = gat[o + r], xv = v.d[parity][o + r];
-            const float next = clip(wv - (v.g32[o + r] + v.ga[o + r]) / L, v.lo32[o + r], v.up32[o + r]);
+            float gq = v.g32[o + r];  // k-split partials in split order
+            for (int sp = 1; sp < v.splits; ++sp) gq += v.g32[size_t(sp) * v.K * v.n + o + r];
+            const float next = clip(wv - (gq + v.ga[o + r]) / L, v.lo32[o + r], v.up32[o + r]);
             const float wn = next + b * (next - xv);
             v.d[parity ^ 1][o + r] = next;
             v.w[parity ^ 1][o + r] = wn;
@@ -459,6 +470,31 @@ void check_dense(const hgqp_dense* d) {
     if (!d) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_dense handle");
 }
 
+void launch_gemm_bf16(hgqp_dense* d, int parity, int splits) {
+    // shapes the kernel assumes: 64-row and 64-RHS tiles, k splits of whole 64-stages
+    if (d->np % kBM || d->kp % kBN || splits < 1 || splits > kMaxSplits || (d->np / splits) % kBK)
+        throw Error(HGQP_DEVICE_ERROR, "hgqp_dense: GEMM shape does not fit the tiling");
+    hipLaunchKernelGGL(gemm_bf16, dim3(d->np / kBM, d->kp / kBN, splits), dim3(256), 0, d->stream, d->np, d->kp,
+                       d->q16, d->v.wb, d->v.db, d->v.rhs, parity, d->v.g32);
+}
+
+// Average time of one BF16 GEMM launch with `splits` k splits.
+double time_gemm_bf16(hgqp_dense* d, int splits, int reps) {
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    launch_gemm_bf16(d, 0, splits);
+    HIPCHECK(hipEventRecord(a, d->stream));
+    for (int r = 0; r < reps; ++r) launch_gemm_bf16(d, 0, splits);
+    HIPCHECK(hipEventRecord(b, d->stream));
+    HIPCHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms) * 1e3 / reps;
+}
+
 void read_rhs(hgqp_dense* d) {
     HIPCHECK(hipMemcpyAsync(d->host_rhs.data(), d->v.rhs, sizeof(Rhs) * d->kp, hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipStreamSynchronize(d->stream));
@@ -523,7 +559,8 @@ int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, co
         for (int64_t j = 0; j < k; ++j) std::memcpy(&cp[size_t(j) * N], c + j * n, size_t(n) * 8);
         HIPCHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
         size_t bytes = Arena::span(N * N, 8) + Arena::span(N * N, 2) + 3 * Arena::span(KP * N, 8) + 2 * Arena::span(N, 8) +
-                       8 * Arena::span(KP * N, 4) + 2 * Arena::span(KP * N, 2) + Arena::span(KP, sizeof(Rhs)) +
+                       7 * Arena::span(KP * N, 4) + Arena::span(kMaxSplits * KP * N, 4) + 2 * Arena::span(KP * N, 2) +
+                       Arena::span(KP, sizeof(Rhs)) +
                        Arena::span(KP * nseg * kPartial, 8) + 2 * Arena::span(N, 8);
         d->arena.reserve(bytes);
         d->q64 = d->arena.take<double>(N * N);
@@ -540,7 +577,8 @@ int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, co
         v.up = updev;
         v.x = d->arena.take<double>(KP * N);
         v.g64 = d->arena.take<double>(KP * N);
-        v.g32 = d->arena.take<float>(KP * N);
+        v.g32 = d->arena.take<float>(kMaxSplits * KP * N);
+        v.splits = 1;
         v.ga = d->arena.take<float>(KP * N);
         v.lo32 = d->arena.take<float>(KP * N);
         v.up32 = d->arena.take<float>(KP * N);
@@ -589,6 +627,19 @@ int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, co
         }
         v.lipschitz = d->lipschitz;
         d->host_rhs.resize(KP);
+        // k splits of the BF16 GEMM: the fastest of 1, 2, 4 on this shape
+        HIPCHECK(hipMemsetAsync(v.wb, 0, KP * N * 2, d->stream));
+        HIPCHECK(hipMemsetAsync(v.db, 0, KP * N * 2, d->stream));
+        HIPCHECK(hipMemsetAsync(v.rhs, 0, KP * sizeof(Rhs), d->stream));
+        double best = std::numeric_limits<double>::infinity();
+        for (int sp = 1; sp <= kMaxSplits; sp *= 2) {
+            if ((d->np / sp) % kBK) continue;
+            const double us = time_gemm_bf16(d.get(), sp, 10);
+            if (us < best) {
+                best = us;
+                v.splits = sp;
+            }
+        }
         HIPCHECK(hipStreamSynchronize(d->stream));
         *out = d.release();
     });
@@ -603,7 +654,7 @@ int hgqp_dense_solve(hgqp_dense* d, double tolerance, uint64_t iteration_limit, 
         const auto t0 = std::chrono::steady_clock::now();
         const DenseView& v = d->v;
         const size_t N = size_t(d->np), KP = size_t(d->kp);
-        const dim3 tiles((d->np / kTile) * (d->kp / kTile)), segs(v.nseg, d->kp);
+        const dim3 segs(v.nseg, d->kp);
         const int64_t limit = int64_t(std::min<uint64_t>(iteration_limit, uint64_t(1) << 62));
         // x0 = clip(0, l, u) for every RHS, fresh state
         {
@@ -636,8 +687,7 @@ int hgqp_dense_solve(hgqp_dense* d, double tolerance, uint64_t iteration_limit, 
             // FP32 / BF16 correction steps until every active RHS froze
             for (;;) {
                 for (int s = 0; s < d->opt.check_every; ++s) {
-                    hipLaunchKernelGGL(gemm_bf16, tiles, dim3(256), 0, d->stream, d->np, d->q16, v.wb, v.db, v.rhs,
-                                       parity, v.g32);
+                    launch_gemm_bf16(d, parity, v.splits);
                     hipLaunchKernelGGL(dense_step, segs, dim3(256), 0, d->stream, v, parity);
                     hipLaunchKernelGGL(dense_finalize_step, dim3(d->kp), dim3(64), 0, d->stream, v, parity, limit);
                     parity ^= 1;
@@ -707,23 +757,7 @@ int hgqp_dense_time_gemm(hgqp_dense* d, int32_t reps, double* avg_us, double* tf
     return guarded([&] {
         check_dense(d);
         if (reps < 1 || !avg_us) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_dense_time_gemm: bad argument");
-        const dim3 tiles((d->np / kTile) * (d->kp / kTile));
-        hipEvent_t a, b;
-        HIPCHECK(hipEventCreate(&a));
-        HIPCHECK(hipEventCreate(&b));
-        hipLaunchKernelGGL(gemm_bf16, tiles, dim3(256), 0, d->stream, d->np, d->q16, d->v.wb, d->v.db, d->v.rhs, 0,
-                           d->v.g32);
-        HIPCHECK(hipEventRecord(a, d->stream));
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(gemm_bf16, tiles, dim3(256), 0, d->stream, d->np, d->q16, d->v.wb, d->v.db, d->v.rhs,
-                               0, d->v.g32);
-        HIPCHECK(hipEventRecord(b, d->stream));
-        HIPCHECK(hipEventSynchronize(b));
-        float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, a, b));
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        *avg_us = double(ms) * 1e3 / reps;
+        *avg_us = time_gemm_bf16(d, d->v.splits, reps);
         if (tflops) *tflops = 2.0 * double(d->np) * double(d->np) * double(d->kp) / (*avg_us * 1e-6) / 1e12;
     });
 }
