@@ -192,30 +192,39 @@ def serve_workload(args):
                                  (args.seed << 20) + k, -1.0, 1.0) for k in range(args.distinct)]
     Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in distinct]
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
-    pick = rng.integers(0, len(distinct), size=args.requests)
+    pick = rng.integers(0, len(distinct), size=max(args.requests, args.saturate))
     print(f"# serve: {args.requests} requests at {args.rate}/s over {arrivals[-1]:.1f}s; "
           f"{len(distinct)} distinct instances", file=sys.stderr, flush=True)
 
-    def pool_server(arrivals, count):
+    def pool_server(arrivals, count, producers=1):
         srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
                           check_every=args.check_every)
         done = {}
         t0 = time.perf_counter()
         try:
-            def producer():
-                for j in range(count):
-                    delay = t0 + arrivals[j] - time.perf_counter()
-                    if delay > 0:
-                        time.sleep(delay)
-                    k = int(pick[j])
-                    srv.submit(distinct[k], tag=j, tolerance=KKT_TOL, lipschitz_value=Ls[k])
-            th = threading.Thread(target=producer)
-            th.start()
+            failed = []
+
+            def producer(first):
+                try:
+                    for j in range(first, count, producers):
+                        delay = t0 + arrivals[j] - time.perf_counter()
+                        if delay > 0:
+                            time.sleep(delay)
+                        k = int(pick[j])
+                        srv.submit(distinct[k], tag=j, tolerance=KKT_TOL, lipschitz_value=Ls[k])
+                except BaseException as e:  # surfaced by the poll loop below
+                    failed.append(e)
+            threads = [threading.Thread(target=producer, args=(i,)) for i in range(producers)]
+            for th in threads:
+                th.start()
             while len(done) < count:
+                if failed:
+                    raise RuntimeError(f"request producer failed: {failed[0]!r}")
                 for r in srv.poll():
                     done[r["tag"]] = r
                 time.sleep(0.0005)
-            th.join()
+            for th in threads:
+                th.join()
             wall = time.perf_counter() - t0
         finally:
             srv.stop()
@@ -256,7 +265,7 @@ def serve_workload(args):
     saturated = None
     if args.saturate:
         at_once = np.zeros(args.saturate)
-        sat_served = pool_server(at_once, args.saturate)
+        sat_served = pool_server(at_once, args.saturate, producers=4)  # as the HTTP route submits
         print(f"# served (saturated): {sat_served}", file=sys.stderr, flush=True)
         sat_plain = plain_loop(at_once, args.saturate) if not args.no_plain else None
         print(f"# plain (saturated): {sat_plain}", file=sys.stderr, flush=True)

@@ -63,6 +63,7 @@ struct SlotHost {
     int64_t n = 0, nnz = 0;
     int64_t row0 = -1, ent0 = -1, blk0 = -1, nblk = 0;
     double* x_out = nullptr;
+    int32_t gen = 0;             // admission number, mirrored in the device slot entry
     std::vector<int32_t> start;  // local row starts (kept for re-blocking)
 };
 
@@ -85,6 +86,8 @@ struct hgqp_pool {
     int* blk_r1 = nullptr;
     int* blk_inst = nullptr;
     PoolInst* table_host = nullptr;  // pinned mirror of the slot table
+    int32_t* free_phase = nullptr;   // pinned constant kSlotFree (source of slot releases)
+    int32_t next_gen = 1;
     hipEvent_t table_event = nullptr;
     bool table_pending = false;
     RangeAlloc rows, ents, blks;
@@ -99,6 +102,7 @@ struct hgqp_pool {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (table_event) (void)hipEventDestroy(table_event);
         if (table_host) (void)hipHostFree(table_host);
+        if (free_phase) (void)hipHostFree(free_phase);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -141,25 +145,27 @@ void validate_options(const hgqp_pool_options& o) {
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: fp32_switch must be positive");
 }
 
-// Writes one slot's blocks for the pool's block shape; returns false when the
-// block budget has no room.
-bool place_blocks(hgqp_pool* p, int s) {
+// Computes one slot's row blocks for the pool's block shape into `stage`
+// (3 x nblk ints: first rows, end rows, slot; room for 3 (n + 1)) and enqueues
+// their upload. `stage` must stay valid until the stream passes this point.
+// Returns false when the block budget has no room.
+bool place_blocks(hgqp_pool* p, int s, const int32_t* start, int32_t* stage) {
     SlotHost& h = p->slots[s];
-    std::vector<int> blk = row_blocks(h.n, h.start.data(), kWave * p->loads, p->rows_per_block);
+    std::vector<int> blk = row_blocks(h.n, start, kWave * p->loads, p->rows_per_block);
     const int64_t nb = int64_t(blk.size()) - 1;
     const int64_t b0 = p->blks.alloc(nb);
     if (b0 < 0) return false;
     h.blk0 = b0;
     h.nblk = nb;
-    std::vector<int> r0(nb), r1(nb), inst(nb, s);
+    int32_t *r0 = stage, *r1 = stage + nb, *inst = stage + 2 * nb;
     for (int64_t k = 0; k < nb; ++k) {
-        r0[k] = int(h.row0 + blk[k]);
-        r1[k] = int(h.row0 + blk[k + 1]);
+        r0[k] = int32_t(h.row0 + blk[k]);
+        r1[k] = int32_t(h.row0 + blk[k + 1]);
+        inst[k] = s;
     }
-    HIPCHECK(hipMemcpyAsync(p->blk_r0 + b0, r0.data(), nb * 4, hipMemcpyHostToDevice, p->stream));
-    HIPCHECK(hipMemcpyAsync(p->blk_r1 + b0, r1.data(), nb * 4, hipMemcpyHostToDevice, p->stream));
-    HIPCHECK(hipMemcpyAsync(p->blk_inst + b0, inst.data(), nb * 4, hipMemcpyHostToDevice, p->stream));
-    HIPCHECK(hipStreamSynchronize(p->stream));  // the host vectors go out of scope
+    HIPCHECK(hipMemcpyAsync(p->blk_r0 + b0, r0, nb * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHECK(hipMemcpyAsync(p->blk_r1 + b0, r1, nb * 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHECK(hipMemcpyAsync(p->blk_inst + b0, inst, nb * 4, hipMemcpyHostToDevice, p->stream));
     return true;
 }
 
@@ -171,9 +177,12 @@ void reblock_all(hgqp_pool* p) {
     HIPCHECK(hipMemcpyAsync(table.data(), p->v.inst, table.size() * sizeof(PoolInst), hipMemcpyDeviceToHost,
                             p->stream));
     HIPCHECK(hipStreamSynchronize(p->stream));
+    std::vector<std::vector<int32_t>> stages(p->opt.max_instances);
     for (int s = 0; s < p->opt.max_instances; ++s) {
         if (!p->slots[s].used) continue;
-        if (!place_blocks(p, s)) throw Error(HGQP_OUT_OF_MEMORY, "hgqp_pool: block budget exceeded while re-blocking");
+        stages[s].resize(3 * (p->slots[s].n + 1));
+        if (!place_blocks(p, s, p->slots[s].start.data(), stages[s].data()))
+            throw Error(HGQP_OUT_OF_MEMORY, "hgqp_pool: block budget exceeded while re-blocking");
         table[s].b0 = int(p->slots[s].blk0);
         table[s].nblk = int(p->slots[s].nblk);
     }
@@ -182,7 +191,8 @@ void reblock_all(hgqp_pool* p) {
     HIPCHECK(hipStreamSynchronize(p->stream));
 }
 
-void enqueue_periods(hgqp_pool* p, int periods) {
+// Enqueues `periods` check periods (graph replays after the first capture).
+void launch_periods(hgqp_pool* p, int periods) {
     for (int k = 0; k < periods; ++k) {
         if (p->opt.use_graph) {
             if (!p->graph) {
@@ -202,6 +212,11 @@ void enqueue_periods(hgqp_pool* p, int periods) {
             HIPCHECK(hipGetLastError());
         }
     }
+}
+
+// Periods, then a snapshot of the slot table for hgqp_pool_collect.
+void enqueue_periods(hgqp_pool* p, int periods) {
+    launch_periods(p, periods);
     HIPCHECK(hipMemcpyAsync(p->table_host, p->v.inst, sizeof(PoolInst) * p->opt.max_instances, hipMemcpyDeviceToHost,
                             p->stream));
     HIPCHECK(hipEventRecord(p->table_event, p->stream));
@@ -324,6 +339,8 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     v.restart = o.momentum == HGQP_MOMENTUM_RESTART ? 1 : 0;
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->table_host), sizeof(PoolInst) * S, hipHostMallocDefault));
     std::memset(p->table_host, 0, sizeof(PoolInst) * S);  // all kSlotFree
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->free_phase), sizeof(int32_t), hipHostMallocDefault));
+    *p->free_phase = kSlotFree;
     HIPCHECK(hipMemcpyAsync(v.inst, p->table_host, sizeof(PoolInst) * S, hipMemcpyHostToDevice, p->stream));
     int zero[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(v.counts, zero, sizeof zero, hipMemcpyHostToDevice, p->stream));
@@ -337,64 +354,76 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     return p.release();
 }
 
-// Admission (host side, between periods). Returns HGQP_BUSY through Error.
-int admit(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
-          const double* cost, const double* lower, const double* upper, double lipschitz, int32_t maximize,
-          double offset, double tol, uint64_t limit, int64_t tag, double* x_out) {
+// Checks an instance in the CSR form of hgqp_box_create: sizes, monotone
+// starts, columns in range, lower <= upper, positive finite L and tolerance.
+// Returns whether H is positive diagonal (the verdict then adds the gap test,
+// qp_admm.cpp:115-116).
+bool validate_instance(int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
+                       const double* cost, const double* lower, const double* upper, const double* x_out,
+                       double lipschitz, double tol, const char* who) {
+    const std::string w(who);
     if (n <= 0 || nnz < 0 || !start || (nnz && (!index || !value)) || !cost || !lower || !upper || !x_out)
-        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_admit: null array or bad size");
+        throw Error(HGQP_INVALID_ARGUMENT, w + ": null array or bad size");
+    if (n >= (int64_t(1) << 31) - 1 || nnz >= (int64_t(1) << 31) - 1)
+        throw Error(HGQP_INVALID_ARGUMENT, w + ": instance too large for 32-bit indices");
     if (!(lipschitz > 0) || !std::isfinite(lipschitz))
-        throw Error(HGQP_UNSUPPORTED, "hgqp_pool_admit: needs a positive finite Lipschitz bound");
-    if (!(tol > 0) || !std::isfinite(tol)) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_admit: bad tolerance");
-    if (start[0] != 0 || start[n] != nnz) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_admit: malformed CSR start");
+        throw Error(HGQP_UNSUPPORTED, w + ": needs a positive finite Lipschitz bound");
+    if (!(tol > 0) || !std::isfinite(tol)) throw Error(HGQP_INVALID_ARGUMENT, w + ": bad tolerance");
+    if (start[0] != 0 || start[n] != nnz) throw Error(HGQP_INVALID_ARGUMENT, w + ": malformed CSR start");
     bool diag = true;
     for (int64_t i = 0; i < n; ++i) {
-        if (start[i + 1] < start[i]) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_admit: CSR start not monotone");
+        if (start[i + 1] < start[i]) throw Error(HGQP_INVALID_ARGUMENT, w + ": CSR start not monotone");
         bool pos = false;
         for (int32_t k = start[i]; k < start[i + 1]; ++k) {
-            if (index[k] < 0 || index[k] >= n) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_admit: column out of range");
+            if (index[k] < 0 || index[k] >= n) throw Error(HGQP_INVALID_ARGUMENT, w + ": column out of range");
             if (index[k] == i) pos = value[k] > 0;
             else diag = false;
         }
         diag = diag && pos;
         if (std::isnan(lower[i]) || std::isnan(upper[i]) || lower[i] > upper[i])
-            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_admit: bounds must satisfy lower <= upper");
+            throw Error(HGQP_INVALID_ARGUMENT, w + ": bounds must satisfy lower <= upper");
     }
-    if (p->free_slots.empty()) return HGQP_BUSY;
+    return diag;
+}
+
+// Host part of an admission: a slot with its row, entry and block ranges; the
+// block table is staged in `blk` (room for 3 (n + 1) ints) and its upload
+// enqueued. Returns the slot, or -1 when no slot or no contiguous room in the
+// budget is free (nothing is held then).
+int reserve(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, int32_t* blk) {
+    if (p->free_slots.empty()) return -1;
     const int64_t r0 = p->rows.alloc(n + 1);
-    if (r0 < 0) return HGQP_BUSY;
+    if (r0 < 0) return -1;
     const int64_t e0 = p->ents.alloc(std::max<int64_t>(nnz, 1));
-    if (e0 < 0) { p->rows.release(r0, n + 1); return HGQP_BUSY; }
+    if (e0 < 0) {
+        p->rows.release(r0, n + 1);
+        return -1;
+    }
     const int s = p->free_slots.back();
     SlotHost& h = p->slots[s];
+    h = SlotHost();
     h.n = n;
     h.nnz = nnz;
     h.row0 = r0;
     h.ent0 = e0;
-    h.start.assign(start, start + n + 1);
-    if (!place_blocks(p, s)) {
+    if (!place_blocks(p, s, start, blk)) {
         p->rows.release(r0, n + 1);
         p->ents.release(e0, std::max<int64_t>(nnz, 1));
-        return HGQP_BUSY;
+        h = SlotHost();
+        return -1;
     }
-    p->free_slots.pop_back();
+    h.start.assign(start, start + n + 1);
     h.used = true;
-    h.tag = tag;
-    h.x_out = x_out;
-    // matrix in global numbering: starts offset by e0, columns by r0
-    std::vector<int32_t> gs(n + 1), gc(nnz);
-    for (int64_t i = 0; i <= n; ++i) gs[i] = int32_t(e0 + start[i]);
-    for (int64_t k = 0; k < nnz; ++k) gc[k] = int32_t(r0 + index[k]);
-    hipStream_t st = p->stream;
-    HIPCHECK(hipMemcpyAsync(const_cast<int*>(p->v.start) + r0, gs.data(), (n + 1) * 4, hipMemcpyHostToDevice, st));
-    if (nnz) {
-        HIPCHECK(hipMemcpyAsync(const_cast<int*>(p->v.col) + e0, gc.data(), nnz * 4, hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.val) + e0, value, nnz * 8, hipMemcpyHostToDevice, st));
-        launch_convert<float, double>(nnz, p->v.val + e0, const_cast<float*>(p->v.val32) + e0, st);
-    }
-    HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.c) + r0, cost, n * 8, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.lo) + r0, lower, n * 8, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.up) + r0, upper, n * 8, hipMemcpyHostToDevice, st));
+    h.gen = p->next_gen++;
+    p->free_slots.pop_back();
+    ++p->resident;
+    return s;
+}
+
+// The device slot entry of a new instance: phase INIT (x0 and the iteration-0
+// FP64 check run in the next period).
+PoolInst slot_entry(const SlotHost& h, double lipschitz, double tol, uint64_t limit, int32_t maximize, double offset,
+                    bool diag) {
     PoolInst t;
     std::memset(&t, 0, sizeof t);
     t.lipschitz = lipschitz;
@@ -409,17 +438,101 @@ int admit(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, const int3
     t.diagonal = diag ? 1 : 0;
     t.b0 = int(h.blk0);
     t.nblk = int(h.nblk);
-    t.r0 = int(r0);
-    t.n = int(n);
-    HIPCHECK(hipMemcpyAsync(p->v.inst + s, &t, sizeof t, hipMemcpyHostToDevice, st));
+    t.r0 = int(h.row0);
+    t.n = int(h.n);
+    t.gen = h.gen;
+    return t;
+}
+
+// Device part of an admission, enqueued on the pool stream: the uploads, the
+// move to pool numbering, the FP32 values and the slot entry. All host memory
+// named here must stay valid until the stream passes this point.
+void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, const double* value, const double* cost,
+            const double* lower, const double* upper, const PoolInst* entry) {
+    const SlotHost& h = p->slots[s];
+    hipStream_t st = p->stream;
+    int* dstart = const_cast<int*>(p->v.start) + h.row0;
+    int* dcol = const_cast<int*>(p->v.col) + h.ent0;
+    HIPCHECK(hipMemcpyAsync(dstart, start, (h.n + 1) * 4, hipMemcpyHostToDevice, st));
+    if (h.nnz) {
+        HIPCHECK(hipMemcpyAsync(dcol, index, h.nnz * 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.val) + h.ent0, value, h.nnz * 8, hipMemcpyHostToDevice, st));
+        launch_convert<float, double>(h.nnz, p->v.val + h.ent0, const_cast<float*>(p->v.val32) + h.ent0, st);
+    }
+    launch_pool_offset(dstart, h.n, int(h.ent0), dcol, h.nnz, int(h.row0), st);
+    HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.c) + h.row0, cost, h.n * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.lo) + h.row0, lower, h.n * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.up) + h.row0, upper, h.n * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(p->v.inst + s, entry, sizeof(PoolInst), hipMemcpyHostToDevice, st));
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(st));  // gs / gc / t are host temporaries
-    ++p->resident;
+}
+
+// The public admission (hgqp_pool_admit): validated, staged in host
+// temporaries, synchronous.
+int admit(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
+          const double* cost, const double* lower, const double* upper, double lipschitz, int32_t maximize,
+          double offset, double tol, uint64_t limit, int64_t tag, double* x_out) {
+    const bool diag =
+        validate_instance(n, nnz, start, index, value, cost, lower, upper, x_out, lipschitz, tol, "hgqp_pool_admit");
+    std::vector<int32_t> blk(3 * (n + 1));
+    const int s = reserve(p, n, nnz, start, blk.data());
+    if (s < 0) return HGQP_BUSY;
+    SlotHost& h = p->slots[s];
+    h.tag = tag;
+    h.x_out = x_out;
+    const PoolInst entry = slot_entry(h, lipschitz, tol, limit, maximize, offset, diag);
+    upload(p, s, start, index, value, cost, lower, upper, &entry);
+    HIPCHECK(hipStreamSynchronize(p->stream));  // blk and entry are host temporaries
     return HGQP_OK;
 }
 
-// Reads the table after the last enqueued periods; finished slots are copied
-// out, freed and queued in p->ready.
+// Result record of a finished slot from its table entry.
+hgqp_pool_result result_of(const SlotHost& h, const PoolInst& t, int s) {
+    hgqp_pool_result r;
+    std::memset(&r, 0, sizeof r);
+    r.tag = h.tag;
+    r.slot = s;
+    r.status = t.status;
+    r.iterations = uint64_t(t.iter);
+    r.restarts = t.restarts;
+    r.checks = t.checks;
+    r.cycles = t.cycle + 1;
+    r.check.objective = t.objective;
+    r.check.primal_residual = t.primal_residual;
+    r.check.stationarity_residual = t.stationarity;
+    r.check.complementarity_residual = std::isfinite(t.objective) ? 0.0 : std::numeric_limits<double>::infinity();
+    r.check.dual_bound = t.dual_bound;
+    r.check.relative_gap = t.relative_gap;
+    r.check.optimality_verified = t.verified;
+    r.check.finite = t.finite;
+    return r;
+}
+
+// Frees a finished slot: its device entry goes FREE in stream order, its ranges
+// and slot return to the host allocators at once (later uploads are ordered
+// after the release on the stream).
+void release_slot(hgqp_pool* p, int s) {
+    SlotHost& h = p->slots[s];
+    HIPCHECK(hipMemcpyAsync(&p->v.inst[s].phase, p->free_phase, sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+    p->rows.release(h.row0, h.n + 1);
+    p->ents.release(h.ent0, std::max<int64_t>(h.nnz, 1));
+    p->blks.release(h.blk0, h.nblk);
+    h = SlotHost();
+    p->free_slots.push_back(s);
+    --p->resident;
+}
+
+// Slots that a table snapshot reports finished and that still hold the
+// instance the snapshot saw.
+std::vector<int> finished_slots(const hgqp_pool* p, const PoolInst* table) {
+    std::vector<int> done;
+    for (int s = 0; s < p->opt.max_instances; ++s)
+        if (p->slots[s].used && table[s].phase == kSlotDone && table[s].gen == p->slots[s].gen) done.push_back(s);
+    return done;
+}
+
+// hgqp_pool_collect: reads the slot table after the queued periods, copies x of
+// finished slots to their callers and frees the slots.
 void harvest(hgqp_pool* p) {
     if (!p->table_pending) {
         HIPCHECK(hipMemcpyAsync(p->table_host, p->v.inst, sizeof(PoolInst) * p->opt.max_instances,
@@ -428,125 +541,227 @@ void harvest(hgqp_pool* p) {
     }
     HIPCHECK(hipEventSynchronize(p->table_event));
     p->table_pending = false;
-    std::vector<int> done;
-    for (int s = 0; s < p->opt.max_instances; ++s)
-        if (p->slots[s].used && p->table_host[s].phase == kSlotDone) done.push_back(s);
+    const std::vector<int> done = finished_slots(p, p->table_host);
     if (done.empty()) return;
-    const int free_phase = kSlotFree;
     for (int s : done) {
-        SlotHost& h = p->slots[s];
+        const SlotHost& h = p->slots[s];
         HIPCHECK(hipMemcpyAsync(h.x_out, p->v.x + h.row0, h.n * 8, hipMemcpyDeviceToHost, p->stream));
-        HIPCHECK(hipMemcpyAsync(&p->v.inst[s].phase, &free_phase, sizeof(int), hipMemcpyHostToDevice, p->stream));
     }
     HIPCHECK(hipStreamSynchronize(p->stream));
     for (int s : done) {
-        SlotHost& h = p->slots[s];
-        const PoolInst& t = p->table_host[s];
-        hgqp_pool_result r;
-        std::memset(&r, 0, sizeof r);
-        r.tag = h.tag;
-        r.slot = s;
-        r.status = t.status;
-        r.iterations = uint64_t(t.iter);
-        r.restarts = t.restarts;
-        r.checks = t.checks;
-        r.cycles = t.cycle + 1;
-        r.check.objective = t.objective;
-        r.check.primal_residual = t.primal_residual;
-        r.check.stationarity_residual = t.stationarity;
-        r.check.complementarity_residual = std::isfinite(t.objective) ? 0.0 : std::numeric_limits<double>::infinity();
-        r.check.dual_bound = t.dual_bound;
-        r.check.relative_gap = t.relative_gap;
-        r.check.optimality_verified = t.verified;
-        r.check.finite = t.finite;
-        p->ready.push_back(r);
-        p->rows.release(h.row0, h.n + 1);
-        p->ents.release(h.ent0, std::max<int64_t>(h.nnz, 1));
-        p->blks.release(h.blk0, h.nblk);
-        h = SlotHost();
-        p->free_slots.push_back(s);
-        --p->resident;
+        p->ready.push_back(result_of(p->slots[s], p->table_host[s], s));
+        release_slot(p, s);
     }
 }
 
 }  // namespace
 
 // ---- serving loop -------------------------------------------------------------------
+namespace {
+
+// Pinned host memory for staged transfers: one hipHostMalloc region handed out
+// first-fit in 256-byte units; thread-safe (submit and the loop share it).
+struct PinnedArena {
+    char* base = nullptr;
+    int64_t units = 0;
+    RangeAlloc ranges;
+    std::mutex mu;
+    static constexpr int64_t kUnit = 256;
+    void init(size_t bytes) {
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&base), bytes, hipHostMallocDefault));
+        units = int64_t(bytes / kUnit);
+        ranges.init(units);
+    }
+    // nullptr when the region has no room (the caller falls back to pageable memory)
+    char* take(size_t bytes) {
+        std::lock_guard<std::mutex> lk(mu);
+        const int64_t u = ranges.alloc(int64_t((bytes + kUnit - 1) / kUnit));
+        return u < 0 ? nullptr : base + u * kUnit;
+    }
+    void give(char* ptr, size_t bytes) {
+        if (!ptr) return;
+        std::lock_guard<std::mutex> lk(mu);
+        ranges.release((ptr - base) / kUnit, int64_t((bytes + kUnit - 1) / kUnit));
+    }
+    ~PinnedArena() {
+        if (base) (void)hipHostFree(base);
+    }
+};
+
+constexpr size_t kStagingBytes = size_t(1) << 30;  // pinned staging of queued and uploading requests
+
+size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
+
+}  // namespace
+
 struct hgqp_server {
     hgqp_pool* pool = nullptr;
+    hipStream_t copy_stream = nullptr;  // x read-back of finished slots, beside the next period
+    hipEvent_t period_done[2] = {nullptr, nullptr};
+    hipEvent_t copies_done = nullptr;
+    PoolInst* tables[2] = {nullptr, nullptr};  // pinned slot-table snapshots, one per period in flight
+    PinnedArena staging;
     std::thread thread;
     std::mutex mu;
     std::condition_variable cv;
     bool stopping = false;
     std::string error;
-    struct Request {
-        int64_t n, nnz;
-        std::vector<int32_t> start, index;
-        std::vector<double> value, cost, lower, upper;
-        double lipschitz, offset, tol;
-        int32_t maximize;
-        uint64_t limit;
-        int64_t tag;
-        double* x_out;
-        std::chrono::steady_clock::time_point submitted, admitted;
-    };
-    std::deque<std::unique_ptr<Request>> queue;              // submitted, not admitted
-    std::map<int64_t, std::unique_ptr<Request>> resident;     // by slot
-    std::deque<hgqp_served> done;
-    int64_t outstanding = 0;                                 // submitted, not polled
 
+    // A submitted instance: its arrays in one host block (pinned staging, or
+    // pageable when the staging region is full), laid out as start, index,
+    // block table, value, cost, lower, upper, slot entry.
+    struct Request {
+        int64_t n = 0, nnz = 0;
+        double lipschitz = 0, offset = 0, tol = 0;
+        int32_t maximize = 0;
+        bool diag = false;
+        uint64_t limit = 0;
+        int64_t tag = 0;
+        double* x_out = nullptr;
+        char* block = nullptr;  // pinned
+        size_t block_bytes = 0;
+        std::vector<char> heap;  // pageable fallback
+        int32_t *start = nullptr, *index = nullptr, *blk = nullptr;
+        double *value = nullptr, *cost = nullptr, *lower = nullptr, *upper = nullptr;
+        PoolInst* entry = nullptr;
+        double* x_stage = nullptr;  // pinned x read-back while resident (nullptr: straight to x_out)
+        int slot = -1;
+        std::chrono::steady_clock::time_point submitted, admitted;
+
+        static size_t bytes_for(int64_t n, int64_t nnz) {
+            return align16((n + 1) * 4) + align16(nnz * 4) + align16(3 * (n + 1) * 4) + align16(nnz * 8) +
+                   3 * align16(n * 8) + align16(sizeof(PoolInst));
+        }
+        void carve(char* p) {
+            start = reinterpret_cast<int32_t*>(p);
+            p += align16((n + 1) * 4);
+            index = reinterpret_cast<int32_t*>(p);
+            p += align16(nnz * 4);
+            blk = reinterpret_cast<int32_t*>(p);
+            p += align16(3 * (n + 1) * 4);
+            value = reinterpret_cast<double*>(p);
+            p += align16(nnz * 8);
+            cost = reinterpret_cast<double*>(p);
+            p += align16(n * 8);
+            lower = reinterpret_cast<double*>(p);
+            p += align16(n * 8);
+            upper = reinterpret_cast<double*>(p);
+            p += align16(n * 8);
+            entry = reinterpret_cast<PoolInst*>(p);
+        }
+    };
+    std::deque<std::unique_ptr<Request>> queue;           // submitted, not admitted
+    std::map<int, std::unique_ptr<Request>> resident;     // by slot (touched by the loop thread only)
+    std::deque<hgqp_served> done;
+    int64_t outstanding = 0;                              // submitted, not polled
+
+    void give_block(Request& r) {
+        staging.give(r.block, r.block_bytes);
+        r.block = nullptr;
+        r.heap.clear();
+        r.heap.shrink_to_fit();
+    }
+
+    // Admits queued requests in arrival order while slots and budget allow;
+    // returns the admitted ones (their uploads are enqueued, not finished).
+    std::vector<Request*> admit_queued() {
+        std::vector<std::unique_ptr<Request>> take;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            while (!queue.empty()) {
+                take.push_back(std::move(queue.front()));
+                queue.pop_front();
+            }
+        }
+        std::vector<Request*> admitted;
+        size_t k = 0;
+        for (; k < take.size(); ++k) {
+            Request& r = *take[k];
+            const int s = reserve(pool, r.n, r.nnz, r.start, r.blk);
+            if (s < 0) break;  // later requests wait behind this one (arrival order)
+            SlotHost& h = pool->slots[s];
+            h.tag = r.tag;
+            h.x_out = r.x_out;
+            *r.entry = slot_entry(h, r.lipschitz, r.tol, r.limit, r.maximize, r.offset, r.diag);
+            upload(pool, s, r.start, r.index, r.value, r.cost, r.lower, r.upper, r.entry);
+            if (!r.block) HIPCHECK(hipStreamSynchronize(pool->stream));  // pageable source: finish now
+            r.x_stage = reinterpret_cast<double*>(staging.take(size_t(r.n) * 8));
+            r.slot = s;
+            r.admitted = std::chrono::steady_clock::now();
+            admitted.push_back(take[k].get());
+            resident[s] = std::move(take[k]);
+        }
+        if (k < take.size()) {
+            std::lock_guard<std::mutex> lk(mu);
+            for (size_t i = take.size(); i-- > k;) queue.push_front(std::move(take[i]));
+        }
+        return admitted;
+    }
+
+    // Reads back x of the slots `table` reports finished (on the copy stream,
+    // beside the period now running), hands the results to poll and frees the
+    // slots in stream order.
+    void deliver(const PoolInst* table, hipEvent_t table_ready) {
+        const std::vector<int> fin = finished_slots(pool, table);
+        if (fin.empty()) return;
+        HIPCHECK(hipStreamWaitEvent(copy_stream, table_ready, 0));
+        for (int s : fin) {
+            const Request& r = *resident[s];
+            double* dst = r.x_stage ? r.x_stage : r.x_out;
+            HIPCHECK(hipMemcpyAsync(dst, pool->v.x + pool->slots[s].row0, r.n * 8, hipMemcpyDeviceToHost, copy_stream));
+        }
+        HIPCHECK(hipEventRecord(copies_done, copy_stream));
+        HIPCHECK(hipEventSynchronize(copies_done));
+        // later uploads into these ranges wait for the read-back on the device too
+        HIPCHECK(hipStreamWaitEvent(pool->stream, copies_done, 0));
+        const auto now = std::chrono::steady_clock::now();
+        std::vector<hgqp_served> out;
+        for (int s : fin) {
+            std::unique_ptr<Request> r = std::move(resident[s]);
+            resident.erase(s);
+            if (r->x_stage) std::memcpy(r->x_out, r->x_stage, size_t(r->n) * 8);
+            staging.give(reinterpret_cast<char*>(r->x_stage), size_t(r->n) * 8);
+            hgqp_served sv;
+            sv.result = result_of(pool->slots[s], table[s], s);
+            sv.queued_seconds = std::chrono::duration<double>(r->admitted - r->submitted).count();
+            sv.latency_seconds = std::chrono::duration<double>(now - r->submitted).count();
+            out.push_back(sv);
+            release_slot(pool, s);
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto& sv : out) done.push_back(sv);
+    }
+
+    // Each turn enqueues period i (after the admissions of this turn), then
+    // waits for period i - 1 and delivers what it finished while period i runs.
     void loop() {
         try {
+            int cur = 0;
+            bool prev = false;                    // a period is in flight from the last turn
+            std::vector<Request*> prev_uploads;   // admitted last turn: staging freed once that period ends
             for (;;) {
-                std::vector<std::unique_ptr<Request>> admit_now;
                 {
                     std::unique_lock<std::mutex> lk(mu);
-                    cv.wait(lk, [&] { return stopping || !queue.empty() || !resident.empty(); });
-                    if (stopping && queue.empty() && resident.empty()) return;
-                    while (!queue.empty()) {
-                        admit_now.push_back(std::move(queue.front()));
-                        queue.pop_front();
-                    }
+                    cv.wait(lk, [&] { return stopping || !queue.empty() || pool->resident > 0 || prev; });
+                    if (stopping && queue.empty() && pool->resident == 0 && !prev) return;
                 }
-                // admit in arrival order while slots and budget allow; the rest wait
-                std::vector<std::unique_ptr<Request>> back;
-                bool full = false;
-                for (auto& r : admit_now) {
-                    if (full) { back.push_back(std::move(r)); continue; }
-                    const int s = pool->free_slots.empty() ? -1 : pool->free_slots.back();
-                    int rc = admit(pool, r->n, r->nnz, r->start.data(), r->index.data(), r->value.data(),
-                                   r->cost.data(), r->lower.data(), r->upper.data(), r->lipschitz, r->maximize,
-                                   r->offset, r->tol, r->limit, r->tag, r->x_out);
-                    if (rc == HGQP_BUSY) { full = true; back.push_back(std::move(r)); continue; }
-                    r->admitted = std::chrono::steady_clock::now();
-                    r->start.clear(); r->start.shrink_to_fit();
-                    r->index.clear(); r->index.shrink_to_fit();
-                    r->value.clear(); r->value.shrink_to_fit();
-                    std::lock_guard<std::mutex> lk(mu);
-                    resident[s] = std::move(r);
+                std::vector<Request*> uploads = admit_queued();
+                const bool period = pool->resident > 0;
+                if (period) {
+                    if (!pool->tuned) autotune(pool);
+                    launch_periods(pool, 1);
+                    HIPCHECK(hipMemcpyAsync(tables[cur], pool->v.inst, sizeof(PoolInst) * pool->opt.max_instances,
+                                            hipMemcpyDeviceToHost, pool->stream));
+                    HIPCHECK(hipEventRecord(period_done[cur], pool->stream));
                 }
-                if (!back.empty()) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    for (auto it = back.rbegin(); it != back.rend(); ++it) queue.push_front(std::move(*it));
+                if (prev) {
+                    HIPCHECK(hipEventSynchronize(period_done[cur ^ 1]));
+                    for (Request* r : prev_uploads) give_block(*r);
+                    deliver(tables[cur ^ 1], period_done[cur ^ 1]);
                 }
-                if (pool->resident == 0) continue;
-                if (!pool->tuned) autotune(pool);
-                enqueue_periods(pool, 1);
-                harvest(pool);
-                if (!pool->ready.empty()) {
-                    const auto now = std::chrono::steady_clock::now();
-                    std::lock_guard<std::mutex> lk(mu);
-                    for (const auto& res : pool->ready) {
-                        auto it = resident.find(res.slot);
-                        hgqp_served sv;
-                        sv.result = res;
-                        sv.queued_seconds = std::chrono::duration<double>(it->second->admitted - it->second->submitted).count();
-                        sv.latency_seconds = std::chrono::duration<double>(now - it->second->submitted).count();
-                        done.push_back(sv);
-                        resident.erase(it);
-                    }
-                    pool->ready.clear();
-                }
+                prev_uploads = std::move(uploads);
+                prev = period;
+                cur ^= 1;
             }
         } catch (const std::exception& e) {
             std::lock_guard<std::mutex> lk(mu);
@@ -562,6 +777,16 @@ struct hgqp_server {
         }
         cv.notify_all();
         if (thread.joinable()) thread.join();
+        if (pool && pool->stream) (void)hipStreamSynchronize(pool->stream);
+        if (copy_stream) {
+            (void)hipStreamSynchronize(copy_stream);
+            (void)hipStreamDestroy(copy_stream);
+        }
+        for (auto& e : period_done)
+            if (e) (void)hipEventDestroy(e);
+        if (copies_done) (void)hipEventDestroy(copies_done);
+        for (auto& t : tables)
+            if (t) (void)hipHostFree(t);
         delete pool;
     }
 };
@@ -663,6 +888,15 @@ int hgqp_server_start(const hgqp_pool_options* options, hgqp_server** out) {
         else hgqp_pool_options_default(&o);
         auto s = std::make_unique<hgqp_server>();
         s->pool = create_pool(o);
+        HIPCHECK(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+        for (auto& e : s->period_done) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&s->copies_done, hipEventDisableTiming));
+        for (auto& t : s->tables) {
+            HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&t), sizeof(PoolInst) * o.max_instances,
+                                   hipHostMallocDefault));
+            std::memset(t, 0, sizeof(PoolInst) * o.max_instances);
+        }
+        s->staging.init(kStagingBytes);
         hgqp_server* raw = s.get();
         s->thread = std::thread([raw] { raw->loop(); });
         *out = s.release();
@@ -675,32 +909,45 @@ int hgqp_server_submit(hgqp_server* server, int64_t n, int64_t nnz, const int32_
                        uint64_t iteration_limit, int64_t tag, double* x_out) {
     return guarded([&] {
         if (!server) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_server handle");
-        if (n <= 0 || nnz < 0 || !start || (nnz && (!index || !value)) || !cost || !lower || !upper || !x_out ||
-            start[0] != 0 || start[n] != nnz)
-            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit: null array or malformed CSR");
+        // checked here, on the caller's thread: the loop never sees an invalid instance
+        const bool diag = validate_instance(n, nnz, start, index, value, cost, lower, upper, x_out, lipschitz,
+                                            tolerance, "hgqp_server_submit");
         if (n + 1 > server->pool->opt.max_rows || nnz > server->pool->opt.max_nnz)
             throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit: instance larger than the pool budget");
         auto r = std::make_unique<hgqp_server::Request>();
         r->n = n;
         r->nnz = nnz;
-        r->start.assign(start, start + n + 1);
-        r->index.assign(index, index + nnz);
-        r->value.assign(value, value + nnz);
-        r->cost.assign(cost, cost + n);
-        r->lower.assign(lower, lower + n);
-        r->upper.assign(upper, upper + n);
+        r->block_bytes = hgqp_server::Request::bytes_for(n, nnz);
+        r->block = server->staging.take(r->block_bytes);
+        if (!r->block) r->heap.resize(r->block_bytes);
+        r->carve(r->block ? r->block : r->heap.data());
+        std::memcpy(r->start, start, size_t(n + 1) * 4);
+        if (nnz) {
+            std::memcpy(r->index, index, size_t(nnz) * 4);
+            std::memcpy(r->value, value, size_t(nnz) * 8);
+        }
+        std::memcpy(r->cost, cost, size_t(n) * 8);
+        std::memcpy(r->lower, lower, size_t(n) * 8);
+        std::memcpy(r->upper, upper, size_t(n) * 8);
         r->lipschitz = lipschitz;
         r->maximize = maximize;
         r->offset = offset;
         r->tol = tolerance;
+        r->diag = diag;
         r->limit = iteration_limit;
         r->tag = tag;
         r->x_out = x_out;
         r->submitted = std::chrono::steady_clock::now();
         {
             std::lock_guard<std::mutex> lk(server->mu);
-            if (!server->error.empty()) throw Error(HGQP_DEVICE_ERROR, "hgqp_server: " + server->error);
-            if (server->stopping) throw Error(HGQP_BUSY, "hgqp_server: stopping");
+            if (!server->error.empty()) {
+                server->give_block(*r);
+                throw Error(HGQP_DEVICE_ERROR, "hgqp_server: " + server->error);
+            }
+            if (server->stopping) {
+                server->give_block(*r);
+                throw Error(HGQP_BUSY, "hgqp_server: stopping");
+            }
             server->queue.push_back(std::move(r));
             ++server->outstanding;
         }

@@ -14,6 +14,8 @@
 //   tol / 2); x = clip(x_a + d) and the FP64 check decides: verified -> done,
 //   else re-anchor at x.
 #include "pool_kernels.hpp"
+
+#include <algorithm>
 #include "sparse_pass.hpp"
 
 #include "hgqp.h"
@@ -267,6 +269,12 @@ __global__ __launch_bounds__(kGroup) void pool_anchor(PoolView v, int parity) {
     }
 }
 
+__global__ __launch_bounds__(kGroup) void pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, int r0) {
+    const int64_t stride = int64_t(gridDim.x) * kGroup;
+    for (int64_t i = int64_t(blockIdx.x) * kGroup + threadIdx.x; i <= n; i += stride) start[i] += e0;
+    for (int64_t k = int64_t(blockIdx.x) * kGroup + threadIdx.x; k < nnz; k += stride) col[k] += r0;
+}
+
 int grid_for(int max_blocks) {
     int g = (max_blocks + kWavesPerGroup - 1) / kWavesPerGroup;
     return g < kMaxGroups ? (g < 1 ? 1 : g) : kMaxGroups;
@@ -302,6 +310,12 @@ void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps,
 void launch_pool_lists(const PoolView& v, hipStream_t stream) {
     hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 0);
     hipLaunchKernelGGL(pool_list_expand, dim3(v.max_inst), dim3(kGroup), 0, stream, v, 0);
+}
+
+void launch_pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, int r0, hipStream_t stream) {
+    const int64_t work = std::max<int64_t>(n + 1, nnz);
+    const int groups = int(std::min<int64_t>((work + kGroup - 1) / kGroup, kMaxGroups));
+    hipLaunchKernelGGL(pool_offset, dim3(groups), dim3(kGroup), 0, stream, start, n, e0, col, nnz, r0);
 }
 
 void launch_pool_steps(const PoolView& v, int loads, int max_blocks, int reps, hipStream_t stream) {

@@ -39,6 +39,7 @@ struct PoolInst {
     int32_t b0, nblk;  // block range
     int32_t r0, n;     // row range
     int32_t frozen;    // parity of the correction a waiting slot stopped at
+    int32_t gen;       // admission number of the slot's instance (host bookkeeping)
 };
 
 struct PoolView {
@@ -84,6 +85,10 @@ void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps,
 
 // Builds the active list from the slot table (after host admission edits).
 void launch_pool_lists(const PoolView& v, hipStream_t stream);
+
+// Moves an uploaded instance from local to pool numbering: start[0..n] += e0
+// (n + 1 row starts) and col[0..nnz) += r0.
+void launch_pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, int r0, hipStream_t stream);
 
 // Times `reps` FP32 step launches over the current active list (autotuning).
 void launch_pool_steps(const PoolView& v, int loads, int max_blocks, int reps, hipStream_t stream);
