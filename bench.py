@@ -198,7 +198,7 @@ def serve_workload(args):
 
     def pool_server(arrivals, count, producers=1):
         srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
-                          check_every=args.check_every)
+                          check_every=args.check_every, use_graph=0 if args.no_graph else 1)
         done = {}
         t0 = time.perf_counter()
         try:
@@ -280,7 +280,8 @@ def serve_workload(args):
         "config": {"workload": f"serving: {args.requests} requests, Poisson lambda={args.rate}/s, n log-uniform in "
                                f"[1e3, 2e5], 8 draws/row, bounds [-1,1], KKT<=1e-6, continuous batching in one pool",
                    "slots": args.slots, "budget_rows": args.budget_rows, "budget_nnz": args.budget_nnz,
-                   "distinct_instances": args.distinct, "check_every": args.check_every},
+                   "distinct_instances": args.distinct, "check_every": args.check_every,
+                   "graph_replay": not args.no_graph},
         "p50_ms": round(served["p50_ms"], 3), "p99_ms": round(served["p99_ms"], 3),
         "served": served, "plain_fp64_one_at_a_time": plain, "saturated": saturated,
         "device": hgqp.device_name(),
@@ -362,6 +363,8 @@ def main():
     ap.add_argument("--budget-rows", type=int, default=8_000_000, help="serve: pool row budget")
     ap.add_argument("--budget-nnz", type=int, default=160_000_000, help="serve: pool entry budget")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-at-a-time FP64 loop")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="serve: launch each period's kernels directly instead of replaying a captured graph")
     ap.add_argument("--saturate", type=int, default=4000,
                     help="serve: also replay this many requests queued at once (capacity); 0 = skip")
     args = ap.parse_args()
