@@ -88,6 +88,35 @@ def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
     return dict(name=name, wall=wall, lat=lat, reps=reps, x=x)
 
 
+def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
+    """The same solve with host buffers on both sides: create (upload of the CSR
+    and vectors, FP32 copies), solve, read x back. Reported beside `value`,
+    which times the solve with inputs already in HBM."""
+    precision, momentum, reduce = MODES[name]
+    cadence = dict(check_every=25, polish_every=0) if name == "fp64_reference" else \
+        dict(check_every=check_every, polish_every=polish_every)
+    up, sv, rd = [], [], []
+    for _ in range(reps):
+        a = time.perf_counter()
+        box = hgqp.Box(inst["start"], inst["index"], inst["value"], inst["cost"], inst["lower"], inst["upper"], L,
+                       precision=precision, momentum=momentum, reduce=reduce, use_graph=0)
+        b = time.perf_counter()
+        try:
+            box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60, **cadence)
+            c = time.perf_counter()
+            box.primal()
+            d = time.perf_counter()
+        finally:
+            box.close()
+        up.append(b - a)
+        sv.append(c - b)
+        rd.append(d - c)
+    total = np.median(np.array(up) + np.array(sv) + np.array(rd))
+    return {"mode": name, "upload_ms": round(float(np.median(up)) * 1e3, 3),
+            "solve_ms": round(float(np.median(sv)) * 1e3, 3), "readback_ms": round(float(np.median(rd)) * 1e3, 3),
+            "solves_per_s": round(1.0 / total, 3), "reps": reps}
+
+
 def cpu_baseline(inst, iterations_needed, threads):
     """The reference's own CPU path (oracle/_ref; the oracle port if it is absent)
     on the same instance: one run that stops after setup and the iteration-0
@@ -450,6 +479,8 @@ def main():
         roofline["fp64_reference_step"] = {"algorithmic_bytes": b64, "avg_launch_us": round(u64, 2),
                                            "achieved": round(b64 / (u64 * 1e-6) / 1e9, 1)}
 
+    pcie = host_inclusive(head, inst, L, args.check_every, args.polish_every)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -467,7 +498,7 @@ def main():
                    "mode": head, "seed": args.seed, "check_every": args.check_every,
                    "polish_every": args.polish_every},
         "p50_ms": round(hm["p50_ms"], 3), "p99_ms": round(hm["p99_ms"], 3),
-        "roofline": roofline, "cpu_baseline": cpu, "modes": modes, "device": device,
+        "roofline": roofline, "cpu_baseline": cpu, "host_inclusive": pcie, "modes": modes, "device": device,
     }
     print(json.dumps(line))
 
