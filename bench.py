@@ -32,43 +32,48 @@ HBM_PEAK_GBS = 8000.0
 KKT_TOL = 1e-6
 METRIC = "box-QP solves/s + p50/p99 latency at KKT≤1e-6; SpMV HBM GB/s vs peak"
 
+STEP = {"scalar": hgqp.STEP_SCALAR, "jacobi": hgqp.STEP_JACOBI}
 MODES = {
-    # name: (precision, momentum, reduce)
-    "fp64_reference": (hgqp.PRECISION_FP64, hgqp.MOMENTUM_REFERENCE, hgqp.REDUCE_ORDERED),
-    "fp64_restart": (hgqp.PRECISION_FP64, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE),
-    "fp32_polish_restart": (hgqp.PRECISION_FP32_POLISH, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE),
-    "fp32_refine_restart": (hgqp.PRECISION_FP32_REFINE, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE),
+    # name: (precision, momentum, reduce, step metric)
+    "fp64_reference": (hgqp.PRECISION_FP64, hgqp.MOMENTUM_REFERENCE, hgqp.REDUCE_ORDERED, hgqp.STEP_SCALAR),
+    "fp64_restart_jacobi": (hgqp.PRECISION_FP64, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE, hgqp.STEP_JACOBI),
+    "fp32_polish_restart_jacobi": (hgqp.PRECISION_FP32_POLISH, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE,
+                                   hgqp.STEP_JACOBI),
+    "fp32_refine_restart": (hgqp.PRECISION_FP32_REFINE, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE, hgqp.STEP_SCALAR),
+    "fp32_refine_restart_jacobi": (hgqp.PRECISION_FP32_REFINE, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE,
+                                   hgqp.STEP_JACOBI),
 }
 
 
-def step_bytes(n, nnz, value_bytes, vector_bytes):
+def step_bytes(n, nnz, value_bytes, vector_bytes, jacobi=False):
     """Algorithmic HBM bytes of one fused step: values + int32 columns once, row
     starts once, the gathered vector once, x, c, lower, upper read, x and the
-    extrapolated point written (7 vectors)."""
-    return nnz * (value_bytes + 4) + 4 * (n + 1) + 7 * n * vector_bytes
+    extrapolated point written (7 vectors), plus the per-row step denominators
+    read by the Jacobi metric (an 8th vector)."""
+    return nnz * (value_bytes + 4) + 4 * (n + 1) + (8 if jacobi else 7) * n * vector_bytes
 
 
-def measured_traffic(use32, n, draws, nnz):
+def measured_traffic(use32, mode, n, draws, nnz):
     """HBM bytes per launch of the FP32 step from the committed rocprofv3 PMC summary
     (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on plain streaming kernels), when this
-    run's kernel and instance are the ones profiled; otherwise None."""
+    run's mode, kernel and instance are the ones profiled; otherwise None."""
     path = os.path.join(ROOT, "profiles", "traffic_planted_n1e6.json")
     if not use32 or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}":
+    if t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or t.get("mode") != mode:
         return None
     return t["traffic_bytes_per_launch"]
 
 
 def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
-    precision, momentum, reduce = MODES[name]
+    precision, momentum, reduce, metric = MODES[name]
     # the reference schedule keeps the reference's check cadence of 25 iterations
     cadence = dict(check_every=25, polish_every=0) if name == "fp64_reference" else \
         dict(check_every=check_every, polish_every=polish_every)
     box = hgqp.Box(inst["start"], inst["index"], inst["value"], inst["cost"], inst["lower"], inst["upper"], L,
-                   precision=precision, momentum=momentum, reduce=reduce, timing=1, use_graph=0)
+                   precision=precision, momentum=momentum, reduce=reduce, step_metric=metric, timing=1, use_graph=0)
     try:
         for _ in range(warmup):
             box.reset()
@@ -92,14 +97,14 @@ def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
     """The same solve with host buffers on both sides: create (upload of the CSR
     and vectors, FP32 copies), solve, read x back. Reported beside `value`,
     which times the solve with inputs already in HBM."""
-    precision, momentum, reduce = MODES[name]
+    precision, momentum, reduce, metric = MODES[name]
     cadence = dict(check_every=25, polish_every=0) if name == "fp64_reference" else \
         dict(check_every=check_every, polish_every=polish_every)
     up, sv, rd = [], [], []
     for _ in range(reps):
         a = time.perf_counter()
         box = hgqp.Box(inst["start"], inst["index"], inst["value"], inst["cost"], inst["lower"], inst["upper"], L,
-                       precision=precision, momentum=momentum, reduce=reduce, use_graph=0)
+                       precision=precision, momentum=momentum, reduce=reduce, step_metric=metric, use_graph=0)
         b = time.perf_counter()
         try:
             box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60, **cadence)
@@ -155,7 +160,7 @@ def batch_workload(args):
     Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in batch]
     print(f"# batch: {len(batch)} instances, rows {rows}, nnz {nnz}", file=sys.stderr, flush=True)
     pool = hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
-                     check_every=args.check_every)
+                     check_every=args.check_every, step_metric=STEP[args.pool_step])
     times, results = [], None
     tuned = None
     try:
@@ -202,7 +207,7 @@ def batch_workload(args):
                                f"draws/row uniform in 4..32, bounds l~U(-2,-0.5), u~U(0.5,2), KKT<=1e-6, "
                                f"one pool, graph-replayed check periods; one step = the whole batch",
                    "rows": rows, "nnz": nnz, "autotuned_block_loads": tuned[0], "autotuned_block_rows": tuned[1],
-                   "check_every": args.check_every},
+                   "check_every": args.check_every, "step_metric": args.pool_step},
         "all_verified": ok, "worst_max_abs_err_vs_planted_x": worst_x, "worst_objective_rel_err_vs_planted": worst_obj,
         "iterations": {"min": int(iters.min()), "median": float(np.median(iters)), "max": int(iters.max())},
         "plain_fp64_single_instance": {"solves_per_s": round(plain_rate, 2), "instances_timed": plain_n,
@@ -227,7 +232,8 @@ def serve_workload(args):
 
     def pool_server(arrivals, count, producers=1):
         srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
-                          check_every=args.check_every, use_graph=0 if args.no_graph else 1)
+                          check_every=args.check_every, use_graph=0 if args.no_graph else 1,
+                          step_metric=STEP[args.pool_step])
         done = {}
         t0 = time.perf_counter()
         try:
@@ -310,6 +316,7 @@ def serve_workload(args):
                                f"[1e3, 2e5], 8 draws/row, bounds [-1,1], KKT<=1e-6, continuous batching in one pool",
                    "slots": args.slots, "budget_rows": args.budget_rows, "budget_nnz": args.budget_nnz,
                    "distinct_instances": args.distinct, "check_every": args.check_every,
+                   "step_metric": args.pool_step,
                    "graph_replay": not args.no_graph},
         "p50_ms": round(served["p50_ms"], 3), "p99_ms": round(served["p99_ms"], 3),
         "served": served, "plain_fp64_one_at_a_time": plain, "saturated": saturated,
@@ -392,6 +399,8 @@ def main():
     ap.add_argument("--budget-rows", type=int, default=8_000_000, help="serve: pool row budget")
     ap.add_argument("--budget-nnz", type=int, default=160_000_000, help="serve: pool entry budget")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-at-a-time FP64 loop")
+    ap.add_argument("--pool-step", choices=sorted(STEP), default="jacobi",
+                    help="batch/serve: step metric of the pool (per-row Jacobi or the scalar 1/L)")
     ap.add_argument("--no-graph", action="store_true",
                     help="serve: launch each period's kernels directly instead of replaying a captured graph")
     ap.add_argument("--saturate", type=int, default=4000,
@@ -466,10 +475,10 @@ def main():
     # roofline of the head mode's dominant kernel: its step kernel in the precision that ran most
     use32 = hm["fp32_step_us"] is not None and hm["fp32_iterations"] >= hm["iterations"] - hm["fp32_iterations"]
     us = hm["fp32_step_us"] if use32 else hm["fp64_step_us"]
-    nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8)
+    nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8, jacobi=MODES[head][3] == hgqp.STEP_JACOBI)
     achieved = nbytes / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(use32, n, args.draws, nnz),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(use32, head, n, args.draws, nnz),
                 "kernel": f"step_kernel<{'float' if use32 else 'double'}>", "algorithmic_bytes": nbytes,
                 "avg_launch_us": round(us, 2)}
     # the FP64 reference-schedule step beside it

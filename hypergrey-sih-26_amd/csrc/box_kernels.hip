@@ -62,6 +62,7 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     epi.xout = bufs.x[parity ^ 1];
     epi.wout = bufs.w[parity ^ 1];
     epi.lipschitz = bufs.lipschitz;
+    epi.dstep = bufs.dstep;
     epi.beta = T(beta);
     epi.dot = 0;
     epi.resid = 0;
@@ -402,6 +403,17 @@ __global__ void convert_kernel(int64_t n, const From* src, To* dst) {
         dst[j] = To(src[j]);
 }
 
+__global__ __launch_bounds__(256) void row_abs_kernel(int64_t n, const int* start, const double* val, double fallback,
+                                                      double* d64, float* d32) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        double s = 0;
+        for (int k = start[i]; k < start[i + 1]; ++k) s += fabs(val[k]);
+        const double d = s > 0 ? s * (1.0 + 0x1p-20) : fallback;
+        if (d64) d64[i] = d;
+        if (d32) d32[i] = float(d);
+    }
+}
+
 int elementwise_grid(int64_t n) {
     int64_t g = (n + 255) / 256;
     return int(g < 8192 ? (g < 1 ? 1 : g) : 8192);
@@ -486,6 +498,12 @@ template <class To, class From>
 void launch_convert(int64_t n, const From* src, To* dst, hipStream_t stream) {
     hipLaunchKernelGGL((convert_kernel<To, From>), dim3(elementwise_grid(n)), dim3(256), 0, stream, n, src, dst);
 }
+void launch_row_abs(int64_t n, const int* start, const double* val, double fallback, double* d64, float* d32,
+                    hipStream_t stream) {
+    hipLaunchKernelGGL(row_abs_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, start, val, fallback, d64,
+                       d32);
+}
+
 template void launch_convert<float, double>(int64_t, const double*, float*, hipStream_t);
 template void launch_convert<double, float>(int64_t, const float*, double*, hipStream_t);
 template void launch_convert<double, double>(int64_t, const double*, double*, hipStream_t);

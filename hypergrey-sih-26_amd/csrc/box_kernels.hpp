@@ -67,6 +67,7 @@ struct StepBuffers {
     T* x[2];
     T* w[2];
     T lipschitz;
+    const T* dstep;  // Jacobi step denominators per row, or null (scalar 1/L)
 };
 
 struct CheckParams {
@@ -84,6 +85,12 @@ struct CheckParams {
 // With the restart schedule and switch_stat > 0, the step's own residual
 // max L|w - x+| / (1 + |x+|) ends an FP32 phase (done = kPhaseEnd) once it is at
 // or below switch_stat, on stagnation, or when it is not finite.
+// Jacobi step denominators: d[i] = (sum_k |val_k| over row i) (1 + 2^-20), or
+// `fallback` for an empty row; written in FP64 (d64) and/or FP32 (d32), either
+// may be null. `start` holds the n + 1 row starts of the rows to process.
+void launch_row_abs(int64_t n, const int* start, const double* val, double fallback, double* d64, float* d32,
+                    hipStream_t stream);
+
 template <class T>
 void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
                  int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat = 0.0);

@@ -89,6 +89,8 @@ struct hgqp_box {
     double* w[2] = {nullptr, nullptr};
     float* x32[2] = {nullptr, nullptr};
     float* w32[2] = {nullptr, nullptr};
+    double* dstep = nullptr;     // Jacobi step denominators (HGQP_STEP_JACOBI)
+    float* dstep32 = nullptr;
     Control* ctrl = nullptr;
     double* partials = nullptr;
     Control* host_slots = nullptr;  // pinned read-back ring
@@ -107,10 +109,10 @@ struct hgqp_box {
     Matrix<double> m64() const { return Matrix<double>{nblocks, blk, start, col, val}; }
     Matrix<float> m32() const { return Matrix<float>{nblocks, blk, start, col, val32}; }
     StepBuffers<double> b64() const {
-        return StepBuffers<double>{c, lo, up, {x[0], x[1]}, {w[0], w[1]}, lipschitz};
+        return StepBuffers<double>{c, lo, up, {x[0], x[1]}, {w[0], w[1]}, lipschitz, dstep};
     }
     StepBuffers<float> b32() const {
-        return StepBuffers<float>{c32, lo32, up32, {x32[0], x32[1]}, {w32[0], w32[1]}, float(lipschitz)};
+        return StepBuffers<float>{c32, lo32, up32, {x32[0], x32[1]}, {w32[0], w32[1]}, float(lipschitz), dstep32};
     }
     bool ordered() const { return opt.reduce == HGQP_REDUCE_ORDERED; }
 
@@ -335,6 +337,7 @@ void hgqp_box_options_default(hgqp_box_options* o) {
     o->maximize = 0;
     o->offset = 0;
     o->diagonal_hessian = -1;
+    o->step_metric = HGQP_STEP_SCALAR;
 }
 
 void hgqp_solve_options_default(hgqp_solve_options* o) {
@@ -406,6 +409,8 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
             throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown momentum schedule");
         if (opt.reduce != HGQP_REDUCE_ORDERED && opt.reduce != HGQP_REDUCE_TREE)
             throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown reduction order");
+        if (opt.step_metric != HGQP_STEP_SCALAR && opt.step_metric != HGQP_STEP_JACOBI)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
 
         auto b = std::make_unique<hgqp_box>();
         b->n = n;
@@ -424,6 +429,8 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
                        Arena::span(1, sizeof(Control)) + Arena::span(size_t(b->groups) * 10 + 16, 8);
         if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
         if (refine) bytes += Arena::span(un, 8);
+        const bool jacobi = opt.step_metric == HGQP_STEP_JACOBI;
+        if (jacobi) bytes += Arena::span(un, 8) + (f32 ? Arena::span(un, 4) : 0);
         HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
         b->arena.reserve(bytes);
         b->blk = b->arena.take<int>(blk.size());
@@ -450,6 +457,10 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
             }
         }
         if (refine) b->g = b->arena.take<double>(un);
+        if (jacobi) {
+            b->dstep = b->arena.take<double>(un);
+            if (f32) b->dstep32 = b->arena.take<float>(un);
+        }
         HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(Control) * hgqp_box::kSlots,
                                hipHostMallocDefault));
         for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -462,6 +473,7 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         HIPCHECK(hipMemcpyAsync(b->c, cost, un * 8, hipMemcpyHostToDevice, b->stream));
         HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
         HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
+        if (jacobi) launch_row_abs(n, b->start, b->val, lipschitz, b->dstep, b->dstep32, b->stream);
         if (f32) {
             if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
             launch_convert<float, double>(n, b->c, b->c32, b->stream);

@@ -123,7 +123,7 @@ size_t pool_bytes(const hgqp_pool_options& o) {
     return Arena::span(R + 1, 4) + Arena::span(E, 4) + Arena::span(E, 8) + Arena::span(E, 4) +  // start col val val32
            3 * Arena::span(B, 4) +                                                            // blocks
            5 * Arena::span(R, 8) +                                                            // c lo up x g
-           7 * Arena::span(R, 4) +                                                            // c32 lo32 up32 d[2] w[2]
+           8 * Arena::span(R, 4) +                                                            // c32 lo32 up32 d[2] w[2] dstep
            Arena::span(S, sizeof(PoolInst)) + 2 * Arena::span(B, 4) + Arena::span(4, 4) + Arena::span(S, 4) +
            Arena::span(B * kPartial, 8) + Arena::span(B, 8);
 }
@@ -143,6 +143,8 @@ void validate_options(const hgqp_pool_options& o) {
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: unknown momentum schedule");
     if (!(o.fp32_switch > 0) || !std::isfinite(o.fp32_switch))
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: fp32_switch must be positive");
+    if (o.step_metric != HGQP_STEP_SCALAR && o.step_metric != HGQP_STEP_JACOBI)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: unknown step metric");
 }
 
 // Computes one slot's row blocks for the pool's block shape into `stage`
@@ -327,6 +329,8 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
         v.d[k] = p->arena.take<float>(R);
         v.w[k] = p->arena.take<float>(R);
     }
+    float* dstep = p->arena.take<float>(R);
+    v.dstep = o.step_metric == HGQP_STEP_JACOBI ? dstep : nullptr;
     v.inst = p->arena.take<PoolInst>(S);
     v.max_inst = int(S);
     v.list_active = p->arena.take<int>(B);
@@ -460,6 +464,7 @@ void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, con
         launch_convert<float, double>(h.nnz, p->v.val + h.ent0, const_cast<float*>(p->v.val32) + h.ent0, st);
     }
     launch_pool_offset(dstart, h.n, int(h.ent0), dcol, h.nnz, int(h.row0), st);
+    if (p->v.dstep) launch_row_abs(h.n, dstart, p->v.val, entry->lipschitz, nullptr, p->v.dstep + h.row0, st);
     HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.c) + h.row0, cost, h.n * 8, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.lo) + h.row0, lower, h.n * 8, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.up) + h.row0, upper, h.n * 8, hipMemcpyHostToDevice, st));
@@ -805,6 +810,7 @@ void hgqp_pool_options_default(hgqp_pool_options* o) {
     o->use_graph = 1;
     o->momentum = HGQP_MOMENTUM_RESTART;
     o->fp32_switch = 10.0;
+    o->step_metric = HGQP_STEP_JACOBI;
 }
 
 int hgqp_pool_bytes(const hgqp_pool_options* options, int64_t* bytes) {

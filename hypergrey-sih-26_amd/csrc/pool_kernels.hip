@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
         epi.xout = v.d[parity ^ 1];
         epi.wout = v.w[parity ^ 1];
         epi.lipschitz = float(s->lipschitz);
+        epi.dstep = v.dstep;
         epi.beta = float(beta);
         epi.dot = 0;
         epi.resid = 0;

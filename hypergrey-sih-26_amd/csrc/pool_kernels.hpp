@@ -63,6 +63,7 @@ struct PoolView {
     float* up32;   // upper - anchor
     float* d[2];   // FP32 correction, ping-pong
     float* w[2];   // FP32 extrapolated correction, ping-pong
+    float* dstep;  // Jacobi step denominators per row (HGQP_STEP_JACOBI), or null: each slot's L
     // slots and lists
     PoolInst* inst;
     int max_inst;

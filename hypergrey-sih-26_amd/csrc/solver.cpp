@@ -154,6 +154,7 @@ bool prepare_box(const QuadraticProblem& q, const QpOptions& opt, QpResult& r, P
                   : opt.precision == HGQP_PRECISION_FP32_REFINE ? "hip_fista_box_fp32_refine"
                                                                 : "hip_fista_box";
     if (opt.momentum == HGQP_MOMENTUM_RESTART) r.algorithm += "_restart";
+    if (opt.step_metric == HGQP_STEP_JACOBI) r.algorithm += "_jacobi";
     if (!check_convex_hessian(q)) {
         r.status = "unsupported_curvature";
         return false;
@@ -216,6 +217,7 @@ QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, 
     bo.precision = opt.precision;
     bo.momentum = opt.momentum;
     bo.reduce = opt.reduce;
+    bo.step_metric = opt.step_metric;
     bo.maximize = p.maximize ? 1 : 0;
     bo.offset = p.offset;
     bo.diagonal_hessian = positive_diagonal ? 1 : 0;
@@ -261,6 +263,7 @@ extern "C" void hgqp_qp_options_default(hgqp_qp_options* o) {
     o->precision = HGQP_PRECISION_FP64;
     o->momentum = HGQP_MOMENTUM_REFERENCE;
     o->reduce = HGQP_REDUCE_ORDERED;
+    o->step_metric = HGQP_STEP_SCALAR;
 }
 
 namespace {
@@ -346,6 +349,7 @@ extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options*
         opt.precision = o.precision;
         opt.momentum = o.momentum;
         opt.reduce = o.reduce;
+        opt.step_metric = o.step_metric;
         const auto t0 = std::chrono::steady_clock::now();
         QpResult r;
         try {

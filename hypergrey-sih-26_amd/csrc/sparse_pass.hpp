@@ -206,18 +206,20 @@ struct StepEpilogue {
     const T* __restrict__ up;
     T* __restrict__ xout;
     T* __restrict__ wout;
+    const T* __restrict__ dstep;  // per-row step denominators (Jacobi metric), or null: L for every row
     T lipschitz;
     T beta;
     double dot;    // restart test: sum (w - next)(next - x)
-    double resid;  // max L |w - next| / (1 + |next|): the step's own gradient-mapping residual
+    double resid;  // max D |w - next| / (1 + |next|): the step's own gradient-mapping residual
     __device__ inline void operator()(int r, T g) {
 #pragma clang fp contract(off)
         const T wv = gat[r], xv = xin[r];
-        const T next = clip(wv - (g + c[r]) / lipschitz, lo[r], up[r]);
+        const T den = dstep ? dstep[r] : lipschitz;
+        const T next = clip(wv - (g + c[r]) / den, lo[r], up[r]);
         wout[r] = next + beta * (next - xv);
         xout[r] = next;
         dot += double(wv - next) * double(next - xv);
-        const double rr = double(lipschitz) * fabs(double(wv) - double(next)) / (1 + fabs(double(next)));
+        const double rr = double(den) * fabs(double(wv) - double(next)) / (1 + fabs(double(next)));
         resid = isfinite(rr) ? fmax(resid, rr) : INFINITY;
     }
 };

@@ -18,6 +18,7 @@ STATUS = ["optimal", "iteration_limit", "time_limit", "numerical_error", "infeas
 PRECISION_FP64, PRECISION_FP32_POLISH, PRECISION_FP32_REFINE = 0, 1, 2
 MOMENTUM_REFERENCE, MOMENTUM_RESTART = 0, 1
 REDUCE_ORDERED, REDUCE_TREE = 0, 1
+STEP_SCALAR, STEP_JACOBI = 0, 1
 
 
 class HgqpError(RuntimeError):
@@ -37,7 +38,7 @@ class Unsupported(HgqpError, ValueError):
 class BoxOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
                 ("use_graph", C.c_int32), ("timing", C.c_int32), ("maximize", C.c_int32),
-                ("offset", C.c_double), ("diagonal_hessian", C.c_int32), ("reserved", C.c_int32)]
+                ("offset", C.c_double), ("diagonal_hessian", C.c_int32), ("step_metric", C.c_int32)]
 
 
 class CheckReport(C.Structure):
@@ -85,7 +86,8 @@ class Problem(C.Structure):
 
 class QpOptions(C.Structure):
     _fields_ = [("time_limit", C.c_double), ("tolerance", C.c_double), ("iteration_limit", C.c_uint64),
-                ("threads", C.c_int32), ("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32)]
+                ("threads", C.c_int32), ("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
+                ("step_metric", C.c_int32), ("reserved", C.c_int32)]
 
 
 class QpResult(C.Structure):
@@ -111,7 +113,8 @@ class Planted(C.Structure):
 class PoolOptions(C.Structure):
     _fields_ = [("max_rows", C.c_int64), ("max_nnz", C.c_int64), ("max_instances", C.c_int32),
                 ("check_every", C.c_int32), ("block_loads", C.c_int32), ("block_rows", C.c_int32),
-                ("use_graph", C.c_int32), ("momentum", C.c_int32), ("fp32_switch", C.c_double)]
+                ("use_graph", C.c_int32), ("momentum", C.c_int32), ("fp32_switch", C.c_double),
+                ("step_metric", C.c_int32), ("reserved", C.c_int32)]
 
 
 class PoolResult(C.Structure):

@@ -71,6 +71,7 @@ struct QpOptions {
     int precision = HGQP_PRECISION_FP64;
     int momentum = HGQP_MOMENTUM_REFERENCE;
     int reduce = HGQP_REDUCE_ORDERED;
+    int step_metric = HGQP_STEP_SCALAR;
 };
 
 // Mirrored, merged CSR (include/sparse_matrix.hpp:5-17): per row, duplicate

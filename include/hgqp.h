@@ -62,6 +62,17 @@ extern "C" {
 #define HGQP_REDUCE_ORDERED 0
 #define HGQP_REDUCE_TREE 1
 
+/* Step metric of the projected gradient step. SCALAR is the reference's
+ * x+ = clip(w - (Hw + c)/L) with L the largest absolute row sum
+ * (accelerated_qp.cpp:25-26, 54-57). JACOBI divides row i by its own absolute
+ * row sum D_ii = sum_k |H_ik| (times 1 + 2^-20; L for an empty row). By
+ * Gershgorin D - H is positive semidefinite, so each step is still a
+ * majorize-minimize step and the box projection stays coordinate-wise; the
+ * step is larger on every row whose sum is below the maximum. No reference
+ * counterpart: results are held to the same verdict. */
+#define HGQP_STEP_SCALAR 0
+#define HGQP_STEP_JACOBI 1
+
 typedef struct hgqp_box hgqp_box;
 
 typedef struct hgqp_box_options {
@@ -75,7 +86,7 @@ typedef struct hgqp_box_options {
     int32_t diagonal_hessian;/* 1 if every off-diagonal triplet is zero (enables the
                                 separable dual bound of qp_admm.cpp:106-112), 0 if
                                 not, -1 derive from the stored matrix              */
-    int32_t reserved;
+    int32_t step_metric;     /* HGQP_STEP_* (default SCALAR)                      */
 } hgqp_box_options;
 
 void hgqp_box_options_default(hgqp_box_options* options);
@@ -198,6 +209,8 @@ typedef struct hgqp_qp_options {
     int32_t precision;        /* HGQP_PRECISION_*                                 */
     int32_t momentum;         /* HGQP_MOMENTUM_*                                  */
     int32_t reduce;           /* HGQP_REDUCE_*                                    */
+    int32_t step_metric;      /* HGQP_STEP_* (default SCALAR)                     */
+    int32_t reserved;
 } hgqp_qp_options;
 
 void hgqp_qp_options_default(hgqp_qp_options* options);
@@ -302,6 +315,8 @@ typedef struct hgqp_pool_options {
     int32_t momentum;        /* HGQP_MOMENTUM_RESTART (default) or _REFERENCE    */
     double fp32_switch;      /* first FP32 cycle ends at this multiple of the
                                 tolerance (default 10)                           */
+    int32_t step_metric;     /* HGQP_STEP_JACOBI (default) or _SCALAR            */
+    int32_t reserved;
 } hgqp_pool_options;
 
 void hgqp_pool_options_default(hgqp_pool_options* options);

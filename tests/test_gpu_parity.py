@@ -106,14 +106,17 @@ def test_tridiagonal_golden(hg):
     assert abs(np.abs(r["primal"] - d["optimum"]).max() - 5.334931689526812e-06) <= 1e-12
 
 
-@pytest.mark.parametrize("momentum,precision", [(1, 0), (1, 1), (0, 1), (1, 2), (0, 2)])
-def test_fast_modes_meet_contract(hg, oracle, momentum, precision):
-    """Restart, FP32 with FP64 polish, FP32 correction cycles anchored at FP64:
-    KKT <= 1e-6 by the oracle's own check, objective within 1e-8 of the FP64 oracle."""
+@pytest.mark.parametrize("momentum,precision,metric", [(1, 0, 0), (1, 1, 0), (0, 1, 0), (1, 2, 0), (0, 2, 0),
+                                                        (1, 0, 1), (1, 1, 1), (1, 2, 1), (0, 2, 1)])
+def test_fast_modes_meet_contract(hg, oracle, momentum, precision, metric):
+    """Restart, FP32 with FP64 polish, FP32 correction cycles anchored at FP64,
+    each with the scalar 1/L step or the per-row Jacobi step: KKT <= 1e-6 by the
+    oracle's own check, objective within 1e-8 of the FP64 oracle, x* recovered."""
     q, d = planted_qp(hg, 10000, 8, 3)
     s, i, v, c, lo, up, L = engine_inputs(oracle, q)
     ref = oracle.solve_box(q, tolerance=1e-9, iteration_limit=20000, time_limit=120)
-    with hg.Box(s, i, v, c, lo, up, L, momentum=momentum, precision=precision, reduce=hg.REDUCE_TREE) as box:
+    with hg.Box(s, i, v, c, lo, up, L, momentum=momentum, precision=precision, reduce=hg.REDUCE_TREE,
+                step_metric=metric) as box:
         rep = box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=120)
         x = box.primal()
     print(rep, flush=True)
@@ -122,3 +125,27 @@ def test_fast_modes_meet_contract(hg, oracle, momentum, precision):
     assert chk["optimality_verified"] and chk["stationarity_residual"] <= KKT_TOL
     assert abs(chk["objective"] - ref["objective"]) <= OBJ_REL * (1 + abs(ref["objective"]))
     assert np.abs(x - d["optimum"]).max() < 1e-4
+
+
+def test_jacobi_step_is_the_row_scaled_projection(hg, oracle):
+    """Jacobi steps with momentum 0 from x0 = clip(0): x+_i = clip(x_i - (Hx + c)_i / D_i)
+    with D_i = (sum_k |H_ik|)(1 + 2^-20), against numpy on the engine's own CSR
+    (the second step starts from the device's first iterate, so Hx is nonzero)."""
+    q, d = planted_qp(hg, 3000, 8, 11)
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    with hg.Box(s, i, v, c, lo, up, L, step_metric=hg.STEP_JACOBI) as box:
+        box.step(0.0)
+        x1 = box.primal()
+        box.step(0.0)
+        x2 = box.primal()
+    n = len(c)
+    rows = np.repeat(np.arange(n), np.diff(s))
+    D = np.bincount(rows, weights=np.abs(v), minlength=n) * (1.0 + 2.0 ** -20)
+
+    def step(x):
+        return np.clip(x - (np.bincount(rows, weights=v * x[i], minlength=n) + c) / D, lo, up)
+
+    x0 = np.clip(np.zeros(n), lo, up)
+    for got, expect in ((x1, step(x0)), (x2, step(x1))):
+        assert np.abs(got - expect).max() <= 1e-13 * (1 + np.abs(expect).max())
+    assert np.abs(x2 - x1).max() > 1e-3  # the gradient moved the iterate
