@@ -15,7 +15,8 @@ constexpr int kGroup = kWave * kWavesPerGroup;
 constexpr int kLoads = 8;
 constexpr int kCap = kWave * kLoads;
 constexpr int kMaxRows = 256;
-constexpr int kMaxGroups = 2048;
+constexpr int kMaxGroups = 2048;        // pool launches (grid fixed when the period graph is captured)
+constexpr int kMaxBoxGroups = 1 << 20;  // single-instance launches
 constexpr int kPartial = 10;  // doubles per partial of an FP64 check
 
 // Momentum source of a step launch.
