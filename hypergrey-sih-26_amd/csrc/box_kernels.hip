@@ -179,8 +179,13 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
     epi.viol = 0;
     epi.value = epi.tangent = epi.separable = DD{0, 0};
     epi.bound_inf = epi.bad = 0;
-    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
-        row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+    if (prm.zero_x) {  // grid-uniform: H 0 = 0 exactly, every row's epilogue with hx = 0
+        for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+            for (int r = m.blk[b] + lane; r < m.blk[b + 1]; r += kWave) epi(r, 0.0);
+    } else {
+        for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+            row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+    }
     // workgroup partial
     {
         double stat = wave_max(epi.stat), viol = wave_max(epi.viol);

@@ -79,6 +79,7 @@ struct CheckParams {
     int set_done;       // 1: a verified / non-finite check stops the loop
     double switch_stat; // FP32 phase: stationarity that ends the phase
     double* gout;       // FP64 check: optional per-row gradient output
+    int zero_x = 0;     // 1: the checked iterate is exactly 0, so Hx = 0 and the matrix is not read
 };
 
 // Launchers. parity selects x[parity], w[parity] as inputs; step launches

@@ -89,6 +89,8 @@ struct hgqp_box {
     double* w[2] = {nullptr, nullptr};
     float* x32[2] = {nullptr, nullptr};
     float* w32[2] = {nullptr, nullptr};
+    bool zero_start = false;     // 0 lies in every box, so x0 = clip(0) = 0
+    bool at_zero = false;        // the current iterate is x0 = 0 (fresh or reset handle)
     double* dstep = nullptr;     // Jacobi step denominators (HGQP_STEP_JACOBI)
     float* dstep32 = nullptr;
     Control* ctrl = nullptr;
@@ -276,7 +278,11 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
                       const CheckParams& cp, std::chrono::steady_clock::time_point t0, double time_limit,
                       std::vector<TimedPeriod>* timed) {
     LoopOutcome out;
-    const int lag = 2;
+    // Periods enqueued ahead of the control read-back. Two hide the host round
+    // trip behind short periods; one suffices once a period's steps take far
+    // longer than the round trip (large instances), and then at most one period
+    // of launches is wasted after the deciding check.
+    const int lag = b->nnz >= (int64_t(1) << 22) ? 1 : 2;
     std::vector<int> pending;  // slot indices of enqueued read-backs, oldest first
     int next_slot = 1;         // slot 0 is kept for synchronous transfers
     bool stop = false;
@@ -418,6 +424,9 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         b->opt = opt;
         b->lipschitz = lipschitz;
         b->diagonal = opt.diagonal_hessian < 0 ? (diag ? 1 : 0) : (opt.diagonal_hessian ? 1 : 0);
+        b->zero_start = true;
+        for (int64_t j = 0; j < n && b->zero_start; ++j) b->zero_start = lower[j] <= 0 && 0 <= upper[j];
+        b->at_zero = b->zero_start;
         std::vector<int> blk = row_blocks(n, start, kCap, kMaxRows);
         b->nblocks = static_cast<int>(blk.size()) - 1;
         b->groups = groups_for(b->nblocks);
@@ -494,6 +503,7 @@ int hgqp_box_step(hgqp_box* b, double momentum) {
                             b->stream);
         HIPCHECK(hipGetLastError());
         b->parity ^= 1;
+        b->at_zero = false;
     });
 }
 
@@ -525,6 +535,7 @@ int hgqp_box_reset(hgqp_box* b) {
         launch_init<double>(b->n, b->lo, b->up, b->x[0], b->w[0], b->stream);
         HIPCHECK(hipGetLastError());
         put_control(b, fresh_control());
+        b->at_zero = b->zero_start;
     });
 }
 
@@ -555,9 +566,14 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         put_control(b, fresh_control());
         uint64_t it = 0;
         bool time_up = false;
-        // check at iteration 0 (accelerated_qp.cpp:40)
-        launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), cp64, b->stream);
-        HIPCHECK(hipGetLastError());
+        // check at iteration 0 (accelerated_qp.cpp:40); at x0 = 0 the gradient is c, no matrix pass
+        {
+            CheckParams first = cp64;
+            first.zero_x = b->at_zero ? 1 : 0;
+            launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), first, b->stream);
+            HIPCHECK(hipGetLastError());
+            b->at_zero = false;
+        }
         Control c = get_control(b);
         // Resets the momentum state and the done flag at iteration `at` for a new phase.
         auto new_phase = [&](uint64_t at) {
