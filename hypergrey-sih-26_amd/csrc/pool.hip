@@ -364,7 +364,7 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
 // qp_admm.cpp:115-116).
 bool validate_instance(int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
                        const double* cost, const double* lower, const double* upper, const double* x_out,
-                       double lipschitz, double tol, const char* who) {
+                       double lipschitz, double tol, const char* who, bool* zero_start) {
     const std::string w(who);
     if (n <= 0 || nnz < 0 || !start || (nnz && (!index || !value)) || !cost || !lower || !upper || !x_out)
         throw Error(HGQP_INVALID_ARGUMENT, w + ": null array or bad size");
@@ -375,8 +375,10 @@ bool validate_instance(int64_t n, int64_t nnz, const int32_t* start, const int32
     if (!(tol > 0) || !std::isfinite(tol)) throw Error(HGQP_INVALID_ARGUMENT, w + ": bad tolerance");
     if (start[0] != 0 || start[n] != nnz) throw Error(HGQP_INVALID_ARGUMENT, w + ": malformed CSR start");
     bool diag = true;
+    *zero_start = true;
     for (int64_t i = 0; i < n; ++i) {
         if (start[i + 1] < start[i]) throw Error(HGQP_INVALID_ARGUMENT, w + ": CSR start not monotone");
+        *zero_start = *zero_start && lower[i] <= 0 && 0 <= upper[i];
         bool pos = false;
         for (int32_t k = start[i]; k < start[i + 1]; ++k) {
             if (index[k] < 0 || index[k] >= n) throw Error(HGQP_INVALID_ARGUMENT, w + ": column out of range");
@@ -427,7 +429,7 @@ int reserve(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, int32_t*
 // The device slot entry of a new instance: phase INIT (x0 and the iteration-0
 // FP64 check run in the next period).
 PoolInst slot_entry(const SlotHost& h, double lipschitz, double tol, uint64_t limit, int32_t maximize, double offset,
-                    bool diag) {
+                    bool diag, bool zero_start) {
     PoolInst t;
     std::memset(&t, 0, sizeof t);
     t.lipschitz = lipschitz;
@@ -445,6 +447,7 @@ PoolInst slot_entry(const SlotHost& h, double lipschitz, double tol, uint64_t li
     t.r0 = int(h.row0);
     t.n = int(h.n);
     t.gen = h.gen;
+    t.zero_start = zero_start ? 1 : 0;
     return t;
 }
 
@@ -477,15 +480,16 @@ void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, con
 int admit(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
           const double* cost, const double* lower, const double* upper, double lipschitz, int32_t maximize,
           double offset, double tol, uint64_t limit, int64_t tag, double* x_out) {
-    const bool diag =
-        validate_instance(n, nnz, start, index, value, cost, lower, upper, x_out, lipschitz, tol, "hgqp_pool_admit");
+    bool zero = false;
+    const bool diag = validate_instance(n, nnz, start, index, value, cost, lower, upper, x_out, lipschitz, tol,
+                                        "hgqp_pool_admit", &zero);
     std::vector<int32_t> blk(3 * (n + 1));
     const int s = reserve(p, n, nnz, start, blk.data());
     if (s < 0) return HGQP_BUSY;
     SlotHost& h = p->slots[s];
     h.tag = tag;
     h.x_out = x_out;
-    const PoolInst entry = slot_entry(h, lipschitz, tol, limit, maximize, offset, diag);
+    const PoolInst entry = slot_entry(h, lipschitz, tol, limit, maximize, offset, diag, zero);
     upload(p, s, start, index, value, cost, lower, upper, &entry);
     HIPCHECK(hipStreamSynchronize(p->stream));  // blk and entry are host temporaries
     return HGQP_OK;
@@ -619,7 +623,7 @@ struct hgqp_server {
         int64_t n = 0, nnz = 0;
         double lipschitz = 0, offset = 0, tol = 0;
         int32_t maximize = 0;
-        bool diag = false;
+        bool diag = false, zero = false;
         uint64_t limit = 0;
         int64_t tag = 0;
         double* x_out = nullptr;
@@ -687,7 +691,7 @@ struct hgqp_server {
             SlotHost& h = pool->slots[s];
             h.tag = r.tag;
             h.x_out = r.x_out;
-            *r.entry = slot_entry(h, r.lipschitz, r.tol, r.limit, r.maximize, r.offset, r.diag);
+            *r.entry = slot_entry(h, r.lipschitz, r.tol, r.limit, r.maximize, r.offset, r.diag, r.zero);
             upload(pool, s, r.start, r.index, r.value, r.cost, r.lower, r.upper, r.entry);
             if (!r.block) HIPCHECK(hipStreamSynchronize(pool->stream));  // pageable source: finish now
             r.x_stage = reinterpret_cast<double*>(staging.take(size_t(r.n) * 8));
@@ -916,8 +920,9 @@ int hgqp_server_submit(hgqp_server* server, int64_t n, int64_t nnz, const int32_
     return guarded([&] {
         if (!server) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_server handle");
         // checked here, on the caller's thread: the loop never sees an invalid instance
+        bool zero = false;
         const bool diag = validate_instance(n, nnz, start, index, value, cost, lower, upper, x_out, lipschitz,
-                                            tolerance, "hgqp_server_submit");
+                                            tolerance, "hgqp_server_submit", &zero);
         if (n + 1 > server->pool->opt.max_rows || nnz > server->pool->opt.max_nnz)
             throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit: instance larger than the pool budget");
         auto r = std::make_unique<hgqp_server::Request>();
@@ -940,6 +945,7 @@ int hgqp_server_submit(hgqp_server* server, int64_t n, int64_t nnz, const int32_
         r->offset = offset;
         r->tol = tolerance;
         r->diag = diag;
+        r->zero = zero;
         r->limit = iteration_limit;
         r->tag = tag;
         r->x_out = x_out;

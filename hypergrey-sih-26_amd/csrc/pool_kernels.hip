@@ -199,7 +199,12 @@ __global__ __launch_bounds__(kGroup) void pool_check64(PoolView v) {
         epi.viol = 0;
         epi.value = epi.tangent = epi.separable = DD{0, 0};
         epi.bound_inf = epi.bad = 0;
-        row_block<double, false, LOADS>(v.start, v.col, v.val, v.x, v.blk_r0[b], v.blk_r1[b], stage[wave], lane, epi);
+        if (v.inst[i].phase == kSlotInit && v.inst[i].zero_start) {  // x0 = 0: H x0 = 0 exactly
+            for (int r = v.blk_r0[b] + lane; r < v.blk_r1[b]; r += kWave) epi(r, 0.0);
+        } else {
+            row_block<double, false, LOADS>(v.start, v.col, v.val, v.x, v.blk_r0[b], v.blk_r1[b], stage[wave], lane,
+                                            epi);
+        }
         const double stat = wave_max(epi.stat), viol = wave_max(epi.viol);
         const DD val = wave_dd(epi.value), tg = wave_dd(epi.tangent), sp = wave_dd(epi.separable);
         const int flags = wave_or(epi.bound_inf | (epi.bad << 1));

@@ -40,6 +40,7 @@ struct PoolInst {
     int32_t r0, n;     // row range
     int32_t frozen;    // parity of the correction a waiting slot stopped at
     int32_t gen;       // admission number of the slot's instance (host bookkeeping)
+    int32_t zero_start;  // 0 lies in every box: x0 = 0, so the first check needs no matrix pass
 };
 
 struct PoolView {
