@@ -375,6 +375,114 @@ int hgqp_device_name(char* buffer, size_t length) {
     });
 }
 
+}  // extern "C"
+
+namespace hgqp {
+
+// Builds a handle from a CSR the caller has checked: host arrays (uploaded), or
+// device arrays from the device ingest (copied on the device; `start` is then
+// still the host copy of the row starts, used for row blocking).
+hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double* cost, const double* lower,
+                     const double* upper, double lipschitz, const hgqp_box_options* options, bool diag) {
+    const int32_t* start = src.start;
+    const int32_t* index = src.index;
+    const double* value = src.value;
+    const hipMemcpyKind kind = src.dev_start ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    require_device();
+    hgqp_box_options opt;
+    if (options) opt = *options;
+    else hgqp_box_options_default(&opt);
+    if (opt.precision != HGQP_PRECISION_FP64 && opt.precision != HGQP_PRECISION_FP32_POLISH &&
+        opt.precision != HGQP_PRECISION_FP32_REFINE)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown precision");
+    if (opt.momentum != HGQP_MOMENTUM_REFERENCE && opt.momentum != HGQP_MOMENTUM_RESTART)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown momentum schedule");
+    if (opt.reduce != HGQP_REDUCE_ORDERED && opt.reduce != HGQP_REDUCE_TREE)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown reduction order");
+    if (opt.step_metric != HGQP_STEP_SCALAR && opt.step_metric != HGQP_STEP_JACOBI)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
+
+    auto b = std::make_unique<hgqp_box>();
+    b->n = n;
+    b->nnz = nnz;
+    b->opt = opt;
+    b->lipschitz = lipschitz;
+    b->diagonal = opt.diagonal_hessian < 0 ? (diag ? 1 : 0) : (opt.diagonal_hessian ? 1 : 0);
+    b->zero_start = true;
+    for (int64_t j = 0; j < n && b->zero_start; ++j) b->zero_start = lower[j] <= 0 && 0 <= upper[j];
+    b->at_zero = b->zero_start;
+    std::vector<int> blk = row_blocks(n, start, kCap, kMaxRows);
+    b->nblocks = static_cast<int>(blk.size()) - 1;
+    b->groups = groups_for(b->nblocks);
+    const bool f32 = opt.precision != HGQP_PRECISION_FP64;
+    const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
+    const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+    size_t bytes = Arena::span(blk.size(), 4) + Arena::span(un + 1, 4) + Arena::span(unnz, 4) +
+                   Arena::span(unnz, 8) + 3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) +
+                   Arena::span(1, sizeof(Control)) + Arena::span(size_t(b->groups) * 10 + 16, 8);
+    if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
+    if (refine) bytes += Arena::span(un, 8);
+    const bool jacobi = opt.step_metric == HGQP_STEP_JACOBI;
+    if (jacobi) bytes += Arena::span(un, 8) + (f32 ? Arena::span(un, 4) : 0);
+    HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->arena.reserve(bytes);
+    b->blk = b->arena.take<int>(blk.size());
+    b->start = b->arena.take<int>(un + 1);
+    b->col = b->arena.take<int>(unnz);
+    b->val = b->arena.take<double>(unnz);
+    b->c = b->arena.take<double>(un);
+    b->lo = b->arena.take<double>(un);
+    b->up = b->arena.take<double>(un);
+    for (int p = 0; p < 2; ++p) {
+        b->x[p] = b->arena.take<double>(un);
+        b->w[p] = b->arena.take<double>(un);
+    }
+    b->ctrl = b->arena.take<Control>(1);
+    b->partials = b->arena.take<double>(size_t(b->groups) * 10 + 16);
+    if (f32) {
+        b->val32 = b->arena.take<float>(unnz);
+        b->c32 = b->arena.take<float>(un);
+        b->lo32 = b->arena.take<float>(un);
+        b->up32 = b->arena.take<float>(un);
+        for (int p = 0; p < 2; ++p) {
+            b->x32[p] = b->arena.take<float>(un);
+            b->w32[p] = b->arena.take<float>(un);
+        }
+    }
+    if (refine) b->g = b->arena.take<double>(un);
+    if (jacobi) {
+        b->dstep = b->arena.take<double>(un);
+        if (f32) b->dstep32 = b->arena.take<float>(un);
+    }
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(Control) * hgqp_box::kSlots,
+                           hipHostMallocDefault));
+    for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHECK(hipMemcpyAsync(b->blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->start, src.dev_start ? src.dev_start : start, (un + 1) * 4, kind, b->stream));
+    if (nnz) {
+        HIPCHECK(hipMemcpyAsync(b->col, index, size_t(nnz) * 4, kind, b->stream));
+        HIPCHECK(hipMemcpyAsync(b->val, value, size_t(nnz) * 8, kind, b->stream));
+    }
+    HIPCHECK(hipMemcpyAsync(b->c, cost, un * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
+    HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
+    if (jacobi) launch_row_abs(n, b->start, b->val, lipschitz, b->dstep, b->dstep32, b->stream);
+    if (f32) {
+        if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
+        launch_convert<float, double>(n, b->c, b->c32, b->stream);
+        launch_convert<float, double>(n, b->lo, b->lo32, b->stream);
+        launch_convert<float, double>(n, b->up, b->up32, b->stream);
+    }
+    launch_init<double>(n, b->lo, b->up, b->x[0], b->w[0], b->stream);
+    HIPCHECK(hipGetLastError());
+    put_control(b.get(), fresh_control());  // synchronizes: the caller's arrays may be released
+    return b.release();
+}
+
+}  // namespace hgqp
+
+extern "C" {
+
 int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t* index, const double* value,
                     const double* cost, const double* lower, const double* upper, double lipschitz,
                     const hgqp_box_options* options, hgqp_box** out) {
@@ -404,95 +512,8 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         for (int64_t j = 0; j < n; ++j)
             if (std::isnan(lower[j]) || std::isnan(upper[j]) || lower[j] > upper[j])
                 throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: bounds must satisfy lower <= upper");
-        require_device();
-        hgqp_box_options opt;
-        if (options) opt = *options;
-        else hgqp_box_options_default(&opt);
-        if (opt.precision != HGQP_PRECISION_FP64 && opt.precision != HGQP_PRECISION_FP32_POLISH &&
-            opt.precision != HGQP_PRECISION_FP32_REFINE)
-            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown precision");
-        if (opt.momentum != HGQP_MOMENTUM_REFERENCE && opt.momentum != HGQP_MOMENTUM_RESTART)
-            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown momentum schedule");
-        if (opt.reduce != HGQP_REDUCE_ORDERED && opt.reduce != HGQP_REDUCE_TREE)
-            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown reduction order");
-        if (opt.step_metric != HGQP_STEP_SCALAR && opt.step_metric != HGQP_STEP_JACOBI)
-            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
-
-        auto b = std::make_unique<hgqp_box>();
-        b->n = n;
-        b->nnz = nnz;
-        b->opt = opt;
-        b->lipschitz = lipschitz;
-        b->diagonal = opt.diagonal_hessian < 0 ? (diag ? 1 : 0) : (opt.diagonal_hessian ? 1 : 0);
-        b->zero_start = true;
-        for (int64_t j = 0; j < n && b->zero_start; ++j) b->zero_start = lower[j] <= 0 && 0 <= upper[j];
-        b->at_zero = b->zero_start;
-        std::vector<int> blk = row_blocks(n, start, kCap, kMaxRows);
-        b->nblocks = static_cast<int>(blk.size()) - 1;
-        b->groups = groups_for(b->nblocks);
-        const bool f32 = opt.precision != HGQP_PRECISION_FP64;
-        const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
-        const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
-        size_t bytes = Arena::span(blk.size(), 4) + Arena::span(un + 1, 4) + Arena::span(unnz, 4) +
-                       Arena::span(unnz, 8) + 3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) +
-                       Arena::span(1, sizeof(Control)) + Arena::span(size_t(b->groups) * 10 + 16, 8);
-        if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
-        if (refine) bytes += Arena::span(un, 8);
-        const bool jacobi = opt.step_metric == HGQP_STEP_JACOBI;
-        if (jacobi) bytes += Arena::span(un, 8) + (f32 ? Arena::span(un, 4) : 0);
-        HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        b->arena.reserve(bytes);
-        b->blk = b->arena.take<int>(blk.size());
-        b->start = b->arena.take<int>(un + 1);
-        b->col = b->arena.take<int>(unnz);
-        b->val = b->arena.take<double>(unnz);
-        b->c = b->arena.take<double>(un);
-        b->lo = b->arena.take<double>(un);
-        b->up = b->arena.take<double>(un);
-        for (int p = 0; p < 2; ++p) {
-            b->x[p] = b->arena.take<double>(un);
-            b->w[p] = b->arena.take<double>(un);
-        }
-        b->ctrl = b->arena.take<Control>(1);
-        b->partials = b->arena.take<double>(size_t(b->groups) * 10 + 16);
-        if (f32) {
-            b->val32 = b->arena.take<float>(unnz);
-            b->c32 = b->arena.take<float>(un);
-            b->lo32 = b->arena.take<float>(un);
-            b->up32 = b->arena.take<float>(un);
-            for (int p = 0; p < 2; ++p) {
-                b->x32[p] = b->arena.take<float>(un);
-                b->w32[p] = b->arena.take<float>(un);
-            }
-        }
-        if (refine) b->g = b->arena.take<double>(un);
-        if (jacobi) {
-            b->dstep = b->arena.take<double>(un);
-            if (f32) b->dstep32 = b->arena.take<float>(un);
-        }
-        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(Control) * hgqp_box::kSlots,
-                               hipHostMallocDefault));
-        for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        HIPCHECK(hipMemcpyAsync(b->blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, b->stream));
-        HIPCHECK(hipMemcpyAsync(b->start, start, (un + 1) * 4, hipMemcpyHostToDevice, b->stream));
-        if (nnz) {
-            HIPCHECK(hipMemcpyAsync(b->col, index, un ? size_t(nnz) * 4 : 0, hipMemcpyHostToDevice, b->stream));
-            HIPCHECK(hipMemcpyAsync(b->val, value, size_t(nnz) * 8, hipMemcpyHostToDevice, b->stream));
-        }
-        HIPCHECK(hipMemcpyAsync(b->c, cost, un * 8, hipMemcpyHostToDevice, b->stream));
-        HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
-        HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
-        if (jacobi) launch_row_abs(n, b->start, b->val, lipschitz, b->dstep, b->dstep32, b->stream);
-        if (f32) {
-            if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
-            launch_convert<float, double>(n, b->c, b->c32, b->stream);
-            launch_convert<float, double>(n, b->lo, b->lo32, b->stream);
-            launch_convert<float, double>(n, b->up, b->up32, b->stream);
-        }
-        launch_init<double>(n, b->lo, b->up, b->x[0], b->w[0], b->stream);
-        HIPCHECK(hipGetLastError());
-        put_control(b.get(), fresh_control());  // synchronizes: the caller's arrays may be released
-        *out = b.release();
+        *out = hgqp::create_box(n, nnz, hgqp::CsrSource{start, index, value, nullptr}, cost, lower, upper, lipschitz,
+                                options, diag);
     });
 }
 

@@ -51,4 +51,18 @@ int guarded(F&& body) {
 // own. Returns the first row of every block followed by n.
 std::vector<int> row_blocks(int64_t n, const int32_t* start, int cap, int max_rows);
 
+// The CSR a box handle is built from: host arrays, or (dev_start set) device
+// arrays from the device ingest, with `start` the host copy of the row starts.
+struct CsrSource {
+    const int32_t* start;
+    const int32_t* index;
+    const double* value;
+    const int32_t* dev_start;
+};
+
+// hgqp_box_create after its input checks: `diag` is the positive-diagonal flag
+// derived from the matrix (used when options->diagonal_hessian is -1).
+hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double* cost, const double* lower,
+                     const double* upper, double lipschitz, const hgqp_box_options* options, bool diag);
+
 }  // namespace hgqp

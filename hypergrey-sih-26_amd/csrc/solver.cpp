@@ -4,13 +4,17 @@
 // the HIP engine in place of the CUDA one. Also the whole-model C entry
 // hgqp_solve_qp.
 #include "hgqp/accelerated_qp.hpp"
+#include "device_util.hpp"
+#include "ingest.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 
 namespace hgqp {
 
@@ -59,16 +63,24 @@ void validate_linear_problem(const LinearProblem& p) {
     }
 }
 
-bool check_convex_hessian(const QuadraticProblem& q) {
-    const std::size_t n = q.linear.cost.size();
-    const double sign = q.linear.maximize ? -1 : 1;
+namespace {
+
+// check_convex_hessian (qp_admm.cpp:30-62) over any term accessor:
+// term(k, &row, &col, &value), k < count. Rows and columns outside [0, n) or a
+// non-finite value throw "Invalid Hessian".
+template <class Term>
+bool convex_terms(std::size_t n, double sign, std::size_t count, Term term) {
     std::vector<double> diagonal(n, 0), radius(n, 0);
-    for (const auto& a : q.hessian) {
-        if (a.row >= n || a.col >= n || !std::isfinite(a.value)) throw std::invalid_argument("Invalid Hessian");
-        if (a.row == a.col) diagonal[a.row] += sign * a.value;
+    for (std::size_t k = 0; k < count; ++k) {
+        int64_t r, c;
+        double v;
+        term(k, &r, &c, &v);
+        if (r < 0 || c < 0 || std::size_t(r) >= n || std::size_t(c) >= n || !std::isfinite(v))
+            throw std::invalid_argument("Invalid Hessian");
+        if (r == c) diagonal[r] += sign * v;
         else {
-            radius[a.row] += std::abs(a.value);
-            radius[a.col] += std::abs(a.value);
+            radius[r] += std::abs(v);
+            radius[c] += std::abs(v);
         }
     }
     bool dd = true;
@@ -80,11 +92,14 @@ bool check_convex_hessian(const QuadraticProblem& q) {
     // Not diagonally dominant: LDL^T of the congruence-scaled matrix with full
     // fill (cap 4e6 fill entries), pivots <= 1e-12 must have a zero column.
     std::vector<std::map<std::size_t, double> > lower(n);
-    for (const auto& a : q.hessian) {
-        std::size_t i = std::max(a.row, a.col), j = std::min(a.row, a.col);
-        if (diagonal[i] > 0 && diagonal[j] > 0) lower[j][i] += sign * a.value / std::sqrt(diagonal[i] * diagonal[j]);
+    for (std::size_t k = 0; k < count; ++k) {
+        int64_t r, c;
+        double v;
+        term(k, &r, &c, &v);
+        const std::size_t i = std::size_t(std::max(r, c)), j = std::size_t(std::min(r, c));
+        if (diagonal[i] > 0 && diagonal[j] > 0) lower[j][i] += sign * v / std::sqrt(diagonal[i] * diagonal[j]);
     }
-    std::size_t fill = q.hessian.size();
+    std::size_t fill = count;
     for (std::size_t k = 0; k < n; ++k) {
         auto kk = lower[k].find(k);
         double d = kk != lower[k].end() ? kk->second : 0;
@@ -106,6 +121,18 @@ bool check_convex_hessian(const QuadraticProblem& q) {
         lower[k].clear();
     }
     return true;
+}
+
+}  // namespace
+
+bool check_convex_hessian(const QuadraticProblem& q) {
+    const auto& h = q.hessian;
+    return convex_terms(q.linear.cost.size(), q.linear.maximize ? -1 : 1, h.size(),
+                        [&](std::size_t k, int64_t* r, int64_t* c, double* v) {
+                            *r = int64_t(h[k].row);
+                            *c = int64_t(h[k].col);
+                            *v = h[k].value;
+                        });
 }
 
 namespace {
@@ -131,22 +158,105 @@ struct BoxHandle {
 
 }  // namespace
 
+// A box QP as the solve entries see it: the reference's QuadraticProblem
+// (terms as QuadraticTerm records) or the arrays of hgqp_problem.
+struct ModelView {
+    std::size_t n = 0, nterms = 0;
+    const QuadraticTerm* terms = nullptr;
+    const int64_t* hrow = nullptr;
+    const int64_t* hcol = nullptr;
+    const double* hval = nullptr;
+    const double *cost = nullptr, *lower = nullptr, *upper = nullptr;
+    double offset = 0;
+    bool maximize = false;
+    void term(std::size_t k, int64_t* r, int64_t* c, double* v) const {
+        if (terms) {
+            *r = int64_t(terms[k].row);
+            *c = int64_t(terms[k].col);
+            *v = terms[k].value;
+        } else {
+            *r = hrow[k];
+            *c = hcol[k];
+            *v = hval[k];
+        }
+    }
+};
+
+static_assert(sizeof(QuadraticTerm) == 24 && offsetof(QuadraticTerm, row) == 0 && offsetof(QuadraticTerm, col) == 8 &&
+                  offsetof(QuadraticTerm, value) == 16,
+              "QuadraticTerm records are uploaded as three 8-byte fields");
+
+ModelView view_of(const QuadraticProblem& q) {
+    ModelView m;
+    m.n = q.linear.cost.size();
+    m.nterms = q.hessian.size();
+    m.terms = q.hessian.data();
+    m.cost = q.linear.cost.data();
+    m.lower = q.linear.lower.data();
+    m.upper = q.linear.upper.data();
+    m.offset = q.linear.offset;
+    m.maximize = q.linear.maximize;
+    return m;
+}
+
+// hgqp_problem as a model: the checks of validate_linear_problem
+// (mps.cpp:47-60) for a model without names, integrality or rows.
+ModelView view_of(const hgqp_problem* problem, const char* who) {
+    if (!problem) throw std::invalid_argument(std::string(who) + ": null problem");
+    if (problem->n <= 0 || problem->nterms < 0 || !std::isfinite(problem->offset))
+        throw std::invalid_argument("Invalid linear problem dimensions or offset");
+    if (!problem->cost || !problem->lower || !problem->upper ||
+        (problem->nterms && (!problem->hrow || !problem->hcol || !problem->hval)))
+        throw std::invalid_argument(std::string(who) + ": null array");
+    ModelView m;
+    m.n = std::size_t(problem->n);
+    m.nterms = std::size_t(problem->nterms);
+    m.hrow = problem->hrow;
+    m.hcol = problem->hcol;
+    m.hval = problem->hval;
+    m.cost = problem->cost;
+    m.lower = problem->lower;
+    m.upper = problem->upper;
+    m.offset = problem->offset;
+    m.maximize = problem->maximize != 0;
+    for (std::size_t j = 0; j < m.n; ++j)
+        if (!std::isfinite(m.cost[j]) || std::isnan(m.lower[j]) || std::isnan(m.upper[j]) || m.lower[j] == kInf ||
+            m.upper[j] == -kInf)
+            throw std::invalid_argument("Invalid variable coefficient or bound");
+    return m;
+}
+
 struct PreparedBox {
-    CsrMatrix h;
+    DeviceCsr csr;  // the sign-normalized mirrored CSR, assembled on the device
     std::vector<double> cost;
     double lipschitz = 0;
     bool positive_diagonal = false;
 };
 
-// The host side of accelerated_qp.cpp:14-35 shared by the single-instance
-// solve and the served path: validation, the convexity gate, the
-// sign-normalized mirrored CSR, L, and the early statuses. Returns false with
-// r.status set when the model ends before any iteration.
-bool prepare_box(const QuadraticProblem& q, const QpOptions& opt, QpResult& r, PreparedBox& out) {
-    validate_linear_problem(q.linear);
-    if (!q.linear.rows.empty()) throw std::invalid_argument("Accelerated box QP requires box constraints only");
-    for (bool v : q.linear.integer)
-        if (v) throw std::invalid_argument("Accelerated box QP requires continuous variables");
+struct StreamHandle {
+    hipStream_t s = nullptr;
+    StreamHandle() { HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~StreamHandle() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+struct DeviceBytes {
+    void* p = nullptr;
+    explicit DeviceBytes(size_t bytes) {
+        if (bytes) HIPCHECK(hipMalloc(&p, bytes));
+    }
+    ~DeviceBytes() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// The host side of accelerated_qp.cpp:14-35 after the model checks, shared by
+// the single-instance solve and the served path: options, the convexity gate
+// (host: one pass over the terms, LDL^T only when not diagonally dominant), the
+// sign-normalized mirrored CSR and L (device ingest), and the early statuses.
+// Returns false with r.status set when the model ends before any iteration.
+bool prepare_model(const ModelView& m, const QpOptions& opt, QpResult& r, PreparedBox& out) {
     if (opt.tolerance <= 0 || !std::isfinite(opt.tolerance) || opt.time_limit < 0 || !std::isfinite(opt.time_limit) ||
         opt.threads < 1)
         throw std::invalid_argument("Invalid box QP options");
@@ -155,76 +265,91 @@ bool prepare_box(const QuadraticProblem& q, const QpOptions& opt, QpResult& r, P
                                                                 : "hip_fista_box";
     if (opt.momentum == HGQP_MOMENTUM_RESTART) r.algorithm += "_restart";
     if (opt.step_metric == HGQP_STEP_JACOBI) r.algorithm += "_jacobi";
-    if (!check_convex_hessian(q)) {
+    const double sign = m.maximize ? -1 : 1;
+    const auto term = [&](std::size_t k, int64_t* rr, int64_t* cc, double* vv) { m.term(k, rr, cc, vv); };
+    if (!convex_terms(m.n, sign, m.nterms, term)) {
         r.status = "unsupported_curvature";
         return false;
     }
-    const auto& p = q.linear;
-    const std::size_t n = p.cost.size();
-    const double sign = p.maximize ? -1 : 1;
-    std::vector<std::vector<std::pair<std::size_t, double> > > rows(n);
-    out.cost = p.cost;
-    for (double& v : out.cost) v *= sign;
-    bool positive_diagonal = true;
-    for (const auto& h : q.hessian) {
-        rows[h.row].push_back({h.col, sign * h.value});
-        if (h.row != h.col) {
-            rows[h.col].push_back({h.row, sign * h.value});
-            if (h.value != 0) positive_diagonal = false;
-        }
+    // positive diagonal: no nonzero off-diagonal term, every diagonal sum > 0
+    bool positive_diagonal = true, all_zero = true;
+    for (std::size_t k = 0; k < m.nterms; ++k) {
+        int64_t rr, cc;
+        double vv;
+        m.term(k, &rr, &cc, &vv);
+        if (vv != 0) all_zero = false;
+        if (rr != cc && vv != 0) positive_diagonal = false;
+        if (!all_zero && !positive_diagonal) break;
     }
+    // every term zero: L = 0 for certain (accelerated_qp.cpp:28); cancelling
+    // duplicates are found after the assembly below
+    if (all_zero) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
     if (positive_diagonal) {
-        std::vector<long double> diagonal(n, 0);
-        for (const auto& h : q.hessian)
-            if (h.row == h.col) diagonal[h.row] += sign * h.value;
+        std::vector<long double> diagonal(m.n, 0);
+        for (std::size_t k = 0; k < m.nterms; ++k) {
+            int64_t rr, cc;
+            double vv;
+            m.term(k, &rr, &cc, &vv);
+            if (rr == cc) diagonal[rr] += sign * vv;
+        }
         for (long double d : diagonal)
             if (!(d > 0)) positive_diagonal = false;
     }
     out.positive_diagonal = positive_diagonal;
-    out.h = CsrMatrix(n, rows);
-    rows.clear();
-    rows.shrink_to_fit();
-    double lipschitz = 0;
-    for (std::size_t i = 0; i < n; ++i) {
-        double sum = 0;
-        for (int k = out.h.start[i]; k < out.h.start[i + 1]; ++k) sum += std::abs(out.h.value[k]);
-        lipschitz = std::max(lipschitz, sum);
+    out.cost.assign(m.cost, m.cost + m.n);
+    for (double& v : out.cost) v *= sign;
+    // CSR and L on the device
+    require_device();
+    StreamHandle st;
+    TermView tv{};
+    std::unique_ptr<DeviceBytes> buf;
+    if (m.terms) {  // QuadraticTerm records: one upload, strided view
+        buf.reset(new DeviceBytes(m.nterms * sizeof(QuadraticTerm)));
+        if (m.nterms)
+            HIPCHECK(hipMemcpyAsync(buf->p, m.terms, m.nterms * sizeof(QuadraticTerm), hipMemcpyHostToDevice, st.s));
+        const int64_t* base = static_cast<const int64_t*>(buf->p);
+        tv = TermView{base, base + 1, reinterpret_cast<const double*>(base + 2), 3};
+    } else {
+        buf.reset(new DeviceBytes(m.nterms * 24));
+        int64_t* base = static_cast<int64_t*>(buf->p);
+        if (m.nterms) {
+            HIPCHECK(hipMemcpyAsync(base, m.hrow, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+            HIPCHECK(hipMemcpyAsync(base + m.nterms, m.hcol, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+            HIPCHECK(hipMemcpyAsync(base + 2 * m.nterms, m.hval, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+        }
+        tv = TermView{base, base + m.nterms, reinterpret_cast<const double*>(base + 2 * m.nterms), 1};
     }
-    if (lipschitz == 0) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
-    out.lipschitz = lipschitz;
-    for (std::size_t j = 0; j < n; ++j)
-        if (p.lower[j] > p.upper[j]) {
+    const IngestReport rep = ingest_terms(int64_t(m.n), int64_t(m.nterms), tv, sign, out.csr, st.s);
+    if (rep.bad_term) throw std::invalid_argument("Invalid Hessian");
+    if (rep.lipschitz == 0) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
+    out.lipschitz = rep.lipschitz;
+    for (std::size_t j = 0; j < m.n; ++j)
+        if (m.lower[j] > m.upper[j]) {
             r.status = "infeasible_bounds";
             return false;
         }
     return true;
 }
 
-QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, const std::string& backend) {
-    const auto t0 = std::chrono::steady_clock::now();
-    if (backend != "hip") throw std::invalid_argument("Backend must be hip");
+// The solve of accelerated_qp.cpp:37-59 on a prepared model: a resident
+// handle built from the device CSR, the device-resident loop, x read back.
+QpResult solve_model(const ModelView& m, const QpOptions& opt, std::chrono::steady_clock::time_point t0) {
     QpResult r;
     PreparedBox pb;
-    if (!prepare_box(q, opt, r, pb)) return r;
-    const auto& p = q.linear;
-    const std::size_t n = p.cost.size();
-    const CsrMatrix& h = pb.h;
-    const std::vector<double>& c = pb.cost;
-    const double lipschitz = pb.lipschitz;
-    const bool positive_diagonal = pb.positive_diagonal;
+    if (!prepare_model(m, opt, r, pb)) return r;
     hgqp_box_options bo;
     hgqp_box_options_default(&bo);
     bo.precision = opt.precision;
     bo.momentum = opt.momentum;
     bo.reduce = opt.reduce;
     bo.step_metric = opt.step_metric;
-    bo.maximize = p.maximize ? 1 : 0;
-    bo.offset = p.offset;
-    bo.diagonal_hessian = positive_diagonal ? 1 : 0;
+    bo.maximize = m.maximize ? 1 : 0;
+    bo.offset = m.offset;
+    bo.diagonal_hessian = pb.positive_diagonal ? 1 : 0;
     BoxHandle box;
-    HipBox::raise(hgqp_box_create(static_cast<int64_t>(n), static_cast<int64_t>(h.value.size()), h.start.data(),
-                                  h.index.data(), h.value.data(), c.data(), p.lower.data(), p.upper.data(), lipschitz,
-                                  &bo, &box.b));
+    box.b = create_box(int64_t(m.n), pb.csr.nnz,
+                       CsrSource{pb.csr.host_start.data(), pb.csr.col, pb.csr.val, pb.csr.start}, pb.cost.data(),
+                       m.lower, m.upper, pb.lipschitz, &bo, pb.positive_diagonal);
     hgqp_solve_options so;
     hgqp_solve_options_default(&so);
     so.tolerance = opt.tolerance;
@@ -244,9 +369,19 @@ QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, 
     r.optimality_verified = rep.check.optimality_verified != 0;
     r.fp32_iterations = static_cast<std::size_t>(rep.fp32_iterations);
     r.restarts = rep.restarts;
-    r.primal.resize(n);
-    HipBox::raise(hgqp_box_primal(box.b, r.primal.data(), static_cast<int64_t>(n)));
+    r.primal.resize(m.n);
+    HipBox::raise(hgqp_box_primal(box.b, r.primal.data(), static_cast<int64_t>(m.n)));
     return r;
+}
+
+QpResult solve_accelerated_box(const QuadraticProblem& q, const QpOptions& opt, const std::string& backend) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (backend != "hip") throw std::invalid_argument("Backend must be hip");
+    validate_linear_problem(q.linear);
+    if (!q.linear.rows.empty()) throw std::invalid_argument("Accelerated box QP requires box constraints only");
+    for (bool v : q.linear.integer)
+        if (v) throw std::invalid_argument("Accelerated box QP requires continuous variables");
+    return solve_model(view_of(q), opt, t0);
 }
 
 }  // namespace hgqp
@@ -266,52 +401,20 @@ extern "C" void hgqp_qp_options_default(hgqp_qp_options* o) {
     o->step_metric = HGQP_STEP_SCALAR;
 }
 
-namespace {
-
-// The C problem view as the reference's QuadraticProblem (qp.hpp:5-10), with
-// names and integrality filled so validation sees a complete model
-// (mps.cpp:49-51).
-QuadraticProblem to_quadratic_problem(const hgqp_problem* problem, const char* who) {
-    if (!problem) throw std::invalid_argument(std::string(who) + ": null problem");
-    if (problem->n <= 0 || problem->nterms < 0) throw std::invalid_argument("Invalid linear problem dimensions or offset");
-    const std::size_t n = static_cast<std::size_t>(problem->n);
-    if (!problem->cost || !problem->lower || !problem->upper ||
-        (problem->nterms && (!problem->hrow || !problem->hcol || !problem->hval)))
-        throw std::invalid_argument(std::string(who) + ": null array");
-    QuadraticProblem q;
-    auto& lp = q.linear;
-    lp.names.resize(n);
-    lp.integer.assign(n, false);
-    lp.cost.assign(problem->cost, problem->cost + n);
-    lp.lower.assign(problem->lower, problem->lower + n);
-    lp.upper.assign(problem->upper, problem->upper + n);
-    lp.offset = problem->offset;
-    lp.maximize = problem->maximize != 0;
-    q.hessian.resize(static_cast<std::size_t>(problem->nterms));
-    for (int64_t k = 0; k < problem->nterms; ++k) {
-        if (problem->hrow[k] < 0 || problem->hcol[k] < 0) throw std::invalid_argument("Invalid Hessian");
-        q.hessian[k] = {static_cast<std::size_t>(problem->hrow[k]), static_cast<std::size_t>(problem->hcol[k]),
-                        problem->hval[k]};
-    }
-    return q;
-}
-
-}  // namespace
-
 extern "C" int hgqp_server_submit_problem(hgqp_server* server, const hgqp_problem* problem, double tolerance,
                                           uint64_t iteration_limit, int64_t tag, double* x_out, int32_t* status) {
     int unsupported = 0;
     int rc = guarded([&] {
         if (!server || !x_out || !status) throw std::invalid_argument("hgqp_server_submit_problem: null argument");
         *status = HGQP_STATUS_RUNNING;
-        const QuadraticProblem q = to_quadratic_problem(problem, "hgqp_server_submit_problem");
+        const ModelView m = view_of(problem, "hgqp_server_submit_problem");
         QpOptions opt;
         opt.tolerance = tolerance;
         QpResult r;
         PreparedBox pb;
         bool ready;
         try {
-            ready = prepare_box(q, opt, r, pb);
+            ready = prepare_model(m, opt, r, pb);
         } catch (const UnsupportedModel&) {
             unsupported = 1;
             throw;
@@ -321,23 +424,66 @@ extern "C" int hgqp_server_submit_problem(hgqp_server* server, const hgqp_proble
                                                       : HGQP_STATUS_UNSUPPORTED_CURVATURE;
             return;
         }
-        const auto& p = q.linear;
-        const int code = hgqp_server_submit(server, problem->n, static_cast<int64_t>(pb.h.value.size()),
-                                            pb.h.start.data(), pb.h.index.data(), pb.h.value.data(), pb.cost.data(),
-                                            p.lower.data(), p.upper.data(), pb.lipschitz, p.maximize ? 1 : 0,
-                                            p.offset, tolerance, iteration_limit, tag, x_out);
+        // the serving loop stages host arrays: read the assembled CSR back
+        std::vector<int32_t> col(size_t(pb.csr.nnz));
+        std::vector<double> val(size_t(pb.csr.nnz));
+        if (pb.csr.nnz) {
+            HIPCHECK(hipMemcpy(col.data(), pb.csr.col, col.size() * 4, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(val.data(), pb.csr.val, val.size() * 8, hipMemcpyDeviceToHost));
+        }
+        const int code = hgqp_server_submit(server, problem->n, pb.csr.nnz, pb.csr.host_start.data(), col.data(),
+                                            val.data(), pb.cost.data(), m.lower, m.upper, pb.lipschitz,
+                                            m.maximize ? 1 : 0, m.offset, tolerance, iteration_limit, tag, x_out);
         if (code != HGQP_OK) throw Error(code, hgqp_last_error());
     });
     return unsupported ? HGQP_UNSUPPORTED : rc;
+}
+
+extern "C" int hgqp_csr_assemble(const hgqp_problem* problem, int64_t capacity, int32_t* start, int32_t* index,
+                                 double* value, int64_t* nnz, double* lipschitz) {
+    return guarded([&] {
+        if (!nnz || !lipschitz) throw std::invalid_argument("hgqp_csr_assemble: null output");
+        const ModelView m = view_of(problem, "hgqp_csr_assemble");
+        const double sign = m.maximize ? -1 : 1;
+        for (std::size_t k = 0; k < m.nterms; ++k) {
+            int64_t r, c;
+            double v;
+            m.term(k, &r, &c, &v);
+            if (r < 0 || c < 0 || std::size_t(r) >= m.n || std::size_t(c) >= m.n || !std::isfinite(v))
+                throw std::invalid_argument("Invalid Hessian");
+        }
+        require_device();
+        StreamHandle st;
+        DeviceBytes buf(m.nterms * 24);
+        int64_t* base = static_cast<int64_t*>(buf.p);
+        if (m.nterms) {
+            HIPCHECK(hipMemcpyAsync(base, m.hrow, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+            HIPCHECK(hipMemcpyAsync(base + m.nterms, m.hcol, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+            HIPCHECK(hipMemcpyAsync(base + 2 * m.nterms, m.hval, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+        }
+        const TermView tv{base, base + m.nterms, reinterpret_cast<const double*>(base + 2 * m.nterms), 1};
+        DeviceCsr csr;
+        const IngestReport rep = ingest_terms(int64_t(m.n), int64_t(m.nterms), tv, sign, csr, st.s);
+        if (rep.bad_term) throw std::invalid_argument("Invalid Hessian");
+        *nnz = csr.nnz;
+        *lipschitz = rep.lipschitz;
+        if (!index || !value) return;
+        if (!start || capacity < csr.nnz) throw std::invalid_argument("hgqp_csr_assemble: output capacity below nnz");
+        std::memcpy(start, csr.host_start.data(), (m.n + 1) * sizeof(int32_t));
+        if (csr.nnz) {
+            HIPCHECK(hipMemcpy(index, csr.col, size_t(csr.nnz) * 4, hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(value, csr.val, size_t(csr.nnz) * 8, hipMemcpyDeviceToHost));
+        }
+    });
 }
 
 extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, hgqp_qp_result* result,
                              double* primal_out) {
     int unsupported = 0;
     int rc = guarded([&] {
+        const auto t0 = std::chrono::steady_clock::now();
         if (!problem || !result) throw std::invalid_argument("hgqp_solve_qp: null problem or result");
-        const QuadraticProblem q = to_quadratic_problem(problem, "hgqp_solve_qp");
-        const std::size_t n = static_cast<std::size_t>(problem->n);
+        const ModelView m = view_of(problem, "hgqp_solve_qp");
         hgqp_qp_options o;
         if (options) o = *options;
         else hgqp_qp_options_default(&o);
@@ -350,10 +496,9 @@ extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options*
         opt.momentum = o.momentum;
         opt.reduce = o.reduce;
         opt.step_metric = o.step_metric;
-        const auto t0 = std::chrono::steady_clock::now();
         QpResult r;
         try {
-            r = solve_accelerated_box(q, opt, "hip");
+            r = solve_model(m, opt, t0);
         } catch (const UnsupportedModel& e) {
             unsupported = 1;
             throw;
@@ -372,7 +517,7 @@ extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options*
         result->restarts = r.restarts;
         result->fp32_iterations = r.fp32_iterations;
         result->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (primal_out && r.primal.size() == n) std::memcpy(primal_out, r.primal.data(), n * sizeof(double));
+        if (primal_out && r.primal.size() == m.n) std::memcpy(primal_out, r.primal.data(), m.n * sizeof(double));
     });
     return unsupported ? HGQP_UNSUPPORTED : rc;
 }

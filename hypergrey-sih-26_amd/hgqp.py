@@ -173,6 +173,7 @@ SIGNATURES = {
     "hgqp_box_info_get": ([C.c_void_p, C.POINTER(BoxInfo)], C.c_int),
     "hgqp_solve_qp": ([C.POINTER(Problem), C.POINTER(QpOptions), C.POINTER(QpResult), P_F64], C.c_int),
     "hgqp_csr_lipschitz": ([C.c_int64, P_I32, P_F64], C.c_double),
+    "hgqp_csr_assemble": ([C.POINTER(Problem), C.c_int64, P_I32, P_I32, P_F64, P_I64, P_F64], C.c_int),
     "hgqp_dense_options_default": ([C.POINTER(DenseOptions)], None),
     "hgqp_dense_create": ([C.c_int64, C.c_int64, P_F64, P_F64, P_F64, P_F64, C.POINTER(DenseOptions),
                            C.POINTER(C.c_void_p)], C.c_int),
@@ -349,6 +350,19 @@ def solve_qp(n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False
     d = r.as_dict()
     d["primal"] = x
     return d
+
+
+def csr_assemble(n, hrow, hcol, hval, cost, lower, upper, maximize=False):
+    """hgqp_csr_assemble: the sign-normalized mirrored CSR and L, assembled on the device."""
+    p, keep = _problem(n, hrow, hcol, hval, cost, lower, upper, 0.0, maximize)
+    nnz, L = C.c_int64(), C.c_double()
+    _raise(lib().hgqp_csr_assemble(C.byref(p), 0, None, None, None, C.byref(nnz), C.byref(L)))
+    start = np.zeros(int(n) + 1, np.int32)
+    index = np.zeros(max(nnz.value, 1), np.int32)
+    value = np.zeros(max(nnz.value, 1))
+    _raise(lib().hgqp_csr_assemble(C.byref(p), len(index), _ptr(start, C.c_int32), _ptr(index, C.c_int32),
+                                   _ptr(value, C.c_double), C.byref(nnz), C.byref(L)))
+    return start, index[:nnz.value].copy(), value[:nnz.value].copy(), L.value
 
 
 def read_qplib(text):

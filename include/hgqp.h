@@ -243,6 +243,18 @@ int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, h
  * (accelerated_qp.cpp:25-26), for callers that assemble the CSR themselves. */
 double hgqp_csr_lipschitz(int64_t n, const int32_t* start, const double* value);
 
+/* The sign-normalized mirrored CSR of a model, assembled on the device: what
+ * accelerated_qp.cpp:20-24 builds through the CsrMatrix constructor
+ * (sparse_matrix.hpp:10-17), bit for bit (each row's entries in term order,
+ * duplicates summed in that order, exact zeros dropped, columns ascending),
+ * and L of accelerated_qp.cpp:25-26. The terms are checked as in
+ * check_convex_hessian (qp_admm.cpp:34). hgqp_solve_qp and
+ * hgqp_server_submit_problem assemble through this path. With index or value
+ * null only *nnz and *lipschitz are returned; otherwise `start` holds n + 1
+ * entries and index / value `capacity` >= *nnz (else HGQP_INVALID_ARGUMENT). */
+int hgqp_csr_assemble(const hgqp_problem* problem, int64_t capacity, int32_t* start, int32_t* index, double* value,
+                      int64_t* nnz, double* lipschitz);
+
 /* ---- dense multi-RHS: one dense Q, many cost vectors -------------------- */
 /* No counterpart in the reference (one sparse instance, one cost vector per
  * process). K box QPs min 1/2 x'Qx + c_j'x over the same box share a dense

@@ -97,11 +97,13 @@ def test_invalid_input_rejected_before_device(hg):
 
 
 def test_host_outcomes_without_device(hg):
-    """Outcomes decided before the engine starts: curvature and bounds."""
+    """Outcomes decided on the host before any device work: curvature (the
+    convexity test runs on the host) and an all-zero Hessian."""
     r = hg.solve_qp(2, [0, 1, 1], [0, 1, 0], [1.0, 1.0, 1.5], [0, 0], [-1, -1], [1, 1])
     assert r["status"] == "unsupported_curvature" and not r["optimality_verified"]
-    r = hg.solve_qp(2, [0, 1], [0, 1], [2.0, 2.0], [-2.0, -4.0], [-10, 3], [10, 2])
-    assert r["status"] == "infeasible_bounds"
+    with pytest.raises(hg.HgqpError) as e:
+        hg.solve_qp(3, [0, 2], [0, 1], [0.0, 0.0], [1.0, 0.0, 0.0], [-1, -1, -1], [1, 1, 1])
+    assert e.value.code == hg.UNSUPPORTED
 
 
 def test_tridiagonal_generator_equals_oracle(hg, oracle):
