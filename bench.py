@@ -134,6 +134,25 @@ def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
             "solves_per_s": round(1.0 / total, 3), "reps": reps}
 
 
+def whole_model(name, inst, check_every, polish_every, reps=2):
+    """The whole-model entry (hgqp_solve_qp) from the reference's triplet form:
+    host checks (validation, convexity), device assembly of the mirrored CSR and
+    L, upload of the vectors, the solve and the read-back of x."""
+    precision, momentum, reduce, metric = MODES[name]
+    hr, hc, hv = hgqp.lower_triplets(inst["start"], inst["index"], inst["value"])
+    n = len(inst["cost"])
+    times, r = [], None
+    for _ in range(reps):
+        a = time.perf_counter()
+        r = hgqp.solve_qp(n, hr, hc, hv, inst["cost"], inst["lower"], inst["upper"], tolerance=KKT_TOL,
+                          iteration_limit=20000, time_limit=120, precision=precision, momentum=momentum,
+                          reduce=reduce, step_metric=metric)
+        times.append(time.perf_counter() - a)
+    return {"mode": name, "terms": int(len(hv)), "seconds": round(float(np.median(times)), 4),
+            "status": r["status"], "iterations": int(r["iterations"]), "algorithm": r["algorithm"],
+            "max_abs_err_vs_planted_x": float(np.abs(r["primal"] - inst["optimum"]).max())}
+
+
 def cpu_baseline(inst, iterations_needed, threads):
     """The reference's own CPU path (oracle/_ref; the oracle port if it is absent)
     on the same instance: one run that stops after setup and the iteration-0
@@ -505,6 +524,8 @@ def main():
                                            "achieved": round(b64 / (u64 * 1e-6) / 1e9, 1)}
 
     pcie = host_inclusive(head, inst, L, args.check_every, args.polish_every)
+    whole = whole_model(head, inst, args.check_every, args.polish_every)
+    print(f"# whole-model entry: {whole}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -523,7 +544,8 @@ def main():
                    "mode": head, "seed": args.seed, "check_every": args.check_every,
                    "polish_every": args.polish_every},
         "p50_ms": round(hm["p50_ms"], 3), "p99_ms": round(hm["p99_ms"], 3),
-        "roofline": roofline, "cpu_baseline": cpu, "host_inclusive": pcie, "modes": modes, "device": device,
+        "roofline": roofline, "cpu_baseline": cpu, "host_inclusive": pcie, "whole_model": whole, "modes": modes,
+        "device": device,
     }
     print(json.dumps(line))
 

@@ -91,7 +91,7 @@ class QpOptions(C.Structure):
 
 
 class QpResult(C.Structure):
-    _fields_ = [("status", C.c_char * 32), ("algorithm", C.c_char * 32), ("objective", C.c_double),
+    _fields_ = [("status", C.c_char * 32), ("algorithm", C.c_char * 64), ("objective", C.c_double),
                 ("primal_residual", C.c_double), ("stationarity_residual", C.c_double),
                 ("complementarity_residual", C.c_double), ("dual_bound", C.c_double), ("relative_gap", C.c_double),
                 ("iterations", C.c_uint64), ("optimality_verified", C.c_int32), ("restarts", C.c_int32),

@@ -217,7 +217,7 @@ void hgqp_qp_options_default(hgqp_qp_options* options);
 
 typedef struct hgqp_qp_result {
     char status[32];          /* QpResult::status string                          */
-    char algorithm[32];       /* "hip_fista_box", "hip_fista_box_fp32_polish"      */
+    char algorithm[64];       /* "hip_fista_box", "hip_fista_box_fp32_polish"      */
     double objective;
     double primal_residual;
     double stationarity_residual;
