@@ -456,9 +456,16 @@ struct hgqp_dense {
     bf16* q16 = nullptr;
     DenseView v{};
     std::vector<Rhs> host_rhs;
+    // a block of check_every correction steps (GEMM, step, decisions) captured
+    // once per starting parity and replayed
+    hipGraphExec_t block[2] = {nullptr, nullptr};
+    int64_t block_limit = -1;
+    int block_steps = 0;
     ~hgqp_dense() {
         if (stream) {
             (void)hipStreamSynchronize(stream);
+            for (auto& g : block)
+                if (g) (void)hipGraphExecDestroy(g);
             (void)hipStreamDestroy(stream);
         }
     }
@@ -543,6 +550,8 @@ int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, co
         auto d = std::make_unique<hgqp_dense>();
         if (options) d->opt = *options;
         else hgqp_dense_options_default(&d->opt);
+        if (d->opt.check_every < 1 || d->opt.check_every > 1000 || d->opt.power_iterations < 0)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_dense_create: need 1 <= check_every <= 1000");
         d->n = n;
         d->k = k;
         d->np = int((n + kRowPad - 1) / kRowPad * kRowPad);
@@ -686,13 +695,34 @@ int hgqp_dense_solve(hgqp_dense* d, double tolerance, uint64_t iteration_limit, 
             if (!active) break;
             // FP32 / BF16 correction steps until every active RHS froze
             for (;;) {
-                for (int s = 0; s < d->opt.check_every; ++s) {
-                    launch_gemm_bf16(d, parity, v.splits);
-                    hipLaunchKernelGGL(dense_step, segs, dim3(256), 0, d->stream, v, parity);
-                    hipLaunchKernelGGL(dense_finalize_step, dim3(d->kp), dim3(64), 0, d->stream, v, parity, limit);
-                    parity ^= 1;
-                    report->bf16_gemms += 1;
+                const int steps = d->opt.check_every;
+                if (d->block_limit != limit || d->block_steps != steps) {
+                    for (auto& g : d->block)
+                        if (g) {
+                            HIPCHECK(hipGraphExecDestroy(g));
+                            g = nullptr;
+                        }
+                    d->block_limit = limit;
+                    d->block_steps = steps;
                 }
+                if (!d->block[parity]) {
+                    hipGraph_t graph;
+                    HIPCHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+                    int p = parity;
+                    for (int s = 0; s < steps; ++s) {
+                        launch_gemm_bf16(d, p, v.splits);
+                        hipLaunchKernelGGL(dense_step, segs, dim3(256), 0, d->stream, v, p);
+                        hipLaunchKernelGGL(dense_finalize_step, dim3(d->kp), dim3(64), 0, d->stream, v, p, limit);
+                        p ^= 1;
+                    }
+                    HIPCHECK(hipStreamEndCapture(d->stream, &graph));
+                    const hipError_t e = hipGraphInstantiate(&d->block[parity], graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    HIPCHECK(e);
+                }
+                HIPCHECK(hipGraphLaunch(d->block[parity], d->stream));
+                parity ^= steps & 1;
+                report->bf16_gemms += steps;
                 HIPCHECK(hipGetLastError());
                 read_rhs(d);
                 int running = 0;
