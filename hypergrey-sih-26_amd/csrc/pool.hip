@@ -758,6 +758,10 @@ struct hgqp_server {
                 const bool period = pool->resident > 0;
                 if (period) {
                     if (!pool->tuned) autotune(pool);
+                    // new instances take their first check now and step in this period
+                    if (!uploads.empty())
+                        launch_pool_checks(pool->v, pool->loads, int(std::min<int64_t>(pool->max_blocks, 1 << 30)),
+                                           pool->stream);
                     launch_periods(pool, 1);
                     HIPCHECK(hipMemcpyAsync(tables[cur], pool->v.inst, sizeof(PoolInst) * pool->opt.max_instances,
                                             hipMemcpyDeviceToHost, pool->stream));

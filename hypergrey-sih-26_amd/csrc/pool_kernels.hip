@@ -286,15 +286,11 @@ int grid_for(int max_blocks) {
     return g < kMaxGroups ? (g < 1 ? 1 : g) : kMaxGroups;
 }
 
+// The check block: verify list, apply, FP64 check, verdicts, re-anchoring,
+// next active list. Slots in INIT or VERIFY take part.
 template <int LOADS>
-void period_impl(const PoolView& v, int max_blocks, int steps, hipStream_t stream) {
+void checks_impl(const PoolView& v, int max_blocks, int parity, hipStream_t stream) {
     const dim3 grid(grid_for(max_blocks)), block(kGroup), slots(v.max_inst);
-    int parity = 0;
-    for (int k = 0; k < steps; ++k) {
-        hipLaunchKernelGGL(pool_step<LOADS>, grid, block, 0, stream, v, parity);
-        hipLaunchKernelGGL(pool_finalize_step, slots, block, 0, stream, v, parity);
-        parity ^= 1;
-    }
     hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 1);
     hipLaunchKernelGGL(pool_list_expand, slots, block, 0, stream, v, 1);
     hipLaunchKernelGGL(pool_apply, grid, block, 0, stream, v, parity);
@@ -305,12 +301,30 @@ void period_impl(const PoolView& v, int max_blocks, int steps, hipStream_t strea
     hipLaunchKernelGGL(pool_list_expand, slots, block, 0, stream, v, 0);
 }
 
+template <int LOADS>
+void period_impl(const PoolView& v, int max_blocks, int steps, hipStream_t stream) {
+    const dim3 grid(grid_for(max_blocks)), block(kGroup), slots(v.max_inst);
+    int parity = 0;
+    for (int k = 0; k < steps; ++k) {
+        hipLaunchKernelGGL(pool_step<LOADS>, grid, block, 0, stream, v, parity);
+        hipLaunchKernelGGL(pool_finalize_step, slots, block, 0, stream, v, parity);
+        parity ^= 1;
+    }
+    checks_impl<LOADS>(v, max_blocks, parity, stream);
+}
+
 }  // namespace
 
 void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps, hipStream_t stream) {
     if (loads == 4) period_impl<4>(v, max_blocks, steps, stream);
     else if (loads == 16) period_impl<16>(v, max_blocks, steps, stream);
     else period_impl<8>(v, max_blocks, steps, stream);
+}
+
+void launch_pool_checks(const PoolView& v, int loads, int max_blocks, hipStream_t stream) {
+    if (loads == 4) checks_impl<4>(v, max_blocks, 0, stream);
+    else if (loads == 16) checks_impl<16>(v, max_blocks, 0, stream);
+    else checks_impl<8>(v, max_blocks, 0, stream);
 }
 
 void launch_pool_lists(const PoolView& v, hipStream_t stream) {

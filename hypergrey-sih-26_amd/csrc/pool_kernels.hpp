@@ -85,6 +85,11 @@ struct PoolView {
 // `steps` must be even so every period starts alike (graph replay).
 void launch_pool_period(const PoolView& v, int loads, int max_blocks, int steps, hipStream_t stream);
 
+// The check block of a period alone, at parity 0 (between periods): newly
+// admitted slots (INIT) get x0, their first FP64 check and an anchor, and join
+// the active list before the next period's steps.
+void launch_pool_checks(const PoolView& v, int loads, int max_blocks, hipStream_t stream);
+
 // Builds the active list from the slot table (after host admission edits).
 void launch_pool_lists(const PoolView& v, hipStream_t stream);
 
