@@ -64,6 +64,17 @@ def test_qplib_single_engine_reproduces_reference_run(client):
     assert np.array_equal(np.array(got["primal"]), rec["primal"])
 
 
+def test_qplib_single_engine_fast_modes_verified(client):
+    text, rec = record_8792()
+    print("case: 8792 single fp32-refine restart jacobi", flush=True)
+    got = client.post("/qp/solve", json={"model": text, "engine": "single", "precision": "fp32-refine",
+                                         "momentum": "restart", "step": "jacobi", "tolerance": KKT_TOL}).json()
+    assert got["status"] == "optimal" and got["optimality_verified"]
+    assert got["algorithm"] == "hip_fista_box_fp32_refine_restart_jacobi"
+    assert got["stationarity_residual"] <= KKT_TOL
+    assert abs(got["objective"] - rec["objective"]) <= 1e-8 * (1 + abs(rec["objective"]))
+
+
 def test_qplib_served_engine_verified(client):
     text, rec = record_8792()
     print("case: 8792 served", flush=True)

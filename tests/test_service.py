@@ -42,10 +42,11 @@ class StandIn:
     def describe(self):
         return {"name": "stand-in"}
 
-    async def solve(self, problem, engine, tolerance, iteration_limit, time_limit, precision, momentum):
+    async def solve(self, problem, engine, tolerance, iteration_limit, time_limit, precision, momentum,
+                    step="scalar"):
         import asyncio
         self.calls.append(dict(problem=problem, engine=engine, tolerance=tolerance, iteration_limit=iteration_limit,
-                               time_limit=time_limit, precision=precision, momentum=momentum))
+                               time_limit=time_limit, precision=precision, momentum=momentum, step=step))
         self.entered.set()
         if self.hold is not None:
             while not self.hold.is_set():
@@ -104,11 +105,12 @@ def test_triplets_reach_engine_and_record_is_json_safe(client, engine):
 def test_qplib_text_is_parsed_by_the_library(client, engine):
     text = qplib("8991")
     r = client.post("/qp/solve", json={"model": text, "engine": "single", "precision": "fp32-refine",
-                                       "time_limit": 5})
+                                       "step": "jacobi", "time_limit": 5})
     assert r.status_code == 200 and r.json()["status"] == "optimal"
     call = engine.calls[-1]
     assert call["problem"]["n"] == 14400 and call["problem"]["rows"] == 0
     assert call["engine"] == "single" and call["precision"] == "fp32-refine" and call["time_limit"] == 5
+    assert call["step"] == "jacobi"
 
 
 @pytest.mark.parametrize("body,code", [
@@ -122,6 +124,7 @@ def test_qplib_text_is_parsed_by_the_library(client, engine):
     ({"problem": tiny_problem(), "time_limit": -1}, 422),
     ({"problem": tiny_problem(), "engine": "cpu"}, 422),
     ({"problem": tiny_problem(), "precision": "fp16"}, 422),
+    ({"problem": tiny_problem(), "step": "newton"}, 422),
 ])
 def test_rejects_invalid_requests(client, engine, body, code):
     assert client.post("/qp/solve", json=body).status_code == code
