@@ -32,16 +32,62 @@ namespace hgqp {
 namespace {
 using namespace dev;
 
-// Calls epi(row, sum) for each row of block b of m.
+// Calls epi(row, sum) for each row of block b of m, continuing from carry.
 template <class T, bool ORDERED, class Epi>
 __device__ inline void row_block(const Matrix<T>& m, const T* __restrict__ gat, int b, T* __restrict__ lds, int lane,
-                                 Epi& epi) {
-    dev::row_block<T, ORDERED, kLoads>(m.start, m.col, m.val, gat, m.blk[b], m.blk[b + 1], lds, lane, epi);
+                                 Epi& epi, const T* carry) {
+    dev::row_block<T, ORDERED, kLoads>(m.start, m.col, m.val, gat, m.blk[b], m.blk[b + 1], lds, lane, epi, carry);
+}
+
+// ---- a column-panel pass before the last: running row sums only ----
+template <class T>
+struct CarryEpilogue {
+    T* out;
+    __device__ inline void operator()(int r, T s) { out[r] = s; }
+};
+
+// What a panel pass gathers and when it has nothing to do: `x` always (checks,
+// immediate steps; ctrl may be null), or, with pick, the launch's step point
+// (x after a restart, else w); with skip, nothing once ctrl->done is set.
+template <class T>
+struct PassSource {
+    const T* w;
+    const T* x;
+    const Control* ctrl;
+    int parity;
+    int pick;
+    int skip;
+};
+
+template <class T, bool ORDERED>
+__global__ __launch_bounds__(kGroup) void carry_kernel(Matrix<T> m, PassSource<T> src, const T* carry_in,
+                                                       T* carry_out) {
+    __shared__ T stage[kWavesPerGroup][kCap];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (src.skip && src.ctrl->done) return;  // grid-uniform
+    const T* gat = src.pick && src.ctrl->use_x[src.parity] ? src.x : src.w;
+    CarryEpilogue<T> epi{carry_out};
+    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+        row_block<T, ORDERED>(m, gat, b, stage[wave], lane, epi, carry_in);
+}
+
+// Launches the passes before the last of m; returns the carry the last pass
+// continues from (null for a one-panel matrix).
+template <class T>
+const T* launch_carry(const PanelMatrix<T>& m, const PassSource<T>& src, bool ordered, hipStream_t stream) {
+    for (int p = 0; p + 1 < m.count; ++p) {
+        const T* in = p ? m.carry : nullptr;
+        dim3 grid(groups_for(m.pass[p].nblocks)), block(kGroup);
+        if (ordered) hipLaunchKernelGGL((carry_kernel<T, true>), grid, block, 0, stream, m.pass[p], src, in, m.carry);
+        else hipLaunchKernelGGL((carry_kernel<T, false>), grid, block, 0, stream, m.pass[p], src, in, m.carry);
+    }
+    return m.count > 1 ? m.carry : nullptr;
 }
 
 template <class T, bool ORDERED, int MOM>
 __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T> bufs, Control* ctrl,
-                                                      double* partials, int parity, double beta_imm) {
+                                                      double* partials, int parity, double beta_imm,
+                                                      const T* carry) {
     __shared__ T stage[kWavesPerGroup][kCap];
     __shared__ double wave_dot[kWavesPerGroup];
     __shared__ double wave_res[kWavesPerGroup];
@@ -67,7 +113,7 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     epi.dot = 0;
     epi.resid = 0;
     for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
-        row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi);
+        row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi, carry);
     if (MOM == kMomentumReference) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctrl->t[parity ^ 1] = next_t;
@@ -160,7 +206,8 @@ __global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const 
 
 template <bool ORDERED>
 __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuffers<double> bufs, Control* ctrl,
-                                                       double* partials, int parity, CheckParams prm) {
+                                                       double* partials, int parity, CheckParams prm,
+                                                       const double* carry) {
     __shared__ double stage[kWavesPerGroup][kCap];
     __shared__ double red[kWavesPerGroup][kPartial];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -184,7 +231,7 @@ __global__ __launch_bounds__(kGroup) void check_kernel(Matrix<double> m, StepBuf
             for (int r = m.blk[b] + lane; r < m.blk[b + 1]; r += kWave) epi(r, 0.0);
     } else {
         for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
-            row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+            row_block<double, ORDERED>(m, epi.x, b, stage[wave], lane, epi, carry);
     }
     // workgroup partial
     {
@@ -303,14 +350,15 @@ struct Check32Epilogue {
 
 template <bool ORDERED>
 __global__ __launch_bounds__(kGroup) void check32_kernel(Matrix<float> m, StepBuffers<float> bufs, Control* ctrl,
-                                                         double* partials, int parity, CheckParams prm) {
+                                                         double* partials, int parity, CheckParams prm,
+                                                         const float* carry) {
     __shared__ float stage[kWavesPerGroup][kCap];
     __shared__ double red[kWavesPerGroup];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (ctrl->done) return;
     Check32Epilogue epi{bufs.x[parity], bufs.c, bufs.lo, bufs.up, 0.0};
     for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
-        row_block<float, ORDERED>(m, epi.x, b, stage[wave], lane, epi);
+        row_block<float, ORDERED>(m, epi.x, b, stage[wave], lane, epi, carry);
     double s = wave_max(epi.stat);
     if (lane == 0) red[wave] = s;
     __syncthreads();
@@ -436,10 +484,14 @@ int groups_for(int nblocks) {
 }
 
 template <class T>
-void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
+void launch_step(const PanelMatrix<T>& pm, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
                  int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat) {
+    const bool dev = momentum != kMomentumImmediate;
+    const T* carry = launch_carry(pm, PassSource<T>{b.w[parity], b.x[parity], ctrl, parity, dev, dev}, ordered,
+                                  stream);
+    const Matrix<T>& m = pm.pass[pm.count - 1];
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
-#define HGQP_STEP(ORD, MOM) hipLaunchKernelGGL((step_kernel<T, ORD, MOM>), grid, block, 0, stream, m, b, ctrl, partials, parity, beta)
+#define HGQP_STEP(ORD, MOM) hipLaunchKernelGGL((step_kernel<T, ORD, MOM>), grid, block, 0, stream, m, b, ctrl, partials, parity, beta, carry)
     if (ordered) {
         if (momentum == kMomentumImmediate) HGQP_STEP(true, kMomentumImmediate);
         else if (momentum == kMomentumReference) HGQP_STEP(true, kMomentumReference);
@@ -455,24 +507,36 @@ void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, dou
                            switch_stat);
 }
 
-template void launch_step<double>(const Matrix<double>&, const StepBuffers<double>&, Control*, double*, int, int,
-                                  bool, double, hipStream_t, double);
-template void launch_step<float>(const Matrix<float>&, const StepBuffers<float>&, Control*, double*, int, int, bool,
-                                 double, hipStream_t, double);
+template void launch_step<double>(const PanelMatrix<double>&, const StepBuffers<double>&, Control*, double*, int,
+                                  int, bool, double, hipStream_t, double);
+template void launch_step<float>(const PanelMatrix<float>&, const StepBuffers<float>&, Control*, double*, int, int,
+                                 bool, double, hipStream_t, double);
 
-void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
+void launch_check(const PanelMatrix<double>& pm, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream) {
+    // at x = 0 the check reads no matrix entries, so the panel passes are skipped
+    const double* carry = params.zero_x ? nullptr
+        : launch_carry(pm, PassSource<double>{b.x[parity], b.x[parity], ctrl, parity, 0, params.set_done}, ordered,
+                       stream);
+    const Matrix<double>& m = pm.pass[pm.count - 1];
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
-    if (ordered) hipLaunchKernelGGL(check_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
-    else hipLaunchKernelGGL(check_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+    if (ordered)
+        hipLaunchKernelGGL(check_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params, carry);
+    else
+        hipLaunchKernelGGL(check_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params, carry);
     hipLaunchKernelGGL(finalize_check, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity, params);
 }
 
-void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
+void launch_check32(const PanelMatrix<float>& pm, const StepBuffers<float>& b, Control* ctrl, double* partials,
                     int parity, bool ordered, const CheckParams& params, hipStream_t stream) {
+    const float* carry =
+        launch_carry(pm, PassSource<float>{b.x[parity], b.x[parity], ctrl, parity, 0, 1}, ordered, stream);
+    const Matrix<float>& m = pm.pass[pm.count - 1];
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
-    if (ordered) hipLaunchKernelGGL(check32_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
-    else hipLaunchKernelGGL(check32_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params);
+    if (ordered)
+        hipLaunchKernelGGL(check32_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params, carry);
+    else
+        hipLaunchKernelGGL(check32_kernel<false>, grid, block, 0, stream, m, b, ctrl, partials, parity, params, carry);
     hipLaunchKernelGGL(finalize_check32, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity, params);
 }
 

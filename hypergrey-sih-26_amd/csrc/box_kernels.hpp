@@ -60,6 +60,21 @@ struct Matrix {
     const T* val;        // [nnz]
 };
 
+// A single instance's matrix as column panels: pass p holds, for every row, the
+// row's entries whose columns lie in panel p (a CSR of its own, columns in
+// stored order). A sparse launch runs the passes in panel order; the passes
+// before the last write each row's running sum to `carry`, the last continues
+// from it and applies the epilogue. One panel is the plain CSR. Panels keep the
+// gathered slice of the input vector resident in every XCD's L2.
+constexpr int kMaxPanels = 16;
+
+template <class T>
+struct PanelMatrix {
+    int count = 1;
+    Matrix<T> pass[kMaxPanels];
+    T* carry = nullptr;  // [n] running row sums between passes (count > 1)
+};
+
 template <class T>
 struct StepBuffers {
     const T* c;
@@ -94,13 +109,13 @@ void launch_row_abs(int64_t n, const int* start, const double* val, double fallb
                     hipStream_t stream);
 
 template <class T>
-void launch_step(const Matrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
+void launch_step(const PanelMatrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
                  int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat = 0.0);
 
-void launch_check(const Matrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
+void launch_check(const PanelMatrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream);
 
-void launch_check32(const Matrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
+void launch_check32(const PanelMatrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
                     int parity, bool ordered, const CheckParams& params, hipStream_t stream);
 
 // FP32 correction cycle anchored at xa with FP64 gradient g (see anchor_kernel)

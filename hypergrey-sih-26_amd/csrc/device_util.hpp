@@ -37,4 +37,20 @@ struct Arena {
     }
 };
 
+// A device allocation owned by a scope (temporaries of handle creation).
+template <class T>
+struct DeviceTemp {
+    T* p = nullptr;
+    DeviceTemp() = default;
+    explicit DeviceTemp(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        if (count) HIPCHECK(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    }
+    ~DeviceTemp() {
+        if (p) (void)hipFree(p);
+    }
+    DeviceTemp(const DeviceTemp&) = delete;
+    DeviceTemp& operator=(const DeviceTemp&) = delete;
+};
+
 }  // namespace hgqp

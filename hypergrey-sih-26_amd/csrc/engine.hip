@@ -4,6 +4,7 @@
 #include "hgqp.h"
 #include "internal.hpp"
 #include "device_util.hpp"
+#include "panels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -71,17 +72,27 @@ using namespace hgqp;
 
 struct hgqp_box {
     int64_t n = 0, nnz = 0;
-    int nblocks = 0, groups = 0;
+    int nblocks = 0, groups = 0;  // row blocks over the FP64 panels; workgroups of the widest launch
     hgqp_box_options opt{};
     int diagonal = 0;
     double lipschitz = 0;
     hipStream_t stream = nullptr;
     Arena arena;
-    int* blk = nullptr;
-    int* start = nullptr;
-    int* col = nullptr;
+    Arena blocks;  // row blocks of every panel and the launch partials
+    // The matrix as column panels (one panel: the plain CSR). The FP32 values
+    // share the FP64 layout when the panel counts agree, else have their own.
+    struct Layout {
+        int count = 1;
+        int* start = nullptr;  // [count (n + 1)]
+        int* col = nullptr;    // [nnz]
+        int nblocks[kMaxPanels] = {};
+        int* blk[kMaxPanels] = {};
+    };
+    Layout l64, l32;
     double* val = nullptr;
     float* val32 = nullptr;
+    double* carry64 = nullptr;  // running row sums between panel passes
+    float* carry32 = nullptr;
     double *c = nullptr, *lo = nullptr, *up = nullptr;
     double* g = nullptr;  // FP64 gradient of the last check (FP32_REFINE)
     float *c32 = nullptr, *lo32 = nullptr, *up32 = nullptr;
@@ -108,8 +119,17 @@ struct hgqp_box {
     std::vector<Graph> graphs;
     std::vector<hipEvent_t> events;  // timing pairs
 
-    Matrix<double> m64() const { return Matrix<double>{nblocks, blk, start, col, val}; }
-    Matrix<float> m32() const { return Matrix<float>{nblocks, blk, start, col, val32}; }
+    template <class T>
+    PanelMatrix<T> panels(const Layout& l, const T* v, T* carry) const {
+        PanelMatrix<T> m;
+        m.count = l.count;
+        m.carry = carry;
+        for (int p = 0; p < l.count; ++p)
+            m.pass[p] = Matrix<T>{l.nblocks[p], l.blk[p], l.start + size_t(p) * size_t(n + 1), l.col, v};
+        return m;
+    }
+    PanelMatrix<double> m64() const { return panels<double>(l64, val, carry64); }
+    PanelMatrix<float> m32() const { return panels<float>(l32, val32, carry32); }
     StepBuffers<double> b64() const {
         return StepBuffers<double>{c, lo, up, {x[0], x[1]}, {w[0], w[1]}, lipschitz, dstep};
     }
@@ -344,6 +364,7 @@ void hgqp_box_options_default(hgqp_box_options* o) {
     o->offset = 0;
     o->diagonal_hessian = -1;
     o->step_metric = HGQP_STEP_SCALAR;
+    o->column_panels = 0;
 }
 
 void hgqp_solve_options_default(hgqp_solve_options* o) {
@@ -387,7 +408,6 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     const int32_t* start = src.start;
     const int32_t* index = src.index;
     const double* value = src.value;
-    const hipMemcpyKind kind = src.dev_start ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     require_device();
     hgqp_box_options opt;
     if (options) opt = *options;
@@ -401,6 +421,8 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown reduction order");
     if (opt.step_metric != HGQP_STEP_SCALAR && opt.step_metric != HGQP_STEP_JACOBI)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
+    if (opt.column_panels < 0 || opt.column_panels > kMaxPanels)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column_panels must be in [0, 16]");
 
     auto b = std::make_unique<hgqp_box>();
     b->n = n;
@@ -411,24 +433,57 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     b->zero_start = true;
     for (int64_t j = 0; j < n && b->zero_start; ++j) b->zero_start = lower[j] <= 0 && 0 <= upper[j];
     b->at_zero = b->zero_start;
-    std::vector<int> blk = row_blocks(n, start, kCap, kMaxRows);
-    b->nblocks = static_cast<int>(blk.size()) - 1;
-    b->groups = groups_for(b->nblocks);
     const bool f32 = opt.precision != HGQP_PRECISION_FP64;
     const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
-    const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
-    size_t bytes = Arena::span(blk.size(), 4) + Arena::span(un + 1, 4) + Arena::span(unnz, 4) +
-                   Arena::span(unnz, 8) + 3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) +
-                   Arena::span(1, sizeof(Control)) + Arena::span(size_t(b->groups) * 10 + 16, 8);
-    if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
-    if (refine) bytes += Arena::span(un, 8);
     const bool jacobi = opt.step_metric == HGQP_STEP_JACOBI;
-    if (jacobi) bytes += Arena::span(un, 8) + (f32 ? Arena::span(un, 4) : 0);
+    const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
     HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+
+    // the plain CSR on the device: the ingest's arrays, or an upload
+    DeviceTemp<int> tstart, tcol;
+    DeviceTemp<double> tval;
+    const int* dstart = src.dev_start;
+    const int* dcol = index;
+    const double* dval = value;
+    if (!src.dev_start) {
+        tstart.alloc(un + 1);
+        tcol.alloc(unnz);
+        tval.alloc(unnz);
+        HIPCHECK(hipMemcpyAsync(tstart.p, start, (un + 1) * 4, hipMemcpyHostToDevice, b->stream));
+        if (nnz) {
+            HIPCHECK(hipMemcpyAsync(tcol.p, index, size_t(nnz) * 4, hipMemcpyHostToDevice, b->stream));
+            HIPCHECK(hipMemcpyAsync(tval.p, value, size_t(nnz) * 8, hipMemcpyHostToDevice, b->stream));
+        }
+        dstart = tstart.p;
+        dcol = tcol.p;
+        dval = tval.p;
+    }
+    // Column panels for a matrix that gathers far from its diagonal. A diagonal
+    // Hessian keeps one panel: its separable bound reads each row's entry.
+    int p64 = 1, p32 = 1;
+    if (!b->diagonal && nnz > 0 && opt.column_panels != 1) {
+        const PanelSurvey sv = survey_panels(n, dstart, dcol, b->stream);
+        if (opt.column_panels == 0) {
+            p64 = panels_for(n, sv, 8);
+            p32 = f32 ? panels_for(n, sv, 4) : 1;
+        } else if (sv.sorted && int64_t(opt.column_panels) * n + 1 < (int64_t(1) << 31)) {
+            p64 = p32 = opt.column_panels;
+        }
+    }
+    const bool own32 = f32 && p32 != p64;
+
+    size_t bytes = Arena::span(size_t(p64) * (un + 1), 4) + Arena::span(unnz, 4) + Arena::span(unnz, 8) +
+                   3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) + Arena::span(1, sizeof(Control));
+    if (p64 > 1) bytes += Arena::span(un, 8);
+    if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
+    if (own32) bytes += Arena::span(size_t(p32) * (un + 1), 4) + Arena::span(unnz, 4);
+    if (f32 && p32 > 1) bytes += Arena::span(un, 4);
+    if (refine) bytes += Arena::span(un, 8);
+    if (jacobi) bytes += Arena::span(un, 8) + (f32 ? Arena::span(un, 4) : 0);
     b->arena.reserve(bytes);
-    b->blk = b->arena.take<int>(blk.size());
-    b->start = b->arena.take<int>(un + 1);
-    b->col = b->arena.take<int>(unnz);
+    b->l64.count = p64;
+    b->l64.start = b->arena.take<int>(size_t(p64) * (un + 1));
+    b->l64.col = b->arena.take<int>(unnz);
     b->val = b->arena.take<double>(unnz);
     b->c = b->arena.take<double>(un);
     b->lo = b->arena.take<double>(un);
@@ -438,7 +493,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         b->w[p] = b->arena.take<double>(un);
     }
     b->ctrl = b->arena.take<Control>(1);
-    b->partials = b->arena.take<double>(size_t(b->groups) * 10 + 16);
+    if (p64 > 1) b->carry64 = b->arena.take<double>(un);
     if (f32) {
         b->val32 = b->arena.take<float>(unnz);
         b->c32 = b->arena.take<float>(un);
@@ -448,6 +503,15 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
             b->x32[p] = b->arena.take<float>(un);
             b->w32[p] = b->arena.take<float>(un);
         }
+        b->l32.count = p32;
+        if (own32) {
+            b->l32.start = b->arena.take<int>(size_t(p32) * (un + 1));
+            b->l32.col = b->arena.take<int>(unnz);
+        } else {
+            b->l32.start = b->l64.start;
+            b->l32.col = b->l64.col;
+        }
+        if (p32 > 1) b->carry32 = b->arena.take<float>(un);
     }
     if (refine) b->g = b->arena.take<double>(un);
     if (jacobi) {
@@ -457,25 +521,77 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&b->host_slots), sizeof(Control) * hgqp_box::kSlots,
                            hipHostMallocDefault));
     for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    HIPCHECK(hipMemcpyAsync(b->blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, b->stream));
-    HIPCHECK(hipMemcpyAsync(b->start, src.dev_start ? src.dev_start : start, (un + 1) * 4, kind, b->stream));
-    if (nnz) {
-        HIPCHECK(hipMemcpyAsync(b->col, index, size_t(nnz) * 4, kind, b->stream));
-        HIPCHECK(hipMemcpyAsync(b->val, value, size_t(nnz) * 8, kind, b->stream));
+
+    // the matrix in its layouts
+    if (p64 == 1) {
+        HIPCHECK(hipMemcpyAsync(b->l64.start, dstart, (un + 1) * 4, hipMemcpyDeviceToDevice, b->stream));
+        if (nnz) {
+            HIPCHECK(hipMemcpyAsync(b->l64.col, dcol, size_t(nnz) * 4, hipMemcpyDeviceToDevice, b->stream));
+            HIPCHECK(hipMemcpyAsync(b->val, dval, size_t(nnz) * 8, hipMemcpyDeviceToDevice, b->stream));
+        }
+    } else {
+        split_panels<double>(n, nnz, p64, dstart, dcol, dval, b->l64.start, b->l64.col, b->val, b->stream);
+    }
+    if (f32 && nnz) {
+        if (own32) split_panels<float>(n, nnz, p32, dstart, dcol, dval, b->l32.start, b->l32.col, b->val32, b->stream);
+        else launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
     }
     HIPCHECK(hipMemcpyAsync(b->c, cost, un * 8, hipMemcpyHostToDevice, b->stream));
     HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
     HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
-    if (jacobi) launch_row_abs(n, b->start, b->val, lipschitz, b->dstep, b->dstep32, b->stream);
+    if (jacobi) launch_row_abs(n, dstart, dval, lipschitz, b->dstep, b->dstep32, b->stream);  // stored order
     if (f32) {
-        if (nnz) launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
         launch_convert<float, double>(n, b->c, b->c32, b->stream);
         launch_convert<float, double>(n, b->lo, b->lo32, b->stream);
         launch_convert<float, double>(n, b->up, b->up32, b->stream);
     }
     launch_init<double>(n, b->lo, b->up, b->x[0], b->w[0], b->stream);
     HIPCHECK(hipGetLastError());
-    put_control(b.get(), fresh_control());  // synchronizes: the caller's arrays may be released
+
+    // row blocks of every panel, from the host copy of its starts
+    auto panel_blocks = [&](const hgqp_box::Layout& l) {
+        std::vector<std::vector<int>> out(l.count);
+        if (l.count == 1) {
+            out[0] = row_blocks(n, start, kCap, kMaxRows);
+            return out;
+        }
+        std::vector<int32_t> hs(size_t(l.count) * (un + 1));
+        HIPCHECK(hipMemcpyAsync(hs.data(), l.start, hs.size() * 4, hipMemcpyDeviceToHost, b->stream));
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        for (int p = 0; p < l.count; ++p) out[p] = row_blocks(n, hs.data() + size_t(p) * (un + 1), kCap, kMaxRows);
+        return out;
+    };
+    const auto blk64 = panel_blocks(b->l64);
+    const auto blk32 = own32 ? panel_blocks(b->l32) : std::vector<std::vector<int>>();
+    size_t bbytes = 0;
+    int groups = 1;
+    for (const auto* set : {&blk64, &blk32})
+        for (const auto& v : *set) {
+            bbytes += Arena::span(v.size(), 4);
+            groups = std::max(groups, groups_for(int(v.size()) - 1));
+        }
+    b->groups = groups;
+    bbytes += Arena::span(size_t(groups) * 10 + 16, 8);
+    b->blocks.reserve(bbytes);
+    auto place = [&](hgqp_box::Layout& l, const std::vector<std::vector<int>>& v) {
+        for (int p = 0; p < l.count; ++p) {
+            l.nblocks[p] = int(v[p].size()) - 1;
+            l.blk[p] = b->blocks.take<int>(v[p].size());
+            HIPCHECK(hipMemcpyAsync(l.blk[p], v[p].data(), v[p].size() * 4, hipMemcpyHostToDevice, b->stream));
+        }
+    };
+    place(b->l64, blk64);
+    if (own32) place(b->l32, blk32);
+    else if (f32) {
+        for (int p = 0; p < p64; ++p) {
+            b->l32.nblocks[p] = b->l64.nblocks[p];
+            b->l32.blk[p] = b->l64.blk[p];
+        }
+    }
+    b->partials = b->blocks.take<double>(size_t(groups) * 10 + 16);
+    b->nblocks = 0;
+    for (int p = 0; p < p64; ++p) b->nblocks += b->l64.nblocks[p];
+    put_control(b.get(), fresh_control());  // synchronizes: the caller's arrays and the temporaries may be released
     return b.release();
 }
 
@@ -700,7 +816,9 @@ int hgqp_box_info_get(const hgqp_box* b, hgqp_box_info* info) {
         info->n = b->n;
         info->nnz = b->nnz;
         info->row_blocks = b->nblocks;
-        info->device_bytes = int64_t(b->arena.size);
+        info->device_bytes = int64_t(b->arena.size + b->blocks.size);
+        info->column_panels = b->l64.count;
+        info->column_panels32 = b->val32 ? b->l32.count : 0;
         info->lipschitz = b->lipschitz;
     });
 }

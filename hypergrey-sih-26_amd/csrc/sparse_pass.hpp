@@ -89,10 +89,15 @@ __device__ inline void momentum_of(double t, double* next_t, double* beta) {
 // Rows [r0, r1) whose entries start[r0]..start[r1] form one row block of at most
 // 64*LOADS entries, or a single longer row. Calls epi(row, sum) once per row
 // from the lane that owns the row; lds holds 64*LOADS values of this wave.
+// With `carry`, each row's sum continues from carry[row] (the running sum of
+// the row's entries in earlier column panels): ORDERED adds this pass's
+// products to it one by one, so a row split over panels in column order sums
+// exactly as one pass would; TREE adds its combined sum to it. `carry` may be
+// the array the epilogue writes: each row is read before its lane writes it.
 template <class T, bool ORDERED, int LOADS, class Epi>
 __device__ inline void row_block(const int* __restrict__ start, const int* __restrict__ col,
                                  const T* __restrict__ val, const T* __restrict__ gat, int r0, int r1,
-                                 T* __restrict__ lds, int lane, Epi& epi) {
+                                 T* __restrict__ lds, int lane, Epi& epi, const T* carry = nullptr) {
     constexpr int CAP = kWave * LOADS;
     const int k0 = start[r0], k1 = start[r1];
     if (k1 - k0 <= CAP) {
@@ -120,7 +125,7 @@ __device__ inline void row_block(const int* __restrict__ start, const int* __res
         if (ORDERED || nrows > kWave / 2) {
             for (int r = r0 + lane; r < r1; r += kWave) {
                 const int a = start[r] - k0, e = start[r + 1] - k0;
-                T s = 0;
+                T s = carry ? carry[r] : T(0);
                 int q = a;
                 // eight LDS reads in flight, then the adds in stored order
                 for (; q + 8 <= e; q += 8) {
@@ -145,12 +150,12 @@ __device__ inline void row_block(const int* __restrict__ start, const int* __res
                 for (int q = a + sub; q < e; q += lpr) s += lds[q];
             }
             for (int msk = lpr >> 1; msk >= 1; msk >>= 1) s += shfl_xor(s, msk);
-            if (sub == 0 && r < r1) epi(r, s);
+            if (sub == 0 && r < r1) epi(r, carry ? carry[r] + s : s);
         }
         wave_sync();
     } else {
         // one long row, streamed in CAP chunks
-        T s = 0;
+        T s = ORDERED && carry ? carry[r0] : T(0);
         for (int base = k0; base < k1; base += CAP) {
             if (ORDERED) {
                 const int last = k1 - 1;
@@ -191,6 +196,7 @@ __device__ inline void row_block(const int* __restrict__ start, const int* __res
         }
         if (!ORDERED) {
             for (int msk = 32; msk >= 1; msk >>= 1) s += shfl_xor(s, msk);
+            if (carry) s = carry[r0] + s;
         }
         if (lane == 0) epi(r0, s);
     }

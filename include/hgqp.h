@@ -87,6 +87,14 @@ typedef struct hgqp_box_options {
                                 separable dual bound of qp_admm.cpp:106-112), 0 if
                                 not, -1 derive from the stored matrix              */
     int32_t step_metric;     /* HGQP_STEP_* (default SCALAR)                      */
+    int32_t column_panels;   /* sparse passes split by column range: 0 (default)
+                                chooses from the matrix (about 2 MiB of the
+                                gathered vector per panel, for matrices that
+                                gather far from the diagonal), 1..16 forces that
+                                many for both precisions. FP64 ORDERED results do
+                                not depend on it: each row still sums its entries
+                                in stored order. Rows must have ascending columns
+                                for more than one; otherwise 1 is used.           */
 } hgqp_box_options;
 
 void hgqp_box_options_default(hgqp_box_options* options);
@@ -175,11 +183,15 @@ int hgqp_box_reset(hgqp_box* box);
 /* Replaces the destructor of CudaBox (cuda_box.cu:13). */
 int hgqp_box_destroy(hgqp_box* box);
 
-/* Static facts of a handle: rows, stored entries, row blocks, device bytes. */
+/* Static facts of a handle: rows, stored entries, row blocks (over the FP64
+ * panels), device bytes, and the column panels of the FP64 and FP32 passes
+ * (0 for FP32 when the handle has no FP32 phase). */
 typedef struct hgqp_box_info {
     int64_t n, nnz, row_blocks;
     int64_t device_bytes;
     double lipschitz;
+    int32_t column_panels;
+    int32_t column_panels32;
 } hgqp_box_info;
 int hgqp_box_info_get(const hgqp_box* box, hgqp_box_info* info);
 

@@ -76,13 +76,17 @@ def test_check_matches_oracle(hg, oracle, case):
     assert abs(dev["relative_gap"] - ref["relative_gap"]) <= 1e-12
 
 
-def test_resident_solve_bit_exact(hg, oracle):
-    """Device-resident loop with the reference schedule equals the CPU path bit for bit."""
+@pytest.mark.parametrize("panels", [1, 4])
+def test_resident_solve_bit_exact(hg, oracle, panels):
+    """Device-resident loop with the reference schedule equals the CPU path bit for
+    bit, with the sparse passes in one panel or split into column panels."""
     q, d = planted_qp(hg, 10000, 8, 1)
     s, i, v, c, lo, up, L = engine_inputs(oracle, q)
     ref = oracle.solve_box(q, tolerance=1e-7, iteration_limit=20000, time_limit=120)
     for graph in (1, 0):
-        with hg.Box(s, i, v, c, lo, up, L, use_graph=graph) as box:
+        print(f"panels={panels} graph={graph}", flush=True)
+        with hg.Box(s, i, v, c, lo, up, L, use_graph=graph, column_panels=panels) as box:
+            assert box.info()["column_panels"] == panels
             rep = box.solve(tolerance=1e-7, iteration_limit=20000, time_limit=120)
             x = box.primal()
         print(rep, flush=True)
@@ -106,17 +110,20 @@ def test_tridiagonal_golden(hg):
     assert abs(np.abs(r["primal"] - d["optimum"]).max() - 5.334931689526812e-06) <= 1e-12
 
 
-@pytest.mark.parametrize("momentum,precision,metric", [(1, 0, 0), (1, 1, 0), (0, 1, 0), (1, 2, 0), (0, 2, 0),
-                                                        (1, 0, 1), (1, 1, 1), (1, 2, 1), (0, 2, 1)])
-def test_fast_modes_meet_contract(hg, oracle, momentum, precision, metric):
+@pytest.mark.parametrize("momentum,precision,metric,panels", [
+    (1, 0, 0, 0), (1, 1, 0, 0), (0, 1, 0, 0), (1, 2, 0, 0), (0, 2, 0, 0),
+    (1, 0, 1, 0), (1, 1, 1, 0), (1, 2, 1, 0), (0, 2, 1, 0),
+    (1, 0, 1, 3), (1, 1, 1, 3), (1, 2, 1, 3), (1, 2, 0, 5)])
+def test_fast_modes_meet_contract(hg, oracle, momentum, precision, metric, panels):
     """Restart, FP32 with FP64 polish, FP32 correction cycles anchored at FP64,
-    each with the scalar 1/L step or the per-row Jacobi step: KKT <= 1e-6 by the
-    oracle's own check, objective within 1e-8 of the FP64 oracle, x* recovered."""
+    each with the scalar 1/L step or the per-row Jacobi step, in one panel or
+    column panels: KKT <= 1e-6 by the oracle's own check, objective within 1e-8
+    of the FP64 oracle, x* recovered."""
     q, d = planted_qp(hg, 10000, 8, 3)
     s, i, v, c, lo, up, L = engine_inputs(oracle, q)
     ref = oracle.solve_box(q, tolerance=1e-9, iteration_limit=20000, time_limit=120)
     with hg.Box(s, i, v, c, lo, up, L, momentum=momentum, precision=precision, reduce=hg.REDUCE_TREE,
-                step_metric=metric) as box:
+                step_metric=metric, column_panels=panels) as box:
         rep = box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=120)
         x = box.primal()
     print(rep, flush=True)
@@ -149,3 +156,71 @@ def test_jacobi_step_is_the_row_scaled_projection(hg, oracle):
     for got, expect in ((x1, step(x0)), (x2, step(x1))):
         assert np.abs(got - expect).max() <= 1e-13 * (1 + np.abs(expect).max())
     assert np.abs(x2 - x1).max() > 1e-3  # the gradient moved the iterate
+
+
+@pytest.mark.parametrize("panels", [2, 3, 16])
+@pytest.mark.parametrize("case", ["tridiagonal", "planted", "long_rows"])
+def test_column_panels_bit_exact(hg, oracle, case, panels):
+    """Sparse passes split into column panels continue each row's running sum
+    from the earlier panels, so FP64 ORDERED steps equal the CPU path bit for bit
+    for any panel count (rows longer than a wave's stage, empty rows and rows
+    with no entries in a panel included); the TREE reduction stays within
+    1e-12; the check equals the one-panel check (stationarity bit for bit)."""
+    q = cases(hg)[case]
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    betas = oracle.betas(40)
+    x0 = np.clip(np.zeros(q.n), lo, up)
+    x_cpu, _ = oracle.fista_steps(s, i, v, c, lo, up, L, betas, x0, x0)
+    print(f"case={case} panels={panels} n={q.n} nnz={len(v)}", flush=True)
+    with hg.Box(s, i, v, c, lo, up, L, column_panels=panels) as box:
+        assert box.info()["column_panels"] == panels
+        for b in betas:
+            box.step(b)
+        x_gpu = box.primal()
+        dev = box.check(1e-7)
+    assert np.array_equal(x_gpu, x_cpu), np.abs(x_gpu - x_cpu).max()
+    with hg.Box(s, i, v, c, lo, up, L) as box:
+        assert box.info()["column_panels"] == 1
+        for b in betas:
+            box.step(b)
+        one = box.check(1e-7)
+    assert dev["stationarity_residual"] == one["stationarity_residual"]
+    assert dev["optimality_verified"] == one["optimality_verified"]
+    for k in ("objective", "dual_bound"):
+        assert abs(dev[k] - one[k]) <= 1e-15 * (1 + abs(one[k]))
+    with hg.Box(s, i, v, c, lo, up, L, column_panels=panels, reduce=hg.REDUCE_TREE) as box:
+        for b in betas:
+            box.step(b)
+        x_tree = box.primal()
+    assert np.abs(x_tree - x_cpu).max() <= 1e-12 * (1 + np.abs(x_cpu).max())
+
+
+def test_full_size_panels_chosen_and_bit_exact(hg, oracle):
+    """Planted n = 1e6, 16 draws (33M stored entries): the handle splits its
+    passes into 4 FP64 and 2 FP32 column panels by itself; FP64 steps still equal
+    the CPU path bit for bit; the FP32 correction-cycle solve with restart and
+    the Jacobi step is verified and recovers x*."""
+    n = 1000000
+    d = hg.gen_planted(n, 16, 2026)
+    s, i, v, c, lo, up = d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"]
+    L = hg.lipschitz(s, v)
+    betas = oracle.betas(6)
+    x0 = np.clip(np.zeros(n), lo, up)
+    x_cpu, _ = oracle.fista_steps(s, i, v, c, lo, up, L, betas, x0, x0, threads=8)
+    print("case: n=1e6 FP64 steps with automatic panels", flush=True)
+    with hg.Box(s, i, v, c, lo, up, L) as box:
+        assert box.info()["column_panels"] == 4
+        for b in betas:
+            box.step(b)
+        assert np.array_equal(box.primal(), x_cpu)
+    print("case: n=1e6 FP32 refine restart Jacobi solve", flush=True)
+    with hg.Box(s, i, v, c, lo, up, L, precision=hg.PRECISION_FP32_REFINE, momentum=hg.MOMENTUM_RESTART,
+                reduce=hg.REDUCE_TREE, step_metric=hg.STEP_JACOBI) as box:
+        info = box.info()
+        assert info["column_panels"] == 4 and info["column_panels32"] == 2
+        rep = box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=120)
+        x = box.primal()
+    print(rep, flush=True)
+    assert rep["status"] == "optimal" and rep["optimality_verified"]
+    assert rep["stationarity_residual"] <= KKT_TOL
+    assert np.abs(x - d["optimum"]).max() < 1e-4
