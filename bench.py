@@ -53,18 +53,20 @@ def step_bytes(n, nnz, value_bytes, vector_bytes, jacobi=False):
     return nnz * (value_bytes + 4) + 4 * (n + 1) + (8 if jacobi else 7) * n * vector_bytes
 
 
-def measured_traffic(use32, mode, n, draws, nnz):
-    """HBM bytes per launch of the FP32 step from the committed rocprofv3 PMC summary
-    (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on plain streaming kernels), when this
-    run's mode, kernel and instance are the ones profiled; otherwise None."""
+def measured_traffic(use32, mode, n, draws, nnz, panels):
+    """HBM bytes per FP32 step (all of its column-panel launches) from the
+    committed rocprofv3 PMC summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on
+    plain streaming kernels), when this run's mode, instance and panel count are
+    the ones profiled; otherwise None."""
     path = os.path.join(ROOT, "profiles", "traffic_planted_n1e6.json")
     if not use32 or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or t.get("mode") != mode:
+    if (t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or t.get("mode") != mode
+            or t.get("column_panels32") != panels):
         return None
-    return t["traffic_bytes_per_launch"]
+    return t["traffic_bytes_per_step"]
 
 
 def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
@@ -76,6 +78,7 @@ def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
     box = hgqp.Box(inst["start"], inst["index"], inst["value"], inst["cost"], inst["lower"], inst["upper"], L,
                    precision=precision, momentum=momentum, reduce=reduce, step_metric=metric, timing=0, use_graph=1)
     try:
+        info = box.info()
         for _ in range(warmup):
             box.reset()
             box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60, **cadence)
@@ -102,7 +105,8 @@ def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
             treps.append(timed.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60, **cadence))
     finally:
         timed.close()
-    return dict(name=name, wall=wall, lat=lat, reps=reps, timed=treps, x=x)
+    return dict(name=name, wall=wall, lat=lat, reps=reps, timed=treps, x=x,
+                panels=(info["column_panels"], info["column_panels32"]))
 
 
 def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
@@ -497,6 +501,7 @@ def main():
             "max_abs_err_vs_planted_x": float(np.abs(r["x"] - inst["optimum"]).max()),
             "fp64_step_us": f64_ms / f64_n * 1e3 if f64_n else None,
             "fp32_step_us": f32_ms / f32_n * 1e3 if f32_n else None,
+            "column_panels": r["panels"][0], "column_panels32": r["panels"][1],
         }
         if ref_x is not None:
             m["max_abs_err_vs_fp64_reference_x"] = float(np.abs(r["x"] - ref_x).max())
@@ -512,10 +517,15 @@ def main():
     us = hm["fp32_step_us"] if use32 else hm["fp64_step_us"]
     nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8, jacobi=MODES[head][3] == hgqp.STEP_JACOBI)
     achieved = nbytes / (us * 1e-6) / 1e9
+    panels = hm["column_panels32"] if use32 else hm["column_panels"]
+    tname = "float" if use32 else "double"
+    kernel = f"step_kernel<{tname}>" if panels == 1 else \
+        f"step_kernel<{tname}> + {panels - 1} carry_kernel<{tname}> (one step over {panels} column panels)"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(use32, head, n, args.draws, nnz),
-                "kernel": f"step_kernel<{'float' if use32 else 'double'}>", "algorithmic_bytes": nbytes,
-                "avg_launch_us": round(us, 2)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(use32, head, n, args.draws, nnz, panels),
+                "kernel": kernel, "algorithmic_bytes": nbytes, "avg_launch_us": round(us, 2),
+                "column_panels": panels}
     # the FP64 reference-schedule step beside it
     if "fp64_reference" in modes and modes["fp64_reference"]["fp64_step_us"]:
         b64 = step_bytes(n, nnz, 8, 8)
