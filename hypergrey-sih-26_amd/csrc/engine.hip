@@ -15,6 +15,7 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace hgqp {
@@ -548,21 +549,46 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     launch_init<double>(n, b->lo, b->up, b->x[0], b->w[0], b->stream);
     HIPCHECK(hipGetLastError());
 
-    // row blocks of every panel, from the host copy of its starts
-    auto panel_blocks = [&](const hgqp_box::Layout& l) {
-        std::vector<std::vector<int>> out(l.count);
-        if (l.count == 1) {
-            out[0] = row_blocks(n, start, kCap, kMaxRows);
-            return out;
-        }
-        std::vector<int32_t> hs(size_t(l.count) * (un + 1));
+    // row blocks of every panel, from the host copy of its starts; the greedy
+    // blocking is a sequential scan over n rows, so each panel gets a host thread
+    std::vector<int32_t> hs64, hs32;
+    auto fetch_starts = [&](const hgqp_box::Layout& l, std::vector<int32_t>& hs) {
+        if (l.count == 1) return;
+        hs.resize(size_t(l.count) * (un + 1));
         HIPCHECK(hipMemcpyAsync(hs.data(), l.start, hs.size() * 4, hipMemcpyDeviceToHost, b->stream));
-        HIPCHECK(hipStreamSynchronize(b->stream));
-        for (int p = 0; p < l.count; ++p) out[p] = row_blocks(n, hs.data() + size_t(p) * (un + 1), kCap, kMaxRows);
-        return out;
     };
-    const auto blk64 = panel_blocks(b->l64);
-    const auto blk32 = own32 ? panel_blocks(b->l32) : std::vector<std::vector<int>>();
+    fetch_starts(b->l64, hs64);
+    if (own32) fetch_starts(b->l32, hs32);
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    std::vector<std::vector<int>> blk64(b->l64.count), blk32(own32 ? b->l32.count : 0);
+    struct Job {
+        std::vector<int>* out;
+        const int32_t* start;
+    };
+    std::vector<Job> jobs;
+    for (int p = 0; p < b->l64.count; ++p)
+        jobs.push_back({&blk64[p], b->l64.count == 1 ? start : hs64.data() + size_t(p) * (un + 1)});
+    for (size_t p = 0; p < blk32.size(); ++p) jobs.push_back({&blk32[p], hs32.data() + p * (un + 1)});
+    std::vector<std::exception_ptr> failed(jobs.size());
+    {
+        std::vector<std::thread> workers;
+        for (size_t q = 1; q < jobs.size(); ++q)
+            workers.emplace_back([&, q] {
+                try {
+                    *jobs[q].out = row_blocks(n, jobs[q].start, kCap, kMaxRows);
+                } catch (...) {
+                    failed[q] = std::current_exception();
+                }
+            });
+        try {
+            *jobs[0].out = row_blocks(n, jobs[0].start, kCap, kMaxRows);
+        } catch (...) {
+            failed[0] = std::current_exception();
+        }
+        for (auto& w : workers) w.join();
+    }
+    for (auto& e : failed)
+        if (e) std::rethrow_exception(e);
     size_t bbytes = 0;
     int groups = 1;
     for (const auto* set : {&blk64, &blk32})

@@ -110,15 +110,24 @@ __global__ void scatter_csr(int64_t m, const uint64_t* ekey, const int* keep, co
 
 // max over rows of sum |value| in CSR order (accelerated_qp.cpp:26), as the bit
 // pattern of a nonnegative double (ordered like its value).
-__global__ void row_abs_max(int64_t n, const int* start, const double* val, unsigned long long* best) {
-    unsigned long long b = 0;
+// One atomic per workgroup: the maximum is combined in the workgroup first
+// (an atomic per thread on one address serializes the launch).
+__global__ __launch_bounds__(kThreads) void row_abs_max(int64_t n, const int* start, const double* val,
+                                                        unsigned long long* best) {
+    __shared__ double wmax[kThreads / 64];
+    double b = 0;
     for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
         double s = 0;
         for (int k = start[i]; k < start[i + 1]; ++k) s += fabs(val[k]);
-        const unsigned long long u = __double_as_longlong(s);
-        b = u > b ? u : b;
+        b = fmax(b, s);
     }
-    if (b) atomicMax(best, b);
+    for (int m = 32; m >= 1; m >>= 1) b = fmax(b, __shfl_xor(b, m, 64));
+    if (threadIdx.x % 64 == 0) wmax[threadIdx.x / 64] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) b = fmax(b, wmax[w]);
+        if (b > 0) atomicMax(best, static_cast<unsigned long long>(__double_as_longlong(b)));
+    }
 }
 
 template <class T>
