@@ -28,14 +28,33 @@ constexpr int kListThreads = 1024;
 
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// XCD-local order over a work list of `len` row blocks. Workgroups are
+// dispatched round-robin over the 8 XCDs (workgroup g runs on XCD g % 8), so
+// XCD x takes the x-th contiguous eighth of the list: the instances one XCD
+// steps at a time, and the vectors they gather, are a slice that its own L2
+// holds, instead of every XCD touching every active instance. Each block's
+// results go to that block's own slots, so the order changes no result.
+struct ListRange {
+    int idx, end, stride;
+};
+
+__device__ inline ListRange xcd_range(int len) {
+    constexpr int kXcds = 8;
+    const int wave = threadIdx.x / kWave;
+    if (gridDim.x % kXcds) return {int(blockIdx.x) * kWavesPerGroup + wave, len, int(gridDim.x) * kWavesPerGroup};
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds, per = gridDim.x / kXcds;
+    const int chunk = (len + kXcds - 1) / kXcds;
+    const int lo = min(len, x * chunk), hi = min(len, lo + chunk);
+    return {lo + j * kWavesPerGroup + wave, hi, per * kWavesPerGroup};
+}
+
 // ---- FP32 correction step over the active blocks -------------------------------
 template <int LOADS>
 __global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
     __shared__ float stage[kWavesPerGroup][kWave * LOADS];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int len = v.counts[0];
-    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
-        const int b = uniform(v.list_active[idx]);
+    for (ListRange q = xcd_range(v.counts[0]); q.idx < q.end; q.idx += q.stride) {
+        const int b = uniform(v.list_active[q.idx]);
         const int i = uniform(v.blk_inst[b]);
         const PoolInst* s = v.inst + i;
         if (s->phase != kSlotActive) continue;  // met its target earlier in this period
@@ -181,9 +200,8 @@ template <int LOADS>
 __global__ __launch_bounds__(kGroup) void pool_check64(PoolView v) {
     __shared__ double stage[kWavesPerGroup][kWave * LOADS];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int len = v.counts[1];
-    for (int idx = blockIdx.x * kWavesPerGroup + wave; idx < len; idx += gridDim.x * kWavesPerGroup) {
-        const int b = uniform(v.list_verify[idx]);
+    for (ListRange q = xcd_range(v.counts[1]); q.idx < q.end; q.idx += q.stride) {
+        const int b = uniform(v.list_verify[q.idx]);
         const int i = uniform(v.blk_inst[b]);
         CheckEpilogue epi;
         epi.x = v.x;
