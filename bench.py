@@ -204,6 +204,7 @@ def batch_workload(args):
                 assert pool.admit(d, tag=k, tolerance=KKT_TOL, lipschitz_value=Ls[k]) is not None
             if tuned is None:
                 tuned = pool.autotune()
+                step_us, step_bytes_all = pool.time_step(6)  # all 512 instances active, before the solve
             t0 = time.perf_counter()
             res = pool.solve_all()
             dt = time.perf_counter() - t0
@@ -243,6 +244,10 @@ def batch_workload(args):
                                f"one pool, graph-replayed check periods; one step = the whole batch",
                    "rows": rows, "nnz": nnz, "autotuned_block_loads": tuned[0], "autotuned_block_rows": tuned[1],
                    "check_every": args.check_every, "step_metric": args.pool_step},
+        "roofline": {"bound": "hbm", "achieved": round(step_bytes_all / step_us / 1e3, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(step_bytes_all / step_us / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"pool_step<{tuned[0]}> (FP32 step, all {len(batch)} instances active)",
+                     "algorithmic_bytes": step_bytes_all, "avg_launch_us": round(step_us, 1)},
         "all_verified": ok, "worst_max_abs_err_vs_planted_x": worst_x, "worst_objective_rel_err_vs_planted": worst_obj,
         "iterations": {"min": int(iters.min()), "median": float(np.median(iters)), "max": int(iters.max())},
         "plain_fp64_single_instance": {"solves_per_s": round(plain_rate, 2), "instances_timed": plain_n,

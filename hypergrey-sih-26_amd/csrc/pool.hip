@@ -226,63 +226,65 @@ void enqueue_periods(hgqp_pool* p, int periods) {
 }
 
 // Tunes block shape on the resident slots: times FP32 steps for each shape.
+// Times `reps` FP32 steps with every resident slot active at the current block
+// shape (2 untimed steps first), then restores the slot table and work lists.
+// Returns the average step time in microseconds.
+double time_all_active(hgqp_pool* p, int reps) {
+    std::vector<PoolInst> table(p->opt.max_instances);
+    HIPCHECK(hipMemcpyAsync(table.data(), p->v.inst, table.size() * sizeof(PoolInst), hipMemcpyDeviceToHost, p->stream));
+    HIPCHECK(hipStreamSynchronize(p->stream));
+    std::vector<PoolInst> active = table;
+    for (auto& t : active)
+        if (t.phase != kSlotFree) { t.phase = kSlotActive; t.t[0] = t.t[1] = 1; t.use_x[0] = t.use_x[1] = 0; }
+    HIPCHECK(hipMemcpyAsync(p->v.inst, active.data(), active.size() * sizeof(PoolInst), hipMemcpyHostToDevice,
+                            p->stream));
+    launch_pool_lists(p->v, p->stream);
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    const int bmax = int(std::min<int64_t>(p->max_blocks, 1 << 30));
+    launch_pool_steps(p->v, p->loads, bmax, 2, p->stream);  // warm
+    HIPCHECK(hipEventRecord(a, p->stream));
+    launch_pool_steps(p->v, p->loads, bmax, reps, p->stream);
+    HIPCHECK(hipEventRecord(b, p->stream));
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIPCHECK(e);
+    HIPCHECK(hipMemcpyAsync(p->v.inst, table.data(), table.size() * sizeof(PoolInst), hipMemcpyHostToDevice, p->stream));
+    launch_pool_lists(p->v, p->stream);
+    HIPCHECK(hipStreamSynchronize(p->stream));
+    return double(ms) * 1e3 / reps;
+}
+
 void autotune(hgqp_pool* p) {
     if (p->tuned) return;
     p->tuned = true;
     const bool tune_loads = p->opt.block_loads == 0, tune_rows = p->opt.block_rows == 0;
     if (!tune_loads && !tune_rows) return;
     if (!p->resident) return;  // nothing to time: keep the defaults
-    // a step needs c32/lo32/up32 and d/w: they hold whatever the slots left; steps
-    // are timed only, and the slots are re-initialized (phase INIT) afterwards
+    // Steps are timed only: they run on whatever c32/lo32/up32 and d/w the slots
+    // hold, and time_all_active restores the slot table afterwards.
     std::vector<int> loads_c, rows_c;
     for (int l : kLoadsChoices)
         if (tune_loads || l == p->opt.block_loads) loads_c.push_back(l);
     for (int r : {64, 256})
         if (tune_rows || r == p->opt.block_rows) rows_c.push_back(r);
-    hipEvent_t a, b;
-    HIPCHECK(hipEventCreate(&a));
-    HIPCHECK(hipEventCreate(&b));
-    float best = std::numeric_limits<float>::infinity();
+    double best = std::numeric_limits<double>::infinity();
     int best_l = p->loads, best_r = p->rows_per_block;
-    // make every resident slot "active" for the timing, then restore INIT
-    std::vector<PoolInst> table(p->opt.max_instances), saved;
-    HIPCHECK(hipMemcpyAsync(table.data(), p->v.inst, table.size() * sizeof(PoolInst), hipMemcpyDeviceToHost, p->stream));
-    HIPCHECK(hipStreamSynchronize(p->stream));
-    saved = table;
-    for (auto& t : table)
-        if (t.phase != kSlotFree) { t.phase = kSlotActive; t.t[0] = t.t[1] = 1; t.use_x[0] = t.use_x[1] = 0; }
     for (int l : loads_c)
         for (int r : rows_c) {
             p->loads = l;
             p->rows_per_block = r;
             reblock_all(p);
-            std::vector<PoolInst> t2(p->opt.max_instances);
-            HIPCHECK(hipMemcpyAsync(t2.data(), p->v.inst, t2.size() * sizeof(PoolInst), hipMemcpyDeviceToHost, p->stream));
-            HIPCHECK(hipStreamSynchronize(p->stream));
-            for (int s = 0; s < p->opt.max_instances; ++s) { table[s].b0 = t2[s].b0; table[s].nblk = t2[s].nblk; }
-            HIPCHECK(hipMemcpyAsync(p->v.inst, table.data(), table.size() * sizeof(PoolInst), hipMemcpyHostToDevice,
-                                    p->stream));
-            launch_pool_lists(p->v, p->stream);
-            const int bmax = int(std::min<int64_t>(p->max_blocks, 1 << 30));
-            launch_pool_steps(p->v, l, bmax, 2, p->stream);  // warm
-            HIPCHECK(hipEventRecord(a, p->stream));
-            launch_pool_steps(p->v, l, bmax, 6, p->stream);
-            HIPCHECK(hipEventRecord(b, p->stream));
-            HIPCHECK(hipEventSynchronize(b));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, a, b));
-            if (ms < best) { best = ms; best_l = l; best_r = r; }
+            const double us = time_all_active(p, 6);
+            if (us < best) { best = us; best_l = l; best_r = r; }
         }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     p->loads = best_l;
     p->rows_per_block = best_r;
     reblock_all(p);
-    std::vector<PoolInst> t2(p->opt.max_instances);
-    HIPCHECK(hipMemcpyAsync(t2.data(), p->v.inst, t2.size() * sizeof(PoolInst), hipMemcpyDeviceToHost, p->stream));
-    HIPCHECK(hipStreamSynchronize(p->stream));
-    for (int s = 0; s < p->opt.max_instances; ++s) { saved[s].b0 = t2[s].b0; saved[s].nblk = t2[s].nblk; }
-    HIPCHECK(hipMemcpyAsync(p->v.inst, saved.data(), saved.size() * sizeof(PoolInst), hipMemcpyHostToDevice, p->stream));
     launch_pool_lists(p->v, p->stream);
     HIPCHECK(hipStreamSynchronize(p->stream));
 }
@@ -886,6 +888,20 @@ int hgqp_pool_autotune(hgqp_pool* pool, int32_t* block_loads, int32_t* block_row
         autotune(pool);
         if (block_loads) *block_loads = pool->loads;
         if (block_rows) *block_rows = pool->rows_per_block;
+    });
+}
+
+int hgqp_pool_time_step(hgqp_pool* pool, int32_t reps, double* avg_us, double* bytes) {
+    return guarded([&] {
+        check_pool(pool);
+        if (reps < 1 || !avg_us || !bytes) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_time_step: need reps >= 1 and outputs");
+        if (!pool->resident) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool_time_step: no resident instance");
+        double b = 0;
+        const double vectors = pool->opt.step_metric == HGQP_STEP_JACOBI ? 8 : 7;
+        for (const SlotHost& sh : pool->slots)
+            if (sh.used) b += 8.0 * double(sh.nnz) + 4.0 * double(sh.n + 1) + vectors * 4.0 * double(sh.n);
+        *bytes = b;
+        *avg_us = time_all_active(pool, reps);
     });
 }
 

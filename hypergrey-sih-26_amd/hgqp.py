@@ -192,6 +192,7 @@ SIGNATURES = {
     "hgqp_pool_run": ([C.c_void_p, C.c_int32], C.c_int),
     "hgqp_pool_collect": ([C.c_void_p, C.POINTER(PoolResult), C.c_int32, P_I32, P_I32], C.c_int),
     "hgqp_pool_autotune": ([C.c_void_p, P_I32, P_I32], C.c_int),
+    "hgqp_pool_time_step": ([C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "hgqp_pool_destroy": ([C.c_void_p], C.c_int),
     "hgqp_server_start": ([C.POINTER(PoolOptions), C.POINTER(C.c_void_p)], C.c_int),
     "hgqp_server_submit": ([C.c_void_p] + _ADMIT[0:], C.c_int),
@@ -522,6 +523,13 @@ class Pool:
         l, r = C.c_int32(), C.c_int32()
         _raise(lib().hgqp_pool_autotune(self.handle, C.byref(l), C.byref(r)))
         return l.value, r.value
+
+    def time_step(self, reps=6):
+        """Average FP32 step time (us) with every resident instance active, and
+        the step's algorithmic bytes."""
+        us, nbytes = C.c_double(), C.c_double()
+        _raise(lib().hgqp_pool_time_step(self.handle, int(reps), C.byref(us), C.byref(nbytes)))
+        return us.value, nbytes.value
 
     def solve_all(self, max_periods=100000):
         """Runs periods until nothing is resident; returns all results."""

@@ -386,6 +386,13 @@ int hgqp_pool_collect(hgqp_pool* pool, hgqp_pool_result* results, int32_t capaci
 /* Chooses block_loads and block_rows by timing FP32 steps over the resident
  * instances (when either was 0 at creation); a no-op otherwise. */
 int hgqp_pool_autotune(hgqp_pool* pool, int32_t* block_loads, int32_t* block_rows);
+/* Measurement, for the roofline: the average time of `reps` back-to-back FP32
+ * steps with every resident instance active (HIP events on the pool's stream;
+ * the slot table is restored afterwards, so call it between solves), and the
+ * step's algorithmic bytes summed over the resident instances: 8 nnz + 4 (n+1)
+ * + 4 n per vector (7 vectors, 8 with the Jacobi metric). No reference
+ * counterpart. */
+int hgqp_pool_time_step(hgqp_pool* pool, int32_t reps, double* avg_us, double* bytes);
 
 int hgqp_pool_destroy(hgqp_pool* pool);
 

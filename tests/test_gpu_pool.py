@@ -104,6 +104,32 @@ def test_iteration_limit_and_autotune(hg):
     assert r["status"] == "optimal"
 
 
+def test_time_step_restores_the_pool(hg):
+    """hgqp_pool_time_step reports the algorithmic bytes of a full FP32 step over
+    the resident instances and leaves them as they were: the solve afterwards
+    equals the solve of the same instances in a pool that was never timed."""
+    batch = hg.gen_batch(6, seed=17, n_min=400, n_max=3000)
+    rows = sum(d["n"] + 1 for d in batch)
+    nnz = sum(len(d["value"]) for d in batch)
+    expect = sum(8 * len(d["value"]) + 4 * (d["n"] + 1) + 8 * 4 * d["n"] for d in batch)  # Jacobi: 8 vectors
+    out = []
+    for timed in (False, True):
+        with hg.Pool(max_rows=rows, max_nnz=nnz, max_instances=6) as p:
+            for k, d in enumerate(batch):
+                assert p.admit(d, tag=k, tolerance=KKT_TOL) is not None
+            if timed:
+                us, nbytes = p.time_step(4)
+                assert us > 0 and nbytes == expect
+            out.append({r["tag"]: r for r in p.solve_all()})
+    for k in range(len(batch)):
+        assert out[0][k]["status"] == out[1][k]["status"] == "optimal"
+        assert out[0][k]["iterations"] == out[1][k]["iterations"]
+        assert np.array_equal(out[0][k]["primal"], out[1][k]["primal"])
+    with hg.Pool(max_rows=rows, max_nnz=nnz, max_instances=6) as p:
+        with pytest.raises(hg.InvalidArgument):
+            p.time_step(4)  # nothing resident
+
+
 def test_server_results_equal_alone(hg):
     batch = hg.gen_batch(12, seed=17, n_min=300, n_max=2500)
     alone = [solve_alone(hg, d) for d in batch[:4]]
