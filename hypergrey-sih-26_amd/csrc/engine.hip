@@ -307,10 +307,19 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
     std::vector<int> pending;  // slot indices of enqueued read-backs, oldest first
     int next_slot = 1;         // slot 0 is kept for synchronous transfers
     bool stop = false;
+    // A phase that ends on the steps' own residual ends on the device at the
+    // exact iteration that meets it, whatever the period length; launches
+    // queued past that point return early but still cost a launch each. So its
+    // periods shrink after the first (check_every, then halving down to 2): the
+    // launches queued past the end are then a few short periods, not two full
+    // ones. Phases decided by a check in every period keep their cadence.
+    const bool shrink = residual_phase(precision, momentum);
+    int period = check_every;
     while (it < limit && !stop) {
-        const int steps = static_cast<int>(std::min<uint64_t>(check_every, limit - it));
+        const int steps = static_cast<int>(std::min<uint64_t>(period, limit - it));
+        if (shrink) period = std::max(2, (period + 1) / 2);
         if (timed) enqueue_period_timed(b, precision, steps, momentum, cp, it, timed);
-        else if (b->opt.use_graph && steps == check_every) {
+        else if (b->opt.use_graph && (steps == check_every || (shrink && steps >= 2))) {
             hipGraphExec_t g = period_graph(b, precision, steps, momentum, cp);
             HIPCHECK(hipGraphLaunch(g, b->stream));
             b->parity ^= (steps & 1);
