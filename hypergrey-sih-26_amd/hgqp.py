@@ -85,6 +85,12 @@ class Problem(C.Structure):
                 ("offset", C.c_double), ("maximize", C.c_int32), ("reserved", C.c_int32)]
 
 
+class Rows(C.Structure):
+    _fields_ = [("m", C.c_int64), ("start", C.POINTER(C.c_int64)), ("col", C.POINTER(C.c_int64)),
+                ("val", C.POINTER(C.c_double)), ("row_lower", C.POINTER(C.c_double)),
+                ("row_upper", C.POINTER(C.c_double))]
+
+
 class QpOptions(C.Structure):
     _fields_ = [("time_limit", C.c_double), ("tolerance", C.c_double), ("iteration_limit", C.c_uint64),
                 ("threads", C.c_int32), ("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
@@ -173,6 +179,10 @@ SIGNATURES = {
     "hgqp_box_destroy": ([C.c_void_p], C.c_int),
     "hgqp_box_info_get": ([C.c_void_p, C.POINTER(BoxInfo)], C.c_int),
     "hgqp_solve_qp": ([C.POINTER(Problem), C.POINTER(QpOptions), C.POINTER(QpResult), P_F64], C.c_int),
+    "hgqp_check_qp_solution": ([C.POINTER(Problem), C.POINTER(Rows), P_F64, P_F64, C.c_double,
+                                C.POINTER(QpResult)], C.c_int),
+    "hgqp_solve_qp_admm": ([C.POINTER(Problem), C.POINTER(Rows), C.POINTER(QpOptions), C.POINTER(QpResult),
+                            P_F64, P_F64], C.c_int),
     "hgqp_csr_lipschitz": ([C.c_int64, P_I32, P_F64], C.c_double),
     "hgqp_csr_assemble": ([C.POINTER(Problem), C.c_int64, P_I32, P_I32, P_F64, P_I64, P_F64], C.c_int),
     "hgqp_dense_options_default": ([C.POINTER(DenseOptions)], None),
@@ -352,6 +362,54 @@ def solve_qp(n, hrow, hcol, hval, cost, lower, upper, offset=0.0, maximize=False
     d = r.as_dict()
     d["primal"] = x
     return d
+
+
+def solve_qp_admm(n, hrow, hcol, hval, cost, lower, upper, rstart, rcol, rval, row_lower, row_upper, offset=0.0,
+                  maximize=False, **options):
+    """hgqp_solve_qp_admm: a convex QP with linear rows (row i: entries
+    rstart[i]:rstart[i+1] of (rcol, rval), row_lower <= a_i x <= row_upper),
+    the reference's scaled ADMM with its iterations on the GPU."""
+    p, keep = _problem(n, hrow, hcol, hval, cost, lower, upper, offset, maximize)
+    rows, keep_rows = _rows(rstart, rcol, rval, row_lower, row_upper)
+    m = rows.m
+    o = qp_options(**options)
+    r = QpResult()
+    x = np.zeros(int(n), dtype=np.float64)
+    y = np.zeros(max(m, 1), dtype=np.float64)
+    _raise(lib().hgqp_solve_qp_admm(C.byref(p), C.byref(rows), C.byref(o), C.byref(r), _ptr(x, C.c_double),
+                                    _ptr(y, C.c_double)))
+    d = r.as_dict()
+    d["primal"] = x
+    d["row_dual"] = y[:m].copy()
+    return d
+
+
+def _rows(rstart, rcol, rval, row_lower, row_upper):
+    rs = np.ascontiguousarray(rstart, dtype=np.int64)
+    rc = np.ascontiguousarray(rcol, dtype=np.int64)
+    rv = np.ascontiguousarray(rval, dtype=np.float64)
+    rl = np.ascontiguousarray(row_lower, dtype=np.float64)
+    ru = np.ascontiguousarray(row_upper, dtype=np.float64)
+    m = len(rl)
+    if len(ru) != m or len(rs) != m + 1:
+        raise InvalidArgument(INVALID_ARGUMENT, "row arrays disagree in length")
+    return Rows(m, _ptr(rs, C.c_int64), _ptr(rc, C.c_int64), _ptr(rv, C.c_double), _ptr(rl, C.c_double),
+                _ptr(ru, C.c_double)), (rs, rc, rv, rl, ru)
+
+
+def check_qp_solution(n, hrow, hcol, hval, cost, lower, upper, rstart, rcol, rval, row_lower, row_upper, x, row_dual,
+                      tolerance=1e-7, offset=0.0, maximize=False):
+    """hgqp_check_qp_solution: the reference's verdict on (x, row duals), on the host."""
+    p, keep = _problem(n, hrow, hcol, hval, cost, lower, upper, offset, maximize)
+    rows, keep_rows = _rows(rstart, rcol, rval, row_lower, row_upper)
+    xv = np.ascontiguousarray(x, dtype=np.float64)
+    yv = np.ascontiguousarray(row_dual, dtype=np.float64)
+    if len(xv) != int(n) or len(yv) != rows.m:
+        raise InvalidArgument(INVALID_ARGUMENT, "x and row duals must have lengths n and m")
+    r = QpResult()
+    _raise(lib().hgqp_check_qp_solution(C.byref(p), C.byref(rows), _ptr(xv, C.c_double),
+                                        _ptr(yv, C.c_double) if rows.m else None, float(tolerance), C.byref(r)))
+    return r.as_dict()
 
 
 def csr_assemble(n, hrow, hcol, hval, cost, lower, upper, maximize=False):

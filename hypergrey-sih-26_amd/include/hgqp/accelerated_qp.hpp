@@ -21,7 +21,8 @@
 namespace hgqp {
 
 // include/mps.hpp:8-21 (the fields a box QP uses are cost, lower, upper,
-// names, integer, offset, maximize; rows must be empty)
+// names, integer, offset, maximize; rows must be empty; solve_qp_admm takes
+// rows with row_lower <= a_i x <= row_upper)
 struct LinearProblem {
     std::string name;
     std::vector<std::string> names;
@@ -100,6 +101,14 @@ bool check_convex_hessian(const QuadraticProblem&);
 // include/accelerated_qp.hpp:17. `backend` names the engine; this library has
 // exactly one, "hip". Any other name throws std::invalid_argument.
 QpResult solve_accelerated_box(const QuadraticProblem&, const QpOptions&, const std::string& backend = "hip");
+
+// include/qp.hpp:32 and src/qp_admm.cpp:118-281: the scaled ADMM with PCG for a
+// model with linear rows, its iterations on the GPU (algorithm "hip_admm_pcg").
+// The reference's diagonal-equality fast path and KKT polish are not run.
+QpResult solve_qp_admm(const QuadraticProblem&, const QpOptions& = QpOptions());
+// include/qp.hpp:34 and src/qp_admm.cpp:63-117 (rows and row duals included),
+// evaluated on the host in long double as the reference does.
+void check_qp_solution(const QuadraticProblem&, QpResult&, double tolerance);
 
 // include/accelerated_qp.hpp:9-13
 struct BoxIteration {

@@ -251,6 +251,40 @@ typedef struct hgqp_qp_result {
 int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options* options, hgqp_qp_result* result,
                   double* primal_out);
 
+/* Linear rows of a model (mps.hpp:13-14: LinearProblem::rows, row_lower,
+ * row_upper): row i holds the entries start[i] .. start[i+1]-1 of (col, val),
+ * and row_lower[i] <= sum_k val_k x[col_k] <= row_upper[i]. */
+typedef struct hgqp_rows {
+    int64_t m;
+    const int64_t* start;     /* [m + 1], start[0] = 0, nondecreasing             */
+    const int64_t* col;       /* [start[m]]                                       */
+    const double* val;        /* [start[m]]                                       */
+    const double* row_lower;  /* [m]                                              */
+    const double* row_upper;  /* [m]                                              */
+} hgqp_rows;
+
+/* Replaces solve_qp_admm(q, options) (qp.hpp:32; qp_admm.cpp:118-281) for a
+ * convex QP with linear rows: the reference's scaling and its ADMM with a PCG
+ * inner solve, the iterations on the GPU (sparse products in the reference's
+ * row order, dot products as double-double sums), the verdict of
+ * check_qp_solution every 25 iterations on the host as the reference. Status
+ * and residuals follow QpResult; algorithm "hip_admm_pcg". Not run: the
+ * reference's diagonal-equality fast path (qp_admm.cpp:127-132) and its KKT
+ * polish (qp_admm.cpp:169-226). primal_out (n) and row_dual_out (m) may be
+ * NULL. Option fields other than time_limit, tolerance, iteration_limit and
+ * threads are ignored; scaling is on, the reference default. */
+int hgqp_solve_qp_admm(const hgqp_problem* problem, const hgqp_rows* rows, const hgqp_qp_options* options,
+                       hgqp_qp_result* result, double* primal_out, double* row_dual_out);
+
+/* check_qp_solution (qp.hpp:34; qp_admm.cpp:63-117) of a point x (n) and row
+ * duals (m; NULL when rows is NULL or m = 0) on a model with optional rows:
+ * objective, scaled violation, complementarity, stationarity of the
+ * Lagrangian gradient, dual bound, gap and the verdict at `tolerance`, in the
+ * reference's long double arithmetic on the host. Needs no device. Fills
+ * status "checked" and algorithm "check_qp_solution". */
+int hgqp_check_qp_solution(const hgqp_problem* problem, const hgqp_rows* rows, const double* x,
+                           const double* row_dual, double tolerance, hgqp_qp_result* result);
+
 /* L = max over rows of the sum of |value| in stored order
  * (accelerated_qp.cpp:25-26), for callers that assemble the CSR themselves. */
 double hgqp_csr_lipschitz(int64_t n, const int32_t* start, const double* value);

@@ -10,6 +10,9 @@
  *   original_violation      src/mps.cpp:279-304
  *   orc_check_convex        src/qp_admm.cpp:30-62     check_convex_hessian
  *   orc_check               src/qp_admm.cpp:63-117    check_qp_solution
+ *   orc_check_rows          src/qp_admm.cpp:63-117    check_qp_solution with linear
+ *                           rows and row duals (original_violation with rows,
+ *                           src/mps.cpp:279-304; validation src/mps.cpp:47-60)
  *   orc_build_csr           src/accelerated_qp.cpp:20-24 + include/sparse_matrix.hpp:10-17
  *   orc_lipschitz           src/accelerated_qp.cpp:25-26
  *   orc_spmv                include/sparse_matrix.hpp:26-33
@@ -50,6 +53,10 @@ struct Problem {
     std::vector<double> hval, cost, lower, upper;
     double offset = 0;
     bool maximize = false;
+    // linear rows: row i is (rcol, rval)[rstart[i] .. rstart[i+1])
+    std::size_t m = 0;
+    std::vector<std::size_t> rstart, rcol;
+    std::vector<double> rval, row_lower, row_upper;
 };
 
 Problem load(const orc_problem* p) {
@@ -72,7 +79,27 @@ Problem load(const orc_problem* p) {
     return q;
 }
 
-/* validate_linear_problem, src/mps.cpp:47-60, for a model without rows. */
+void load_rows(const orc_rows* r, Problem& q) {
+    if (!r) return;
+    if (r->m < 0) throw std::invalid_argument("Invalid linear problem dimensions or offset");
+    q.m = static_cast<std::size_t>(r->m);
+    q.rstart.resize(q.m + 1);
+    for (std::size_t i = 0; i <= q.m; ++i) {
+        if (r->start[i] < 0 || (i && r->start[i] < r->start[i - 1])) throw std::invalid_argument("Invalid sparse row entry");
+        q.rstart[i] = static_cast<std::size_t>(r->start[i]);
+    }
+    const std::size_t e = q.rstart[q.m];
+    q.rcol.resize(e);
+    for (std::size_t k = 0; k < e; ++k) {
+        if (r->col[k] < 0) throw std::invalid_argument("Invalid sparse row entry");
+        q.rcol[k] = static_cast<std::size_t>(r->col[k]);
+    }
+    q.rval.assign(r->val, r->val + e);
+    q.row_lower.assign(r->lower, r->lower + q.m);
+    q.row_upper.assign(r->upper, r->upper + q.m);
+}
+
+/* validate_linear_problem, src/mps.cpp:47-60. */
 void validate(const Problem& p) {
     if (!p.n || !std::isfinite(p.offset))
         throw std::invalid_argument("Invalid linear problem dimensions or offset");
@@ -80,9 +107,16 @@ void validate(const Problem& p) {
         if (!std::isfinite(p.cost[j]) || std::isnan(p.lower[j]) || std::isnan(p.upper[j]) ||
             p.lower[j] == kInf || p.upper[j] == -kInf)
             throw std::invalid_argument("Invalid variable coefficient or bound");
+    for (std::size_t i = 0; i < p.m; ++i) {
+        if (std::isnan(p.row_lower[i]) || std::isnan(p.row_upper[i]) || p.row_lower[i] == kInf ||
+            p.row_upper[i] == -kInf)
+            throw std::invalid_argument("Invalid row bound");
+        for (std::size_t k = p.rstart[i]; k < p.rstart[i + 1]; ++k)
+            if (p.rcol[k] >= p.n || !std::isfinite(p.rval[k])) throw std::invalid_argument("Invalid sparse row entry");
+    }
 }
 
-/* original_violation(p, x, scaled=true), src/mps.cpp:279-304 (no rows). */
+/* original_violation(p, x, scaled=true), src/mps.cpp:279-304. */
 double original_violation(const Problem& p, const std::vector<double>& x) {
     validate(p);
     if (x.size() != p.n) throw std::invalid_argument("Wrong original solution size");
@@ -97,6 +131,17 @@ double original_violation(const Problem& p, const std::vector<double>& x) {
         if (!std::isfinite(x[j])) return kInf;
         check(x[j], p.lower[j], true, std::abs(x[j]));
         check(x[j], p.upper[j], false, std::abs(x[j]));
+    }
+    for (std::size_t i = 0; i < p.m; ++i) {
+        long double activity = 0, scale = 0;
+        for (std::size_t k = p.rstart[i]; k < p.rstart[i + 1]; ++k) {
+            long double term = static_cast<long double>(p.rval[k]) * x[p.rcol[k]];
+            activity += term;
+            scale += std::abs(term);
+        }
+        if (!std::isfinite(activity)) return kInf;
+        check(activity, p.row_lower[i], true, scale);
+        check(activity, p.row_upper[i], false, scale);
     }
     return result;
 }
@@ -158,17 +203,19 @@ struct Result {
     bool verified = false, positive_diagonal = false;
 };
 
-/* check_qp_solution, src/qp_admm.cpp:63-117, for a model without rows and an
- * empty row dual (the only shape the box path produces). */
-void check_solution(const Problem& p, const std::vector<double>& x, Result& r, double tol) {
+/* check_qp_solution, src/qp_admm.cpp:63-117. `y` is the row dual (length m;
+ * empty for a box model). */
+void check_solution(const Problem& p, const std::vector<double>& x, Result& r, double tol,
+                    const std::vector<double>& y = std::vector<double>()) {
     double sign = p.maximize ? -1 : 1;
     r.verified = false;
     r.objective = std::numeric_limits<double>::quiet_NaN();
     r.dual_bound = -kInf;
     r.primal_residual = r.stationarity = r.complementarity = r.relative_gap = kInf;
     r.positive_diagonal = false;
-    if (x.size() != p.n) return;
+    if (x.size() != p.n || y.size() != p.m) return;
     for (double v : x) if (!std::isfinite(v)) return;
+    for (double v : y) if (!std::isfinite(v)) return;
     std::vector<double> gradient(x.size(), 0);
     long double value = sign * p.offset;
     std::vector<long double> diagonal, linear;
@@ -200,6 +247,24 @@ void check_solution(const Problem& p, const std::vector<double>& x, Result& r, d
     r.primal_residual = original_violation(p, x);
     r.complementarity = 0;
     long double bound = value, dual_constant = sign * p.offset;
+    for (std::size_t i = 0; i < p.m; ++i) {
+        long double activity = 0;
+        for (std::size_t k = p.rstart[i]; k < p.rstart[i + 1]; ++k) {
+            const std::size_t j = p.rcol[k];
+            activity += static_cast<long double>(p.rval[k]) * x[j];
+            gradient[j] += p.rval[k] * y[i];
+            if (positive_diagonal) linear[j] += static_cast<long double>(p.rval[k]) * y[i];
+        }
+        const double yi = y[i], target = yi >= 0 ? p.row_upper[i] : p.row_lower[i];
+        if (yi != 0) {
+            dual_constant -= static_cast<long double>(yi) * target;
+            long double correction = yi * (activity - target);
+            bound += correction;
+            double residual = static_cast<double>(std::abs(correction) / (1 + std::abs(value)));
+            if (!std::isfinite(residual)) residual = kInf;
+            r.complementarity = std::max(r.complementarity, residual);
+        }
+    }
     r.stationarity = 0;
     for (std::size_t j = 0; j < x.size(); ++j) {
         const double g = gradient[j];
@@ -392,6 +457,20 @@ int orc_check_convex(const orc_problem* p) {
     int verdict = 0;
     int rc = guarded([&] { verdict = check_convex(load(p)) ? 1 : 0; });
     return rc == ORC_OK ? verdict : -rc;
+}
+
+/* check_qp_solution for a model with linear rows and its row duals. */
+int orc_check_rows(const orc_problem* p, const orc_rows* rows, const double* x, const double* row_dual, double tol,
+                   orc_result* out) {
+    return guarded([&] {
+        Problem q = load(p);
+        load_rows(rows, q);
+        validate(q);
+        std::vector<double> xv(x, x + q.n), yv(row_dual, row_dual + q.m);
+        Result r;
+        check_solution(q, xv, r, tol, yv);
+        export_result(r, out);
+    });
 }
 
 int orc_check(const orc_problem* p, const double* x, double tol, orc_result* out) {

@@ -28,6 +28,11 @@ class Problem(C.Structure):
     ]
 
 
+class RowsStruct(C.Structure):
+    _fields_ = [("m", C.c_int64), ("start", C.POINTER(C.c_int64)), ("col", C.POINTER(C.c_int64)),
+                ("val", C.POINTER(C.c_double)), ("lower", C.POINTER(C.c_double)), ("upper", C.POINTER(C.c_double))]
+
+
 class Result(C.Structure):
     _fields_ = [
         ("status", C.c_char * 32), ("algorithm", C.c_char * 32),
@@ -81,6 +86,23 @@ class BoxQP:
                        _ptr(self.upper, C.c_double), self.offset, int(self.maximize))
 
 
+class Rows:
+    """Linear rows of a model (mps.hpp:13-14) in CSR form: row i holds
+    (col, val)[start[i]:start[i+1]], with row_lower <= a_i x <= row_upper."""
+
+    def __init__(self, start, col, val, lower, upper):
+        self.start = np.ascontiguousarray(start, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int64)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.lower = np.ascontiguousarray(lower, dtype=np.float64)
+        self.upper = np.ascontiguousarray(upper, dtype=np.float64)
+        self.m = len(self.lower)
+
+    def struct(self):
+        return RowsStruct(self.m, _ptr(self.start, C.c_int64), _ptr(self.col, C.c_int64), _ptr(self.val, C.c_double),
+                          _ptr(self.lower, C.c_double), _ptr(self.upper, C.c_double))
+
+
 class Oracle:
     def __init__(self, kind="oracle"):
         path, prefix = LIBS[kind]
@@ -97,6 +119,11 @@ class Oracle:
                                    C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         f("solve_box").argtypes = [C.POINTER(Problem), C.POINTER(Options), C.POINTER(Result), C.POINTER(C.c_double)]
         f("validate").argtypes = [C.POINTER(Problem)]
+        f("check_rows").argtypes = [C.POINTER(Problem), C.POINTER(RowsStruct), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_double, C.POINTER(Result)]
+        if kind == "reference":
+            self.lib.ref_solve_admm.argtypes = [C.POINTER(Problem), C.POINTER(RowsStruct), C.POINTER(Options),
+                                                C.POINTER(Result), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         if kind == "oracle":
             self.lib.orc_lipschitz.restype = C.c_double
             self.lib.orc_lipschitz.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
@@ -195,6 +222,33 @@ class Oracle:
                                          _ptr(cost, C.c_double), _ptr(lower, C.c_double), _ptr(upper, C.c_double),
                                          _ptr(opt, C.c_double))
         return BoxQP(n, hrow, hcol, hval, cost, lower, upper), opt, f
+
+
+    def check_rows(self, q, rows, x, row_dual, tol=1e-7):
+        """check_qp_solution (qp_admm.cpp:63-117) of (x, row_dual) on a model with rows."""
+        s, rs = q.struct(), rows.struct()
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(row_dual, dtype=np.float64)
+        r = Result()
+        self._raise(self._fn("check_rows")(C.byref(s), C.byref(rs), _ptr(x, C.c_double), _ptr(y, C.c_double),
+                                           tol, C.byref(r)))
+        return r.as_dict()
+
+    def solve_admm(self, q, rows, tolerance=1e-7, iteration_limit=200000, time_limit=60.0, threads=1):
+        """The reference's own solve_qp_admm (reference library only)."""
+        if self.kind != "reference":
+            raise OracleError(3, "the ADMM path is driven through the compiled reference only")
+        s, rs = q.struct(), rows.struct()
+        o = Options(time_limit, tolerance, iteration_limit, threads)
+        r = Result()
+        x = np.zeros(q.n)
+        y = np.zeros(rows.m)
+        self._raise(self.lib.ref_solve_admm(C.byref(s), C.byref(rs), C.byref(o), C.byref(r), _ptr(x, C.c_double),
+                                            _ptr(y, C.c_double)))
+        d = r.as_dict()
+        d["primal"] = x
+        d["row_dual"] = y
+        return d
 
 
 def available(kind):

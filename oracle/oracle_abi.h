@@ -30,6 +30,17 @@ typedef struct orc_problem {
     int32_t maximize;        /* 0 = minimize, 1 = maximize */
 } orc_problem;
 
+/* Linear rows of a model (mps.hpp:13-14: LinearProblem::rows, row_lower,
+ * row_upper): row i holds the entries start[i] .. start[i+1]-1 of (col, val). */
+typedef struct orc_rows {
+    int64_t m;
+    const int64_t* start;    /* [m + 1] */
+    const int64_t* col;      /* [start[m]] */
+    const double* val;       /* [start[m]] */
+    const double* lower;     /* [m] */
+    const double* upper;     /* [m] */
+} orc_rows;
+
 typedef struct orc_result {
     char status[32];
     char algorithm[32];

@@ -8,9 +8,10 @@
  * ref_ prefix and the same flat structures (oracle_abi.h).
  *
  * Reference interfaces used: sovereign::solve_accelerated_box
- * (include/accelerated_qp.hpp:17), check_qp_solution and check_convex_hessian
- * (include/qp.hpp:33-34), validate_linear_problem (include/mps.hpp:31),
- * CsrMatrix (include/sparse_matrix.hpp:5-34).
+ * (include/accelerated_qp.hpp:17), solve_qp_admm (include/qp.hpp:32),
+ * check_qp_solution and check_convex_hessian (include/qp.hpp:33-34),
+ * validate_linear_problem (include/mps.hpp:31), CsrMatrix
+ * (include/sparse_matrix.hpp:5-34).
  */
 #include "oracle_abi.h"
 
@@ -45,6 +46,21 @@ sovereign::QuadraticProblem to_reference(const orc_problem* p) {
         q.hessian.push_back({static_cast<std::size_t>(p->hrow[k]), static_cast<std::size_t>(p->hcol[k]), p->hval[k]});
     }
     return q;
+}
+
+void add_rows(const orc_rows* r, sovereign::QuadraticProblem& q) {
+    if (!r) return;
+    if (r->m < 0) throw std::invalid_argument("Invalid linear problem dimensions or offset");
+    auto& lp = q.linear;
+    const std::size_t m = static_cast<std::size_t>(r->m);
+    lp.rows.resize(m);
+    for (std::size_t i = 0; i < m; ++i)
+        for (int64_t k = r->start[i]; k < r->start[i + 1]; ++k) {
+            if (r->col[k] < 0) throw std::invalid_argument("Invalid sparse row entry");
+            lp.rows[i].push_back({static_cast<std::size_t>(r->col[k]), r->val[k]});
+        }
+    lp.row_lower.assign(r->lower, r->lower + m);
+    lp.row_upper.assign(r->upper, r->upper + m);
 }
 
 void export_result(const sovereign::QpResult& r, orc_result* out) {
@@ -129,6 +145,41 @@ int ref_solve_box(const orc_problem* p, const orc_options* opt, orc_result* out,
         export_result(r, out);
         if (primal_out && r.primal.size() == q.linear.cost.size())
             std::memcpy(primal_out, r.primal.data(), r.primal.size() * sizeof(double));
+    });
+}
+
+int ref_check_rows(const orc_problem* p, const orc_rows* rows, const double* x, const double* row_dual, double tol,
+                   orc_result* out) {
+    return guarded([&] {
+        auto q = to_reference(p);
+        add_rows(rows, q);
+        sovereign::validate_linear_problem(q.linear);
+        sovereign::QpResult r;
+        r.primal.assign(x, x + q.linear.cost.size());
+        r.row_dual.assign(row_dual, row_dual + q.linear.rows.size());
+        sovereign::check_qp_solution(q, r, tol);
+        export_result(r, out);
+    });
+}
+
+/* The reference's ADMM (qp_admm.cpp:118-281) on a model with rows; primal_out
+ * (n) and row_dual_out (m) receive its result when it has one. */
+int ref_solve_admm(const orc_problem* p, const orc_rows* rows, const orc_options* opt, orc_result* out,
+                   double* primal_out, double* row_dual_out) {
+    return guarded([&] {
+        auto q = to_reference(p);
+        add_rows(rows, q);
+        sovereign::QpOptions o;
+        o.time_limit = opt->time_limit;
+        o.tolerance = opt->tolerance;
+        o.iteration_limit = static_cast<std::size_t>(opt->iteration_limit);
+        o.threads = opt->threads;
+        auto r = sovereign::solve_qp_admm(q, o);
+        export_result(r, out);
+        if (primal_out && r.primal.size() == q.linear.cost.size())
+            std::memcpy(primal_out, r.primal.data(), r.primal.size() * sizeof(double));
+        if (row_dual_out && r.row_dual.size() == q.linear.rows.size())
+            std::memcpy(row_dual_out, r.row_dual.data(), r.row_dual.size() * sizeof(double));
     });
 }
 

@@ -41,3 +41,53 @@ def random_csr_qp(n, seed, long_rows=(), density=0.01, empty_rows=()):
     cost = rng.normal(0, 1, size=n)
     lower = -rng.uniform(0.5, 2, size=n); upper = rng.uniform(0.5, 2, size=n)
     return BoxQP(n, hr, hc, hv, cost, lower, upper)
+
+
+def rows_qp(n, m_eq, m_ineq, seed, row_nnz=6, free=0.1):
+    """Convex QP with linear rows, feasible by construction: sparse strictly
+    diagonally dominant H (lower triplets), m_eq equality rows and m_ineq
+    inequality rows (two- or one-sided) through a point x0 inside the box,
+    bounds [-1, 1] with a fraction `free` of the variables unbounded."""
+    from oracle import Rows
+    rng = np.random.default_rng(seed)
+    k = 3 * n
+    r = rng.integers(0, n, size=k)
+    c = rng.integers(0, n, size=k)
+    keep = r > c
+    r, c = r[keep], c[keep]
+    v = rng.normal(0, 0.1, size=len(r))
+    radius = np.zeros(n)
+    np.add.at(radius, r, np.abs(v))
+    np.add.at(radius, c, np.abs(v))
+    diag = radius + rng.uniform(0.5, 2.0, size=n)
+    hr = np.concatenate([r, np.arange(n)])
+    hc = np.concatenate([c, np.arange(n)])
+    hv = np.concatenate([v, diag])
+    lower = -np.ones(n)
+    upper = np.ones(n)
+    free_idx = rng.random(n) < free
+    lower[free_idx] = -np.inf
+    upper[free_idx] = np.inf
+    x0 = rng.uniform(-0.5, 0.5, size=n)
+    m = m_eq + m_ineq
+    start = np.zeros(m + 1, dtype=np.int64)
+    cols, vals = [], []
+    for i in range(m):
+        cj = np.sort(rng.choice(n, size=row_nnz, replace=False))
+        cols.append(cj)
+        vals.append(rng.normal(0, 1, size=row_nnz))
+        start[i + 1] = start[i] + row_nnz
+    col = np.concatenate(cols) if m else np.zeros(0, dtype=np.int64)
+    val = np.concatenate(vals) if m else np.zeros(0)
+    act = np.array([vals[i] @ x0[cols[i]] for i in range(m)])
+    rl = act.copy()
+    ru = act.copy()
+    # inequality rows are two-sided with finite bounds: the check takes the
+    # bound on the side of the dual's sign, so a finite far side keeps the
+    # complementarity term finite for an inactive row's small ADMM dual
+    for i in range(m_eq, m):
+        rl[i] = act[i] - rng.uniform(0.05, 1.0)
+        ru[i] = act[i] + rng.uniform(0.05, 1.0)
+    cost = rng.normal(0, 1, size=n)
+    q = BoxQP(n, hr, hc, hv, cost, lower, upper)
+    return q, Rows(start, col, val, rl, ru)
