@@ -413,6 +413,106 @@ def dense_workload(args):
     }
 
 
+def gen_rows_qp(n, m_eq, m_ineq, seed, row_nnz=6, free=0.1):
+    """Convex QP with linear rows, feasible by construction: a sparse strictly
+    diagonally dominant H (lower triplets, ~1.5 off-diagonals per row), m_eq
+    equality rows and m_ineq two-sided inequality rows with row_nnz entries
+    each through a point x0 inside the box, bounds [-1, 1] with a fraction
+    `free` of the variables unbounded. Rows as CSR (start, col, val)."""
+    rng = np.random.default_rng(seed)
+    r = rng.integers(0, n, size=3 * n)
+    c = rng.integers(0, n, size=3 * n)
+    keep = r > c
+    r, c = r[keep], c[keep]
+    v = rng.normal(0, 0.1, size=len(r))
+    radius = np.zeros(n)
+    np.add.at(radius, r, np.abs(v))
+    np.add.at(radius, c, np.abs(v))
+    diag = radius + rng.uniform(0.5, 2.0, size=n)
+    lower, upper = -np.ones(n), np.ones(n)
+    free_idx = rng.random(n) < free
+    lower[free_idx], upper[free_idx] = -np.inf, np.inf
+    x0 = rng.uniform(-0.5, 0.5, size=n)
+    m = m_eq + m_ineq
+    col = np.sort(np.array([rng.choice(n, size=row_nnz, replace=False) for _ in range(m)]), axis=1)
+    val = rng.normal(0, 1, size=(m, row_nnz))
+    act = np.einsum("ij,ij->i", val, x0[col])
+    rl, ru = act.copy(), act.copy()
+    rl[m_eq:] -= rng.uniform(0.05, 1.0, size=m_ineq)
+    ru[m_eq:] += rng.uniform(0.05, 1.0, size=m_ineq)
+    return dict(n=n, hrow=np.concatenate([r, np.arange(n)]), hcol=np.concatenate([c, np.arange(n)]),
+                hval=np.concatenate([v, diag]), cost=rng.normal(0, 1, size=n), lower=lower, upper=upper,
+                rstart=np.arange(m + 1, dtype=np.int64) * row_nnz, rcol=col.reshape(-1).astype(np.int64),
+                rval=val.reshape(-1), row_lower=rl, row_upper=ru, m=m)
+
+
+def admm_workload(args):
+    """Convex QP with linear rows through the GPU ADMM (hgqp_solve_qp_admm),
+    against the reference's own CPU ADMM (oracle/_ref) on the same model."""
+    n = args.admm_n
+    p = gen_rows_qp(n, n // 20, 3 * n // 20, args.seed)
+    nnz_h = 2 * (len(p["hval"]) - n) + n
+    nnz_a = len(p["rval"]) + n  # the identity rows of the variable bounds join A
+    M = p["m"] + n
+    keys = ("n", "hrow", "hcol", "hval", "cost", "lower", "upper", "rstart", "rcol", "rval", "row_lower", "row_upper")
+
+    def solve():
+        return hgqp.solve_qp_admm(*(p[k] for k in keys), tolerance=KKT_TOL, iteration_limit=20000, time_limit=300,
+                                  use_graph=0 if args.no_graph else 1)
+
+    for _ in range(args.warmup):
+        solve()
+    lat, res = [], []
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        res.append(solve())
+        lat.append(time.perf_counter() - a)
+    r = res[-1]
+    ok = all(x["status"] == "optimal" and x["optimality_verified"] for x in res)
+    same = all(np.array_equal(x["primal"], r["primal"]) for x in res)
+    wall = float(np.median(lat))
+    # one PCG iteration: t = rho (A d), H d, kd = H d + sigma d + A' t, the
+    # vector updates (next, r, pre, d) - reads and writes of each array once
+    pcg_bytes = (12 * nnz_a + 4 * (M + 1) + 8 * n + 16 * M) + (12 * nnz_h + 4 * (n + 1) + 16 * n) + \
+                (12 * nnz_a + 4 * (n + 1) + 8 * M + 24 * n) + 88 * n
+    inner = int(r["inner_iterations"])
+    achieved = inner * pcg_bytes / r["solve_seconds"] / 1e9
+    out = {
+        "metric": METRIC, "value": round(1.0 / wall, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded linearly constrained QP, host arrays in)",
+        "config": {"workload": f"linearly constrained QP: n={n}, {n // 20} equality + {3 * n // 20} two-sided "
+                               f"inequality rows (6 entries each), sparse diagonally dominant H, bounds [-1,1] "
+                               f"(10% free), KKT<=1e-6; one step = one complete ADMM solve from host arrays",
+                   "nnz_h": nnz_h, "nnz_a_with_bound_rows": nnz_a, "rows": p["m"]},
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "PCG inner iterations (rho_a, h_product, kd_product, pcg_update, pcg_direction): "
+                               "algorithmic bytes of all of them over the whole solve time",
+                     "algorithmic_bytes_per_pcg_iteration": pcg_bytes, "pcg_iterations": inner},
+        "iterations": int(r["iterations"]), "objective": r["objective"], "all_verified": ok,
+        "repeat_runs_identical": same, "algorithm": r["algorithm"], "device": hgqp.device_name(),
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import BoxQP, Oracle, Rows, available
+        if available("reference"):
+            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            q = BoxQP(n, p["hrow"], p["hcol"], p["hval"], p["cost"], p["lower"], p["upper"])
+            rows = Rows(p["rstart"], p["rcol"], p["rval"], p["row_lower"], p["row_upper"])
+            a = time.perf_counter()
+            rr = Oracle("reference").solve_admm(q, rows, tolerance=KKT_TOL, time_limit=300, threads=threads)
+            t = time.perf_counter() - a
+            out["cpu_baseline"] = {
+                "value": 1.0 / t, "unit": "solves/s", "cores": threads, "kind": "reference",
+                "sample": f"oracle/_ref (the reference's solve_qp_admm), {threads} OpenMP threads, one complete "
+                          f"solve of the same model: {rr['status']} ({rr['algorithm']}), {rr['iterations']} "
+                          f"iterations, {t:.2f}s",
+                "objective_rel_diff_vs_gpu": abs(rr["objective"] - r["objective"]) / (1 + abs(rr["objective"]))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -426,7 +526,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--check-every", type=int, default=10, help="check cadence of the non-reference modes")
     ap.add_argument("--polish-every", type=int, default=5, help="FP64 check cadence after an FP32 phase")
-    ap.add_argument("--workload", choices=["single", "batch", "serve", "dense"], default="single")
+    ap.add_argument("--workload", choices=["single", "batch", "serve", "dense", "admm"], default="single")
+    ap.add_argument("--admm-n", type=int, default=100_000, help="admm: variables (rows: n/20 equality, 3n/20 inequality)")
     ap.add_argument("--dense-n", type=int, default=4096)
     ap.add_argument("--dense-k", type=int, default=256)
     ap.add_argument("--dense-check", type=int, default=256, help="dense: RHS re-checked by the oracle")
@@ -442,7 +543,7 @@ def main():
     ap.add_argument("--pool-step", choices=sorted(STEP), default="jacobi",
                     help="batch/serve: step metric of the pool (per-row Jacobi or the scalar 1/L)")
     ap.add_argument("--no-graph", action="store_true",
-                    help="serve: launch each period's kernels directly instead of replaying a captured graph")
+                    help="serve, admm: launch each kernel directly instead of replaying captured graphs")
     ap.add_argument("--saturate", type=int, default=4000,
                     help="serve: also replay this many requests queued at once (capacity); 0 = skip")
     args = ap.parse_args()
@@ -458,6 +559,9 @@ def main():
         return
     if args.workload == "dense":
         print(json.dumps(dense_workload(args)))
+        return
+    if args.workload == "admm":
+        print(json.dumps(admm_workload(args)))
         return
 
     device = hgqp.device_name()

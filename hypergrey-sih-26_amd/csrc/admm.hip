@@ -429,6 +429,15 @@ __global__ __launch_bounds__(kGroup) void norms_at(View v) {
 
 int grid_elem(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096)); }
 
+// Captured launch sequences of one solve, destroyed with it.
+struct Graphs {
+    std::vector<hipGraphExec_t> all;
+    std::map<int, hipGraphExec_t> batch;
+    ~Graphs() {
+        for (auto e : all) (void)hipGraphExecDestroy(e);
+    }
+};
+
 struct Stream {
     hipStream_t s = nullptr;
     Stream() { HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
@@ -495,6 +504,7 @@ void fill_qp_result(const QpResult& r, hgqp_qp_result* out) {
     out->optimality_verified = r.optimality_verified ? 1 : 0;
     out->restarts = 0;
     out->fp32_iterations = 0;
+    out->inner_iterations = r.inner_iterations;
 }
 
 // ---- host ------------------------------------------------------------------------
@@ -850,38 +860,77 @@ QpResult solve_qp_admm(const QuadraticProblem& original, const QpOptions& opt) {
             setup();
         }
     };
-    // one ADMM iteration (qp_admm.cpp:244-263); PCG inner iterations go in
-    // batches, each launch skipping work once the solve has stopped on the
-    // device, so a batch boundary never changes the result
-    int batch = 4;
-    Pcg state{};
-    auto step = [&](std::size_t iteration) {
-        const double factor = std::min(1e-7, .05 / std::pow(static_cast<double>(iteration + 1), 1.6));
+    // one ADMM iteration (qp_admm.cpp:244-263). PCG inner iterations go in
+    // batches; each launch skips its work once the solve has stopped on the
+    // device, so a batch boundary never changes the result. The preparation,
+    // the batches and the update replay as captured graphs (the launches are
+    // small: the products at these sizes take a few microseconds each).
+    auto prep_launches = [&] {
         hipLaunchKernelGGL(make_yz, dim3(grid_elem(int64_t(M))), blk, 0, st, v);
         hipLaunchKernelGGL(make_rhs, dim3(gat), blk, 0, st, v);
         hipLaunchKernelGGL(rho_a, dim3(ga), blk, 0, st, v, (const double*)v.next, 0);
         hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.next, 0);
         hipLaunchKernelGGL(sys_prep, dim3(gat), blk, 0, st, v);
+    };
+    auto inner_launches = [&](int count) {
+        for (int k = 0; k < count; ++k) {
+            hipLaunchKernelGGL(rho_a, dim3(ga), blk, 0, st, v, (const double*)v.d, 1);
+            hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.d, 1);
+            hipLaunchKernelGGL(kd_product, dim3(gat), blk, 0, st, v);
+            hipLaunchKernelGGL(finish_den, dim3(1), dim3(kFinalThreads), 0, st, v, gat);
+            hipLaunchKernelGGL(pcg_update, dim3(groups_e), blk, 0, st, v);
+            hipLaunchKernelGGL(finish_rz, dim3(1), dim3(kFinalThreads), 0, st, v, groups_e);
+            hipLaunchKernelGGL(pcg_direction, dim3(groups_e), blk, 0, st, v);
+        }
+    };
+    auto update_launches = [&] {
+        hipLaunchKernelGGL(relax_zy, dim3(ga), blk, 0, st, v);
+        hipLaunchKernelGGL(relax_x, dim3(groups_e), blk, 0, st, v);
+    };
+    Graphs graphs;
+    auto capture = [&](auto&& launches) {
+        hipGraph_t g;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launches();
+        HIPCHECK(hipStreamEndCapture(st, &g));
+        hipGraphExec_t e;
+        const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(err);
+        graphs.all.push_back(e);
+        return e;
+    };
+    const hipGraphExec_t prep_graph = opt.use_graph ? capture(prep_launches) : nullptr;
+    const hipGraphExec_t update_graph = opt.use_graph ? capture(update_launches) : nullptr;
+    auto batch_graph = [&](int count) {
+        auto it = graphs.batch.find(count);
+        if (it != graphs.batch.end()) return it->second;
+        const hipGraphExec_t e = capture([&] { inner_launches(count); });
+        graphs.batch[count] = e;
+        return e;
+    };
+    int batch = 4;
+    Pcg state{};
+    auto step = [&](std::size_t iteration) {
+        const double factor = std::min(1e-7, .05 / std::pow(static_cast<double>(iteration + 1), 1.6));
+        if (opt.use_graph) HIPCHECK(hipGraphLaunch(prep_graph, st));
+        else prep_launches();
         hipLaunchKernelGGL(finish_prep, dim3(1), dim3(kFinalThreads), 0, st, v, gat, factor);
         for (;;) {
-            for (int k = 0; k < batch; ++k) {
-                hipLaunchKernelGGL(rho_a, dim3(ga), blk, 0, st, v, (const double*)v.d, 1);
-                hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.d, 1);
-                hipLaunchKernelGGL(kd_product, dim3(gat), blk, 0, st, v);
-                hipLaunchKernelGGL(finish_den, dim3(1), dim3(kFinalThreads), 0, st, v, gat);
-                hipLaunchKernelGGL(pcg_update, dim3(groups_e), blk, 0, st, v);
-                hipLaunchKernelGGL(finish_rz, dim3(1), dim3(kFinalThreads), 0, st, v, groups_e);
-                hipLaunchKernelGGL(pcg_direction, dim3(groups_e), blk, 0, st, v);
-            }
+            if (opt.use_graph) HIPCHECK(hipGraphLaunch(batch_graph(batch), st));
+            else inner_launches(batch);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(&state, pcg.p, sizeof state, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
             if (!state.active) break;
             if (elapsed() >= opt.time_limit) break;  // the reference polls the clock inside PCG too (:258)
         }
-        batch = std::max(2, state.inner + 2);
-        hipLaunchKernelGGL(relax_zy, dim3(ga), blk, 0, st, v);
-        hipLaunchKernelGGL(relax_x, dim3(groups_e), blk, 0, st, v);
+        // the next batch covers this solve's count and two more, in steps of 4
+        // so that few distinct graphs are captured
+        result.inner_iterations += std::size_t(state.inner);
+        batch = std::min(64, (state.inner + 2 + 3) / 4 * 4);
+        if (opt.use_graph) HIPCHECK(hipGraphLaunch(update_graph, st));
+        else update_launches();
         HIPCHECK(hipGetLastError());
     };
 
@@ -929,6 +978,7 @@ extern "C" int hgqp_solve_qp_admm(const hgqp_problem* problem, const hgqp_rows* 
         opt.tolerance = o.tolerance;
         opt.iteration_limit = std::size_t(o.iteration_limit);
         opt.threads = o.threads;
+        opt.use_graph = o.use_graph != 0;
         QuadraticProblem q = to_quadratic_problem(*problem, rows);
         const auto t0 = std::chrono::steady_clock::now();
         QpResult r = solve_qp_admm(q, opt);

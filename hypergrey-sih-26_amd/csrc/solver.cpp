@@ -343,6 +343,7 @@ QpResult solve_model(const ModelView& m, const QpOptions& opt, std::chrono::stea
     bo.momentum = opt.momentum;
     bo.reduce = opt.reduce;
     bo.step_metric = opt.step_metric;
+    bo.use_graph = opt.use_graph ? 1 : 0;
     bo.maximize = m.maximize ? 1 : 0;
     bo.offset = m.offset;
     bo.diagonal_hessian = pb.positive_diagonal ? 1 : 0;
@@ -399,6 +400,7 @@ extern "C" void hgqp_qp_options_default(hgqp_qp_options* o) {
     o->momentum = HGQP_MOMENTUM_REFERENCE;
     o->reduce = HGQP_REDUCE_ORDERED;
     o->step_metric = HGQP_STEP_SCALAR;
+    o->use_graph = 1;
 }
 
 extern "C" int hgqp_server_submit_problem(hgqp_server* server, const hgqp_problem* problem, double tolerance,
@@ -496,6 +498,7 @@ extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options*
         opt.momentum = o.momentum;
         opt.reduce = o.reduce;
         opt.step_metric = o.step_metric;
+        opt.use_graph = o.use_graph != 0;
         QpResult r;
         try {
             r = solve_model(m, opt, t0);

@@ -94,7 +94,7 @@ class Rows(C.Structure):
 class QpOptions(C.Structure):
     _fields_ = [("time_limit", C.c_double), ("tolerance", C.c_double), ("iteration_limit", C.c_uint64),
                 ("threads", C.c_int32), ("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
-                ("step_metric", C.c_int32), ("reserved", C.c_int32)]
+                ("step_metric", C.c_int32), ("use_graph", C.c_int32)]
 
 
 class QpResult(C.Structure):
@@ -102,7 +102,7 @@ class QpResult(C.Structure):
                 ("primal_residual", C.c_double), ("stationarity_residual", C.c_double),
                 ("complementarity_residual", C.c_double), ("dual_bound", C.c_double), ("relative_gap", C.c_double),
                 ("iterations", C.c_uint64), ("optimality_verified", C.c_int32), ("restarts", C.c_int32),
-                ("fp32_iterations", C.c_uint64), ("solve_seconds", C.c_double)]
+                ("fp32_iterations", C.c_uint64), ("solve_seconds", C.c_double), ("inner_iterations", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -61,6 +61,7 @@ struct QpResult {
     // backend extras
     std::size_t fp32_iterations = 0;
     int restarts = 0;
+    std::size_t inner_iterations = 0;  // ADMM: PCG iterations of the solve
 };
 
 // include/qp.hpp:24-29 plus the backend's iteration arithmetic.
@@ -73,6 +74,7 @@ struct QpOptions {
     int momentum = HGQP_MOMENTUM_REFERENCE;
     int reduce = HGQP_REDUCE_ORDERED;
     int step_metric = HGQP_STEP_SCALAR;
+    bool use_graph = true;  // replay captured HIP graphs (false: direct launches, same results)
 };
 
 // Mirrored, merged CSR (include/sparse_matrix.hpp:5-17): per row, duplicate

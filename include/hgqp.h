@@ -222,7 +222,8 @@ typedef struct hgqp_qp_options {
     int32_t momentum;         /* HGQP_MOMENTUM_*                                  */
     int32_t reduce;           /* HGQP_REDUCE_*                                    */
     int32_t step_metric;      /* HGQP_STEP_* (default SCALAR)                     */
-    int32_t reserved;
+    int32_t use_graph;        /* 1 (default): replay captured HIP graphs; 0: launch
+                                 each kernel directly (same results)              */
 } hgqp_qp_options;
 
 void hgqp_qp_options_default(hgqp_qp_options* options);
@@ -241,6 +242,7 @@ typedef struct hgqp_qp_result {
     int32_t restarts;
     uint64_t fp32_iterations;
     double solve_seconds;
+    uint64_t inner_iterations;  /* hgqp_solve_qp_admm: PCG iterations of the solve  */
 } hgqp_qp_result;
 
 /* Replaces solve_accelerated_box(q, options, cuda=true)

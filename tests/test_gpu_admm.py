@@ -101,3 +101,14 @@ def test_admm_iteration_limit(hg):
     q, rows = rows_qp(3000, 200, 600, 3)
     r = solve(hg, q, rows, iteration_limit=30, time_limit=120, tolerance=1e-12)
     assert r["status"] == "iteration_limit" and r["iterations"] == 30 and not r["optimality_verified"]
+
+
+def test_admm_graphs_change_nothing(hg):
+    """Graph-replayed and directly launched iterations give the same solve, bit
+    for bit (each launch decides on the device whether it has work)."""
+    q, rows = rows_qp(3000, 200, 600, 3)
+    a = solve(hg, q, rows, iteration_limit=20000, time_limit=120, use_graph=1)
+    b = solve(hg, q, rows, iteration_limit=20000, time_limit=120, use_graph=0)
+    assert a["status"] == b["status"] == "optimal"
+    assert a["iterations"] == b["iterations"] and a["inner_iterations"] == b["inner_iterations"]
+    assert np.array_equal(a["primal"], b["primal"]) and np.array_equal(a["row_dual"], b["row_dual"])
