@@ -488,7 +488,7 @@ def admm_workload(args):
         "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "PCG inner iterations (rho_a, h_product, kd_product, pcg_update, pcg_direction): "
+                     "kernel": "PCG iterations (operator_half, kd_product, pcg_update, pcg_direction): "
                                "algorithmic bytes of all of them over the whole solve time",
                      "algorithmic_bytes_per_pcg_iteration": pcg_bytes, "pcg_iterations": inner},
         "iterations": int(r["iterations"]), "objective": r["objective"], "all_verified": ok,

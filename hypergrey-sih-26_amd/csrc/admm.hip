@@ -59,10 +59,15 @@ struct DevCsr {
     const double* val;
 };
 
-// PCG state, in device memory.
+// PCG state, in device memory. rz is double-buffered by the parity of the
+// iteration within a batch (batches have an even length, so every batch
+// starts at slot 0): an iteration reads rz[slot] and its last kernel writes
+// rz[slot ^ 1]. `done` only ever goes from 0 to 1 during a solve, so a
+// workgroup that sees it set by a sibling can only skip work that no longer
+// matters.
 struct Pcg {
-    double rz, alpha, beta, rnorm, accuracy;
-    int32_t active, inner, max_inner, pad;
+    double rz[2], rnorm, accuracy;
+    int32_t done, broken, inner, max_inner;
 };
 
 struct View {
@@ -80,7 +85,6 @@ struct View {
 
 int groups_of(const DevCsr& m) { return groups_for(m.nblocks); }
 
-__device__ inline bool skip(const View& v, int guarded) { return guarded && !v.s->active; }
 
 // Workgroup partial: dd sum of `sum` and two maxima, in wave then workgroup order.
 __device__ inline void write_partial(double* out, DD sum, double m0, double m1) {
@@ -111,13 +115,15 @@ __device__ inline void write_partial(double* out, DD sum, double m0, double m1) 
     }
 }
 
-// Combines `groups` partials in workgroup order (one workgroup of kFinalThreads).
+// Combines `groups` partials in a fixed order by one workgroup of THREADS
+// threads; every workgroup that calls it on the same partials gets the same bits.
+template <int THREADS = kFinalThreads>
 __device__ inline void combine(const double* part, int groups, DD* sum, double* m0, double* m1) {
-    __shared__ double red[kFinalThreads / kWave][kPart];
+    __shared__ double red[THREADS / kWave][kPart];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     DD s{0, 0};
     double a = 0, b = 0;
-    for (int g = threadIdx.x; g < groups; g += kFinalThreads) {
+    for (int g = threadIdx.x; g < groups; g += THREADS) {
         s = dd_add(s, DD{part[g * kPart], part[g * kPart + 1]});
         a = fmax(a, part[g * kPart + 2]);
         b = fmax(b, part[g * kPart + 3]);
@@ -134,7 +140,7 @@ __device__ inline void combine(const double* part, int groups, DD* sum, double* 
     __syncthreads();
     DD t{red[0][0], red[0][1]};
     double ta = red[0][2], tb = red[0][3];
-    for (int w = 1; w < kFinalThreads / kWave; ++w) {
+    for (int w = 1; w < THREADS / kWave; ++w) {
         t = dd_add(t, DD{red[w][0], red[w][1]});
         ta = fmax(ta, red[w][2]);
         tb = fmax(tb, red[w][3]);
@@ -144,12 +150,17 @@ __device__ inline void combine(const double* part, int groups, DD* sum, double* 
     *m1 = tb;
 }
 
-// One ORDERED row pass of m with gather vector `gat`; epi(row, sum).
+// One ORDERED row pass of m with gather vector `gat` by workgroups wg of nwg
+// (the launch's whole grid by default); epi(row, sum).
 template <class Epi>
-__device__ inline void pass(const DevCsr& m, const double* gat, Epi& epi) {
+__device__ inline void pass(const DevCsr& m, const double* gat, Epi& epi, int wg = -1, int nwg = 0) {
     __shared__ double stage[kWavesPerGroup][kCap];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+    if (wg < 0) {
+        wg = blockIdx.x;
+        nwg = gridDim.x;
+    }
+    for (int b = wg * kWavesPerGroup + wave; b < m.nblocks; b += nwg * kWavesPerGroup)
         dev::row_block<double, true, kLoads>(m.start, m.col, m.val, gat, m.blk[b], m.blk[b + 1], stage[wave], lane,
                                              epi);
 }
@@ -229,20 +240,26 @@ struct RhoA {
         v.t[i] = s * v.rho[i];
     }
 };
-__global__ __launch_bounds__(kGroup) void rho_a(View v, const double* g, int guarded) {
-    if (skip(v, guarded)) return;
-    RhoA e{v};
-    pass(v.A, g, e);
-}
-
 struct HProduct {
     View v;
     __device__ void operator()(int j, double s) { v.hv[j] = s; }
 };
-__global__ __launch_bounds__(kGroup) void h_product(View v, const double* g, int guarded) {
-    if (skip(v, guarded)) return;
+__global__ __launch_bounds__(kGroup) void h_product(View v, const double* g) {
     HProduct e{v};
     pass(v.H, g, e);
+}
+
+// The two independent products of the system operator in one launch:
+// workgroups [0, groups_a) do t = rho (A g), the rest hv = H g.
+__global__ __launch_bounds__(kGroup) void operator_half(View v, const double* g, int guarded, int groups_a) {
+    if (guarded && v.s->done) return;
+    if (int(blockIdx.x) < groups_a) {
+        RhoA e{v};
+        pass(v.A, g, e, blockIdx.x, groups_a);
+    } else {
+        HProduct e{v};
+        pass(v.H, g, e, blockIdx.x - groups_a, gridDim.x - groups_a);
+    }
 }
 
 // sys = H next + sigma next + A' t; r = rhs - sys; pre = r / P; d = pre;
@@ -278,11 +295,12 @@ __global__ __launch_bounds__(kFinalThreads) void finish_prep(View v, int groups_
     combine(v.part2, groups_n, &unused, &rhs_norm, &u1);
     if (threadIdx.x != 0) return;
     Pcg* s = v.s;
-    s->rz = rz.hi + rz.lo;
+    s->rz[0] = rz.hi + rz.lo;
     s->rnorm = rn;
     s->accuracy = factor * (1 + rhs_norm);
     s->inner = 0;
-    s->active = s->max_inner > 0 && rn > s->accuracy;
+    s->broken = 0;
+    s->done = !(s->max_inner > 0 && rn > s->accuracy);
 }
 
 // kd = H d + sigma d + A' t; partial dd(d . kd) (qp_admm.cpp:254)
@@ -297,29 +315,31 @@ struct Kd {
     }
 };
 __global__ __launch_bounds__(kGroup) void kd_product(View v) {
-    if (!v.s->active) return;
+    if (v.s->done) return;
     Kd e{v};
     pass(v.At, v.t, e);
     write_partial(v.part, e.dot, 0, 0);
 }
 
-// alpha = rz / (d . kd), or stop on a non-positive or non-finite denominator
-__global__ __launch_bounds__(kFinalThreads) void finish_den(View v, int groups_n) {
-    if (!v.s->active) return;
+// alpha = rz / (d . kd), the same in every workgroup (each combines the
+// d . kd partials in one fixed order), or stop on a non-positive or
+// non-finite denominator (qp_admm.cpp:254); then next += alpha d,
+// r -= alpha kd, pre = r / P and the partials of r . pre and max |r| (:255).
+__global__ __launch_bounds__(kGroup) void pcg_update(View v, int slot, int groups_kd) {
+#pragma clang fp contract(off)
+    if (v.s->done) return;
     DD den;
     double u0, u1;
-    combine(v.part, groups_n, &den, &u0, &u1);
-    if (threadIdx.x != 0) return;
+    combine<kGroup>(v.part, groups_kd, &den, &u0, &u1);
     const double dn = den.hi + den.lo;
-    if (!(dn > 0) || !isfinite(dn)) v.s->active = 0;
-    else v.s->alpha = v.s->rz / dn;
-}
-
-// next += alpha d; r -= alpha kd; pre = r / P; partial dd(r . pre), max |r| (:255)
-__global__ __launch_bounds__(kGroup) void pcg_update(View v) {
-#pragma clang fp contract(off)
-    if (!v.s->active) return;
-    const double alpha = v.s->alpha;
+    if (!(dn > 0) || !isfinite(dn)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            v.s->broken = 1;
+            v.s->done = 1;
+        }
+        return;
+    }
+    const double alpha = v.s->rz[slot] / dn;
     DD dot{0, 0};
     double mx = 0;
     for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < v.n; j += int64_t(gridDim.x) * blockDim.x) {
@@ -331,32 +351,29 @@ __global__ __launch_bounds__(kGroup) void pcg_update(View v) {
         dot = dd_add(dot, dd_prod(r, p));
         mx = fmax(mx, fabs(r));
     }
-    write_partial(v.part, dot, mx, 0);
+    write_partial(v.part2, dot, mx, 0);
 }
 
-// beta = (r . pre) / rz; the loop goes on while |r| > accuracy (:251, 256)
-__global__ __launch_bounds__(kFinalThreads) void finish_rz(View v, int groups_e) {
-    if (!v.s->active) return;
+// beta = (r . pre) / rz and d = pre + beta d (:256-257), beta the same in
+// every workgroup; workgroup 0 then records rz, |r| and the iteration and ends
+// the solve once |r| <= accuracy or the iteration cap is reached (:251).
+__global__ __launch_bounds__(kGroup) void pcg_direction(View v, int slot, int groups_e) {
+#pragma clang fp contract(off)
+    if (v.s->done) return;
     DD rz;
     double rn, u1;
-    combine(v.part, groups_e, &rz, &rn, &u1);
-    if (threadIdx.x != 0) return;
-    Pcg* s = v.s;
+    combine<kGroup>(v.part2, groups_e, &rz, &rn, &u1);
     const double next_rz = rz.hi + rz.lo;
-    s->beta = next_rz / s->rz;
-    s->rz = next_rz;
-    s->rnorm = rn;
-    s->inner += 1;
-    s->active = s->inner < s->max_inner && rn > s->accuracy;
-}
-
-// d = pre + beta d (:257)
-__global__ void pcg_direction(View v) {
-#pragma clang fp contract(off)
-    if (!v.s->active) return;
-    const double beta = v.s->beta;
+    const double beta = next_rz / v.s->rz[slot];
     for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < v.n; j += int64_t(gridDim.x) * blockDim.x)
         v.d[j] = v.pre[j] + beta * v.d[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Pcg* s = v.s;
+        s->rz[slot ^ 1] = next_rz;
+        s->rnorm = rn;
+        s->inner += 1;
+        if (!(s->inner < s->max_inner && rn > s->accuracy)) s->done = 1;
+    }
 }
 
 // relaxed z, y update from A next (qp_admm.cpp:260-262)
@@ -846,7 +863,7 @@ QpResult solve_qp_admm(const QuadraticProblem& original, const QpOptions& opt) {
         hipLaunchKernelGGL(norms_a, dim3(ga), blk, 0, st, v);
         const double primal_abs = partial_max(v.part, ga, 2), nax = partial_max(v.part, ga, 3);
         const double nz = partial_max(v.part2, ga, 2);
-        hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.x, 0);
+        hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.x);
         hipLaunchKernelGGL(norms_at, dim3(gat), blk, 0, st, v);
         HIPCHECK(hipGetLastError());
         const double dual_abs = partial_max(v.part, gat, 2), nhx = partial_max(v.part, gat, 3);
@@ -868,19 +885,16 @@ QpResult solve_qp_admm(const QuadraticProblem& original, const QpOptions& opt) {
     auto prep_launches = [&] {
         hipLaunchKernelGGL(make_yz, dim3(grid_elem(int64_t(M))), blk, 0, st, v);
         hipLaunchKernelGGL(make_rhs, dim3(gat), blk, 0, st, v);
-        hipLaunchKernelGGL(rho_a, dim3(ga), blk, 0, st, v, (const double*)v.next, 0);
-        hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.next, 0);
+        hipLaunchKernelGGL(operator_half, dim3(ga + gh), blk, 0, st, v, (const double*)v.next, 0, ga);
         hipLaunchKernelGGL(sys_prep, dim3(gat), blk, 0, st, v);
     };
+    // `count` is even: iteration k of a batch uses rz slot k & 1
     auto inner_launches = [&](int count) {
         for (int k = 0; k < count; ++k) {
-            hipLaunchKernelGGL(rho_a, dim3(ga), blk, 0, st, v, (const double*)v.d, 1);
-            hipLaunchKernelGGL(h_product, dim3(gh), blk, 0, st, v, (const double*)v.d, 1);
+            hipLaunchKernelGGL(operator_half, dim3(ga + gh), blk, 0, st, v, (const double*)v.d, 1, ga);
             hipLaunchKernelGGL(kd_product, dim3(gat), blk, 0, st, v);
-            hipLaunchKernelGGL(finish_den, dim3(1), dim3(kFinalThreads), 0, st, v, gat);
-            hipLaunchKernelGGL(pcg_update, dim3(groups_e), blk, 0, st, v);
-            hipLaunchKernelGGL(finish_rz, dim3(1), dim3(kFinalThreads), 0, st, v, groups_e);
-            hipLaunchKernelGGL(pcg_direction, dim3(groups_e), blk, 0, st, v);
+            hipLaunchKernelGGL(pcg_update, dim3(groups_e), blk, 0, st, v, k & 1, gat);
+            hipLaunchKernelGGL(pcg_direction, dim3(groups_e), blk, 0, st, v, k & 1, groups_e);
         }
     };
     auto update_launches = [&] {
@@ -922,7 +936,7 @@ QpResult solve_qp_admm(const QuadraticProblem& original, const QpOptions& opt) {
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(&state, pcg.p, sizeof state, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
-            if (!state.active) break;
+            if (state.done) break;
             if (elapsed() >= opt.time_limit) break;  // the reference polls the clock inside PCG too (:258)
         }
         // the next batch covers this solve's count and two more, in steps of 4
