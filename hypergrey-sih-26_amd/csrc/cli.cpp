@@ -1,5 +1,6 @@
-// hgqp_cli: the QPLIB box-QP branch of the reference's solver_cli
-// (02-software/solver/src/main.cpp:52-163) with the HIP engine:
+// hgqp_cli: the QPLIB branch of the reference's solver_cli
+// (02-software/solver/src/main.cpp:52-163) with the HIP engines: box QPs on
+// the accelerated box path, models with linear rows on the ADMM (main.cpp:145):
 //
 //   hgqp_cli MODEL.qplib [--json] [--backend hip] [--threads N] [--tolerance T]
 //            [--iteration-limit N] [--time-limit S] [--precision fp64|fp32-polish|fp32-refine]
@@ -107,15 +108,15 @@ int main(int argc, char** argv) {
         std::ifstream input(path);
         if (!input) throw std::runtime_error("Cannot open model");
         if (path.size() < 6 || path.substr(path.size() - 6) != ".qplib")
-            throw std::runtime_error("hgqp_cli reads QPLIB box QPs only");
+            throw std::runtime_error("hgqp_cli reads QPLIB models only");
         auto problem = hgqp::read_qplib(input);
         qo.time_limit = time_limit;
         qo.threads = threads;
         if (custom_iterations) qo.iteration_limit = iteration_limit;
         if (custom_tolerance) qo.tolerance = tolerance;
-        if (!problem.linear.rows.empty())
-            throw std::runtime_error("HIP backend supports convex box QPs; use CPU for linearly constrained QP");
-        const auto result = hgqp::solve_accelerated_box(problem, qo, backend);
+        // main.cpp:145: box QPs take the accelerated box path, models with rows the ADMM
+        const auto result = problem.linear.rows.empty() ? hgqp::solve_accelerated_box(problem, qo, backend)
+                                                        : hgqp::solve_qp_admm(problem, qo);
         const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - total_start).count();
         std::cout << std::setprecision(17);
         if (json) {

@@ -163,6 +163,8 @@ struct hgqp_qplib {
     QuadraticProblem q;
     std::vector<int64_t> hrow, hcol;
     std::vector<double> hval;
+    std::vector<int64_t> rstart, rcol;  // the linear rows as CSR
+    std::vector<double> rval;
 };
 
 extern "C" int hgqp_qplib_parse(const char* text, size_t length, hgqp_qplib** out) {
@@ -182,6 +184,14 @@ extern "C" int hgqp_qplib_parse(const char* text, size_t length, hgqp_qplib** ou
             h->hrow.push_back(int64_t(t.row));
             h->hcol.push_back(int64_t(t.col));
             h->hval.push_back(t.value);
+        }
+        h->rstart.push_back(0);
+        for (const auto& row : h->q.linear.rows) {
+            for (const auto& a : row) {
+                h->rcol.push_back(int64_t(a.first));
+                h->rval.push_back(a.second);
+            }
+            h->rstart.push_back(int64_t(h->rcol.size()));
         }
         *out = h.release();
     });
@@ -203,6 +213,19 @@ extern "C" int hgqp_qplib_problem(const hgqp_qplib* model, hgqp_problem* view, i
         view->offset = p.offset;
         view->maximize = p.maximize ? 1 : 0;
         if (rows) *rows = int64_t(p.rows.size());
+    });
+}
+
+extern "C" int hgqp_qplib_rows(const hgqp_qplib* model, hgqp_rows* rows) {
+    return guarded([&] {
+        if (!model || !rows) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_qplib_rows: null argument");
+        const auto& p = model->q.linear;
+        rows->m = int64_t(p.rows.size());
+        rows->start = model->rstart.data();
+        rows->col = model->rcol.data();
+        rows->val = model->rval.data();
+        rows->row_lower = p.row_lower.data();
+        rows->row_upper = p.row_upper.data();
     });
 }
 

@@ -194,6 +194,7 @@ SIGNATURES = {
     "hgqp_dense_destroy": ([C.c_void_p], C.c_int),
     "hgqp_qplib_parse": ([C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)], C.c_int),
     "hgqp_qplib_problem": ([C.c_void_p, C.POINTER(Problem), P_I64], C.c_int),
+    "hgqp_qplib_rows": ([C.c_void_p, C.POINTER(Rows)], C.c_int),
     "hgqp_qplib_free": ([C.c_void_p], C.c_int),
     "hgqp_pool_options_default": ([C.POINTER(PoolOptions)], None),
     "hgqp_pool_bytes": ([C.POINTER(PoolOptions), P_I64], C.c_int),
@@ -427,7 +428,9 @@ def csr_assemble(n, hrow, hcol, hval, cost, lower, upper, maximize=False):
 
 def read_qplib(text):
     """Parses QPLIB text (hgqp_qplib_parse) into a dict of numpy arrays in the
-    triplet form of solve_qp, plus 'rows' (linear rows; the box engine needs 0)."""
+    triplet form of solve_qp, plus 'rows' (the number of linear rows; the box
+    engine needs 0) and the rows as CSR (rstart, rcol, rval, row_lower,
+    row_upper), the form of solve_qp_admm."""
     if isinstance(text, str):
         text = text.encode()
     h = C.c_void_p()
@@ -437,10 +440,17 @@ def read_qplib(text):
         _raise(lib().hgqp_qplib_problem(h, C.byref(view), C.byref(rows)))
         n, m = view.n, view.nterms
         arr = lambda p, k, t: np.ctypeslib.as_array(p, shape=(k,)).astype(t, copy=True) if k else np.zeros(0, t)
-        return dict(n=n, hrow=arr(view.hrow, m, np.int64), hcol=arr(view.hcol, m, np.int64),
-                    hval=arr(view.hval, m, np.float64), cost=arr(view.cost, n, np.float64),
-                    lower=arr(view.lower, n, np.float64), upper=arr(view.upper, n, np.float64),
-                    offset=view.offset, maximize=bool(view.maximize), rows=rows.value)
+        out = dict(n=n, hrow=arr(view.hrow, m, np.int64), hcol=arr(view.hcol, m, np.int64),
+                   hval=arr(view.hval, m, np.float64), cost=arr(view.cost, n, np.float64),
+                   lower=arr(view.lower, n, np.float64), upper=arr(view.upper, n, np.float64),
+                   offset=view.offset, maximize=bool(view.maximize), rows=rows.value)
+        rv = Rows()
+        _raise(lib().hgqp_qplib_rows(h, C.byref(rv)))
+        k = int(rv.start[rv.m]) if rv.m else 0
+        out.update(rstart=arr(rv.start, rv.m + 1, np.int64), rcol=arr(rv.col, k, np.int64),
+                   rval=arr(rv.val, k, np.float64), row_lower=arr(rv.row_lower, rv.m, np.float64),
+                   row_upper=arr(rv.row_upper, rv.m, np.float64))
+        return out
     finally:
         lib().hgqp_qplib_free(h)
 

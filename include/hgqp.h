@@ -348,10 +348,14 @@ int hgqp_dense_destroy(hgqp_dense* dense);
  * the reference reader's rules and messages (read_qplib, qp.cpp:67-117); a
  * malformed model returns HGQP_INVALID_ARGUMENT. hgqp_qplib_problem fills a
  * view whose arrays live until hgqp_qplib_free; *rows receives the number of
- * linear rows (the box engine requires 0). */
+ * linear rows (0 for the box engine; a model with rows goes to
+ * hgqp_solve_qp_admm). */
 typedef struct hgqp_qplib hgqp_qplib;
 int hgqp_qplib_parse(const char* text, size_t length, hgqp_qplib** out);
 int hgqp_qplib_problem(const hgqp_qplib* model, hgqp_problem* view, int64_t* rows);
+/* The model's linear rows as a CSR view (arrays live until hgqp_qplib_free),
+ * the input of hgqp_solve_qp_admm. */
+int hgqp_qplib_rows(const hgqp_qplib* model, hgqp_rows* rows);
 int hgqp_qplib_free(hgqp_qplib* model);
 
 /* ---- instance pool: many box QPs resident at once ---------------------- */

@@ -57,6 +57,16 @@ class StandIn:
                                               primal_residual=0.0, stationarity_residual=1e-9,
                                               complementarity_residual=0.0, primal=[0.0] * n), 0.01, "stand-in")
 
+    async def solve_rows(self, problem, tolerance, iteration_limit, time_limit):
+        self.calls.append(dict(problem=problem, engine="admm", tolerance=tolerance, iteration_limit=iteration_limit,
+                               time_limit=time_limit))
+        n = problem["n"]
+        return service.result_record(n, dict(status="optimal", objective=self.objective, dual_bound=-math.inf,
+                                              relative_gap=math.nan, iterations=25, optimality_verified=True,
+                                              primal_residual=0.0, stationarity_residual=1e-9,
+                                              complementarity_residual=0.0, primal=[0.0] * n), 0.01, "stand-in",
+                                     rows=problem["rows"])
+
     def close(self):
         pass
 
@@ -139,14 +149,41 @@ def test_nonfinite_input_gets_a_json_safe_422(client, value):
     json.dumps(r.json(), allow_nan=False)
 
 
-def test_unreadable_or_unsupported_models_are_records(client, engine):
+# min 1/2 (x1^2 + x2^2)  s.t.  x1 + x2 = 1,  -10 <= x <= 10  (x* = (1/2, 1/2), objective 1/4)
+QCL = """twovar QCL minimize
+2
+1
+2
+1 1 1.0
+2 2 1.0
+0 0
+0
+2
+1 1 1.0
+1 2 1.0
+1e30
+1 0
+1 0
+-10 0
+10 0
+0 0
+0 0
+0 0
+0
+0
+"""
+
+
+def test_unreadable_models_are_records_and_rows_go_to_admm(client, engine):
     bad = client.post("/qp/solve", json={"model": "8991 QCB minimize\n2\n"}).json()
     assert bad["status"] == "input_rejected" and bad["primal"] == [] and bad["objective"] is None
-    # a continuous QP with linear rows (type QCL) is not a box QP
-    qcl = "m QCL minimize\n1\n1\n1\n1 1 2.0\n0\n0\n0\n0\n1\n1 1 1.0\n1e30\n-1\n0\n1\n0\n-1e30\n0\n1e30\n0\n0\n0\n0\n0\n0\n0\n"
-    rec = client.post("/qp/solve", json={"model": qcl}).json()
-    assert rec["status"] == "input_rejected" and "box" in rec["message"]
     assert not engine.calls
+    # a continuous QP with linear rows (type QCL) takes the ADMM path
+    rec = client.post("/qp/solve", json={"model": QCL, "engine": "served"}).json()
+    assert rec["status"] == "optimal" and rec["rows"] == 1
+    call = engine.calls[-1]
+    assert call["engine"] == "admm" and call["problem"]["rows"] == 1
+    assert list(call["problem"]["rstart"]) == [0, 2] and list(call["problem"]["rcol"]) == [0, 1]
 
 
 def test_dimension_limits_are_records(engine):
