@@ -126,3 +126,20 @@ def test_cli_exit_codes(tmp_path):
     assert out.returncode == 2 and out.stderr.startswith("error: ")
     out = subprocess.run([CLI, str(path), "--threads", "0"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 2
+
+
+@pytest.mark.gpu
+def test_cli_sends_linear_rows_to_admm(tmp_path):
+    """main.cpp:145: a model with rows goes to solve_qp_admm; the JSON carries
+    the row duals. min 1/2 (x1^2 + x2^2) s.t. x1 + x2 = 1: x* = (1/2, 1/2), y = -1/2."""
+    from test_service import QCL
+    path = tmp_path / "twovar.qplib"
+    path.write_text(QCL)
+    out = subprocess.run([CLI, str(path), "--json", "--tolerance", "1e-9"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout)
+    assert r["status"] == "optimal" and r["algorithm"] == "hip_admm_pcg" and r["rows"] == 1
+    assert abs(r["objective"] - 0.25) <= 1e-8
+    assert np.abs(np.array(r["primal"]) - 0.5).max() <= 1e-6
+    assert abs(r["row_dual"][0] + 0.5) <= 1e-6
