@@ -91,3 +91,28 @@ def rows_qp(n, m_eq, m_ineq, seed, row_nnz=6, free=0.1):
     cost = rng.normal(0, 1, size=n)
     q = BoxQP(n, hr, hc, hv, cost, lower, upper)
     return q, Rows(start, col, val, rl, ru)
+
+
+# min 1/2 (x1^2 + x2^2)  s.t.  x1 + x2 = 1,  -10 <= x <= 10  (x* = (1/2, 1/2), objective 1/4)
+QCL = """twovar QCL minimize
+2
+1
+2
+1 1 1.0
+2 2 1.0
+0 0
+0
+2
+1 1 1.0
+1 2 1.0
+1e30
+1 0
+1 0
+-10 0
+10 0
+0 0
+0 0
+0 0
+0
+0
+"""

@@ -102,7 +102,7 @@ def test_concurrent_requests_equal_alone(client, hg):
 def test_linear_rows_take_the_admm_path(client, hg):
     """A QPLIB model with linear rows (type QCL) goes to hgqp_solve_qp_admm:
     min 1/2 (x1^2 + x2^2) s.t. x1 + x2 = 1 has x* = (1/2, 1/2), objective 1/4."""
-    from test_service import QCL
+    from helpers import QCL
     print("case: QCL two-variable model", flush=True)
     got = client.post("/qp/solve", json={"model": QCL, "tolerance": 1e-9}).json()
     assert got["status"] == "optimal" and got["optimality_verified"] and got["rows"] == 1

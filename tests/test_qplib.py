@@ -132,7 +132,7 @@ def test_cli_exit_codes(tmp_path):
 def test_cli_sends_linear_rows_to_admm(tmp_path):
     """main.cpp:145: a model with rows goes to solve_qp_admm; the JSON carries
     the row duals. min 1/2 (x1^2 + x2^2) s.t. x1 + x2 = 1: x* = (1/2, 1/2), y = -1/2."""
-    from test_service import QCL
+    from helpers import QCL
     path = tmp_path / "twovar.qplib"
     path.write_text(QCL)
     out = subprocess.run([CLI, str(path), "--json", "--tolerance", "1e-9"], capture_output=True, text=True,

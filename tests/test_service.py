@@ -20,6 +20,7 @@ fastapi = pytest.importorskip("fastapi")
 from fastapi.testclient import TestClient  # noqa: E402
 
 import service  # noqa: E402
+from helpers import QCL  # noqa: E402
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -149,29 +150,6 @@ def test_nonfinite_input_gets_a_json_safe_422(client, value):
     json.dumps(r.json(), allow_nan=False)
 
 
-# min 1/2 (x1^2 + x2^2)  s.t.  x1 + x2 = 1,  -10 <= x <= 10  (x* = (1/2, 1/2), objective 1/4)
-QCL = """twovar QCL minimize
-2
-1
-2
-1 1 1.0
-2 2 1.0
-0 0
-0
-2
-1 1 1.0
-1 2 1.0
-1e30
-1 0
-1 0
--10 0
-10 0
-0 0
-0 0
-0 0
-0
-0
-"""
 
 
 def test_unreadable_models_are_records_and_rows_go_to_admm(client, engine):
