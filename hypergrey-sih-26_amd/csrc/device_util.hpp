@@ -37,6 +37,13 @@ struct Arena {
     }
 };
 
+// Host-to-device copy of caller memory through two pinned staging buffers
+// (process-wide, 2 x 64 MiB, allocated on first use and held until exit):
+// host threads fill one buffer while the DMA engine drains the other. Small
+// copies go straight through hipMemcpyAsync. Returns once `src` has been read;
+// the copy completes in stream order on `stream`.
+void upload(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 // A device allocation owned by a scope (temporaries of handle creation).
 template <class T>
 struct DeviceTemp {

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -46,6 +47,48 @@ void require_device() {
     HIPCHECK(hipGetDeviceProperties(&prop, dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         throw Error(HGQP_NO_DEVICE, std::string("hgqp is built for gfx950; device is ") + prop.gcnArchName);
+}
+
+namespace {
+
+struct Staging {
+    static constexpr size_t kChunk = size_t(64) << 20;
+    std::mutex lock;
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t drained[2] = {nullptr, nullptr};
+};
+
+Staging& staging() {
+    static Staging* s = new Staging;  // held until the process ends
+    return *s;
+}
+
+}  // namespace
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+    if (bytes < (size_t(16) << 20)) {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        return;
+    }
+    Staging& s = staging();
+    std::lock_guard<std::mutex> hold(s.lock);
+    for (int b = 0; b < 2; ++b) {
+        if (!s.buf[b]) HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&s.buf[b]), Staging::kChunk, hipHostMallocDefault));
+        if (!s.drained[b]) HIPCHECK(hipEventCreateWithFlags(&s.drained[b], hipEventDisableTiming));
+    }
+    const int threads = host_threads();
+    int b = 0;
+    for (size_t off = 0; off < bytes; off += Staging::kChunk, b ^= 1) {
+        const size_t len = std::min(Staging::kChunk, bytes - off);
+        HIPCHECK(hipEventSynchronize(s.drained[b]));  // the previous copy out of this buffer is done
+        const char* from = static_cast<const char*>(src) + off;
+        char* to = s.buf[b];
+        parallel_ranges(int64_t(len), threads, [&](int64_t lo, int64_t hi, int) {
+            std::memcpy(to + lo, from + lo, size_t(hi - lo));
+        });
+        HIPCHECK(hipMemcpyAsync(static_cast<char*>(dst) + off, to, len, hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipEventRecord(s.drained[b], stream));
+    }
 }
 
 std::vector<int> row_blocks(int64_t n, const int32_t* start, int cap, int max_rows) {
@@ -459,10 +502,10 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         tstart.alloc(un + 1);
         tcol.alloc(unnz);
         tval.alloc(unnz);
-        HIPCHECK(hipMemcpyAsync(tstart.p, start, (un + 1) * 4, hipMemcpyHostToDevice, b->stream));
+        upload(tstart.p, start, (un + 1) * 4, b->stream);
         if (nnz) {
-            HIPCHECK(hipMemcpyAsync(tcol.p, index, size_t(nnz) * 4, hipMemcpyHostToDevice, b->stream));
-            HIPCHECK(hipMemcpyAsync(tval.p, value, size_t(nnz) * 8, hipMemcpyHostToDevice, b->stream));
+            upload(tcol.p, index, size_t(nnz) * 4, b->stream);
+            upload(tval.p, value, size_t(nnz) * 8, b->stream);
         }
         dstart = tstart.p;
         dcol = tcol.p;
@@ -546,9 +589,9 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         if (own32) split_panels<float>(n, nnz, p32, dstart, dcol, dval, b->l32.start, b->l32.col, b->val32, b->stream);
         else launch_convert<float, double>(nnz, b->val, b->val32, b->stream);  // round to nearest
     }
-    HIPCHECK(hipMemcpyAsync(b->c, cost, un * 8, hipMemcpyHostToDevice, b->stream));
-    HIPCHECK(hipMemcpyAsync(b->lo, lower, un * 8, hipMemcpyHostToDevice, b->stream));
-    HIPCHECK(hipMemcpyAsync(b->up, upper, un * 8, hipMemcpyHostToDevice, b->stream));
+    upload(b->c, cost, un * 8, b->stream);
+    upload(b->lo, lower, un * 8, b->stream);
+    upload(b->up, upper, un * 8, b->stream);
     if (jacobi) launch_row_abs(n, dstart, dval, lipschitz, b->dstep, b->dstep32, b->stream);  // stored order
     if (f32) {
         launch_convert<float, double>(n, b->c, b->c32, b->stream);
@@ -650,16 +693,25 @@ int hgqp_box_create(int64_t n, int64_t nnz, const int32_t* start, const int32_t*
         if (start[0] != 0 || start[n] != nnz) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: malformed CSR start");
         for (int64_t i = 0; i < n; ++i)
             if (start[i + 1] < start[i]) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: CSR start not monotone");
-        bool diag = true;
-        for (int64_t i = 0; i < n; ++i) {
-            bool has_pos_diag = false;
-            for (int32_t k = start[i]; k < start[i + 1]; ++k) {
-                if (index[k] < 0 || index[k] >= n) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column out of range");
-                if (index[k] == i) has_pos_diag = value[k] > 0;
-                else diag = false;
+        // column range and the positive-diagonal flag, rows split over host threads
+        const int parts = nnz >= (int64_t(1) << 22) ? host_threads() : 1;
+        std::vector<char> diag_part(size_t(parts), 1);
+        parallel_ranges(n, parts, [&](int64_t r0, int64_t r1, int part) {
+            bool d = true;
+            for (int64_t i = r0; i < r1; ++i) {
+                bool has_pos_diag = false;
+                for (int32_t k = start[i]; k < start[i + 1]; ++k) {
+                    if (index[k] < 0 || index[k] >= n)
+                        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column out of range");
+                    if (index[k] == i) has_pos_diag = value[k] > 0;
+                    else d = false;
+                }
+                d = d && has_pos_diag;
             }
-            diag = diag && has_pos_diag;
-        }
+            diag_part[size_t(part)] = d;
+        });
+        bool diag = true;
+        for (char d : diag_part) diag = diag && d;
         for (int64_t j = 0; j < n; ++j)
             if (std::isnan(lower[j]) || std::isnan(upper[j]) || lower[j] > upper[j])
                 throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: bounds must satisfy lower <= upper");

@@ -2,11 +2,13 @@
 #pragma once
 #include "hgqp.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <exception>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace hgqp {
@@ -44,6 +46,34 @@ int guarded(F&& body) {
         set_last_error("unknown error");
         return HGQP_DEVICE_ERROR;
     }
+}
+
+// Host threads for the O(nnz) host passes: at most 16.
+inline int host_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return int(std::max(1u, std::min(16u, h)));
+}
+
+// Runs body(begin, end, part) over `parts` contiguous slices of [0, n), one
+// host thread each (the calling thread takes slice 0); rethrows the first
+// exception of the lowest slice that failed.
+template <class F>
+void parallel_ranges(int64_t n, int parts, F body) {
+    parts = int(std::max<int64_t>(1, std::min<int64_t>(parts, std::max<int64_t>(n, 1))));
+    std::vector<std::exception_ptr> failed(static_cast<size_t>(parts));
+    auto run = [&](int p) {
+        try {
+            body(n * p / parts, n * (p + 1) / parts, p);
+        } catch (...) {
+            failed[size_t(p)] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> workers;
+    for (int p = 1; p < parts; ++p) workers.emplace_back(run, p);
+    run(0);
+    for (auto& w : workers) w.join();
+    for (auto& e : failed)
+        if (e) std::rethrow_exception(e);
 }
 
 // Row blocks of a CSR (host side): consecutive rows with at most `cap` stored

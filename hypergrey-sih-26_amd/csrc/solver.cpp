@@ -306,16 +306,16 @@ bool prepare_model(const ModelView& m, const QpOptions& opt, QpResult& r, Prepar
     if (m.terms) {  // QuadraticTerm records: one upload, strided view
         buf.reset(new DeviceBytes(m.nterms * sizeof(QuadraticTerm)));
         if (m.nterms)
-            HIPCHECK(hipMemcpyAsync(buf->p, m.terms, m.nterms * sizeof(QuadraticTerm), hipMemcpyHostToDevice, st.s));
+            upload(buf->p, m.terms, m.nterms * sizeof(QuadraticTerm), st.s);
         const int64_t* base = static_cast<const int64_t*>(buf->p);
         tv = TermView{base, base + 1, reinterpret_cast<const double*>(base + 2), 3};
     } else {
         buf.reset(new DeviceBytes(m.nterms * 24));
         int64_t* base = static_cast<int64_t*>(buf->p);
         if (m.nterms) {
-            HIPCHECK(hipMemcpyAsync(base, m.hrow, m.nterms * 8, hipMemcpyHostToDevice, st.s));
-            HIPCHECK(hipMemcpyAsync(base + m.nterms, m.hcol, m.nterms * 8, hipMemcpyHostToDevice, st.s));
-            HIPCHECK(hipMemcpyAsync(base + 2 * m.nterms, m.hval, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+            upload(base, m.hrow, m.nterms * 8, st.s);
+            upload(base + m.nterms, m.hcol, m.nterms * 8, st.s);
+            upload(base + 2 * m.nterms, m.hval, m.nterms * 8, st.s);
         }
         tv = TermView{base, base + m.nterms, reinterpret_cast<const double*>(base + 2 * m.nterms), 1};
     }
@@ -459,9 +459,9 @@ extern "C" int hgqp_csr_assemble(const hgqp_problem* problem, int64_t capacity, 
         DeviceBytes buf(m.nterms * 24);
         int64_t* base = static_cast<int64_t*>(buf.p);
         if (m.nterms) {
-            HIPCHECK(hipMemcpyAsync(base, m.hrow, m.nterms * 8, hipMemcpyHostToDevice, st.s));
-            HIPCHECK(hipMemcpyAsync(base + m.nterms, m.hcol, m.nterms * 8, hipMemcpyHostToDevice, st.s));
-            HIPCHECK(hipMemcpyAsync(base + 2 * m.nterms, m.hval, m.nterms * 8, hipMemcpyHostToDevice, st.s));
+            upload(base, m.hrow, m.nterms * 8, st.s);
+            upload(base + m.nterms, m.hcol, m.nterms * 8, st.s);
+            upload(base + 2 * m.nterms, m.hval, m.nterms * 8, st.s);
         }
         const TermView tv{base, base + m.nterms, reinterpret_cast<const double*>(base + 2 * m.nterms), 1};
         DeviceCsr csr;
