@@ -461,7 +461,6 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     const int32_t* start = src.start;
     const int32_t* index = src.index;
     const double* value = src.value;
-    require_device();
     hgqp_box_options opt;
     if (options) opt = *options;
     else hgqp_box_options_default(&opt);
@@ -476,6 +475,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
     if (opt.column_panels < 0 || opt.column_panels > kMaxPanels)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column_panels must be in [0, 16]");
+    require_device();
 
     auto b = std::make_unique<hgqp_box>();
     b->n = n;

@@ -224,3 +224,30 @@ def test_full_size_panels_chosen_and_bit_exact(hg, oracle):
     assert rep["status"] == "optimal" and rep["optimality_verified"]
     assert rep["stationarity_residual"] <= KKT_TOL
     assert np.abs(x - d["optimum"]).max() < 1e-4
+
+
+def test_column_panels_fall_back_to_one(hg, oracle):
+    """Panels need each row's columns ascending (the split keeps stored order
+    only then) and are never used for a diagonal Hessian (its separable bound
+    reads each row's entry): both keep one panel even when more are asked for,
+    and the FP64 steps still equal the CPU path on the stored order."""
+    q, d = planted_qp(hg, 3000, 8, 11)
+    s, i, v, c, lo, up, L = engine_inputs(oracle, q)
+    # reverse each row's stored order: a valid CSR whose columns descend
+    i2, v2 = i.copy(), v.copy()
+    for r in range(len(s) - 1):
+        i2[s[r]:s[r + 1]] = i[s[r]:s[r + 1]][::-1]
+        v2[s[r]:s[r + 1]] = v[s[r]:s[r + 1]][::-1]
+    betas = oracle.betas(20)
+    x0 = np.clip(np.zeros(q.n), lo, up)
+    x_cpu, _ = oracle.fista_steps(s, i2, v2, c, lo, up, L, betas, x0, x0)
+    with hg.Box(s, i2, v2, c, lo, up, L, column_panels=4) as box:
+        assert box.info()["column_panels"] == 1
+        for b in betas:
+            box.step(b)
+        assert np.array_equal(box.primal(), x_cpu)
+    n = 500
+    ds = np.arange(n + 1, dtype=np.int32)
+    with hg.Box(ds, np.arange(n, dtype=np.int32), np.full(n, 2.0), np.ones(n), -np.ones(n), np.ones(n), 2.0,
+                column_panels=4) as box:
+        assert box.info()["column_panels"] == 1
