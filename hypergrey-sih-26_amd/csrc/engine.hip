@@ -492,13 +492,21 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
     HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
 
-    // the plain CSR on the device: the ingest's arrays, or an upload
+    // Column panels are possible only for a matrix that is not diagonal and that
+    // the size rule (n 8 bytes >= 3 MiB, so at least two 2 MiB panels) or the
+    // caller gives more than one; otherwise the host CSR goes straight into the
+    // handle's own arrays, with no survey and no temporaries.
+    const bool may_panel = !b->diagonal && nnz > 0 &&
+                           (opt.column_panels > 1 || (opt.column_panels == 0 && n * 8 >= 3 * kPanelBytes / 2));
+    // the plain CSR on the device: the ingest's arrays, or an upload into
+    // temporaries for the survey and the split
     DeviceTemp<int> tstart, tcol;
     DeviceTemp<double> tval;
     const int* dstart = src.dev_start;
     const int* dcol = index;
     const double* dval = value;
-    if (!src.dev_start) {
+    const bool staged = !src.dev_start && may_panel;
+    if (staged) {
         tstart.alloc(un + 1);
         tcol.alloc(unnz);
         tval.alloc(unnz);
@@ -514,7 +522,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     // Column panels for a matrix that gathers far from its diagonal. A diagonal
     // Hessian keeps one panel: its separable bound reads each row's entry.
     int p64 = 1, p32 = 1;
-    if (!b->diagonal && nnz > 0 && opt.column_panels != 1) {
+    if (may_panel) {
         const PanelSurvey sv = survey_panels(n, dstart, dcol, b->stream);
         if (opt.column_panels == 0) {
             p64 = panels_for(n, sv, 8);
@@ -576,12 +584,21 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     for (auto& e : b->slot_event) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     // the matrix in its layouts
-    if (p64 == 1) {
+    if (p64 == 1 && (src.dev_start || staged)) {
         HIPCHECK(hipMemcpyAsync(b->l64.start, dstart, (un + 1) * 4, hipMemcpyDeviceToDevice, b->stream));
         if (nnz) {
             HIPCHECK(hipMemcpyAsync(b->l64.col, dcol, size_t(nnz) * 4, hipMemcpyDeviceToDevice, b->stream));
             HIPCHECK(hipMemcpyAsync(b->val, dval, size_t(nnz) * 8, hipMemcpyDeviceToDevice, b->stream));
         }
+    } else if (p64 == 1) {  // host CSR straight into the handle
+        upload(b->l64.start, start, (un + 1) * 4, b->stream);
+        if (nnz) {
+            upload(b->l64.col, index, size_t(nnz) * 4, b->stream);
+            upload(b->val, value, size_t(nnz) * 8, b->stream);
+        }
+        dstart = b->l64.start;
+        dcol = b->l64.col;
+        dval = b->val;
     } else {
         split_panels<double>(n, nnz, p64, dstart, dcol, dval, b->l64.start, b->l64.col, b->val, b->stream);
     }
