@@ -169,21 +169,31 @@ def cpu_baseline(inst, iterations_needed, threads):
     o = Oracle(kind)
     hr, hc, hv = hgqp.lower_triplets(inst["start"], inst["index"], inst["value"])
     q = BoxQP(inst["n"], hr, hc, hv, inst["cost"], inst["lower"], inst["upper"])
-    a = time.perf_counter()
-    o.solve_box(q, tolerance=KKT_TOL, iteration_limit=0, threads=threads, time_limit=600)
-    t_setup = time.perf_counter() - a
-    a = time.perf_counter()
-    o.solve_box(q, tolerance=KKT_TOL, iteration_limit=100, threads=threads, time_limit=600)
-    t100 = time.perf_counter() - a
-    per_iter = max(t100 - t_setup, 0.0) / 100
-    full = t_setup + per_iter * iterations_needed
-    return {"value": 1.0 / full, "unit": "solves/s", "cores": threads,
-            "kind": "reference" if kind == "reference" else "port",
-            "sample": (f"{'oracle/_ref (reference CPU sources)' if kind == 'reference' else 'oracle port'}, "
-                       f"{threads} OpenMP threads: setup+check {t_setup:.2f}s, setup+100 iterations {t100:.2f}s; "
-                       f"full solve = setup + {iterations_needed} iterations x {per_iter * 1e3:.1f} ms "
-                       f"(checks every 25 included) = {full:.2f}s"),
-            "setup_seconds": t_setup, "ms_per_iteration": per_iter * 1e3}
+    source = 'oracle/_ref (reference CPU sources)' if kind == 'reference' else 'oracle port'
+
+    def sample(nthreads, sample_iterations):
+        a = time.perf_counter()
+        o.solve_box(q, tolerance=KKT_TOL, iteration_limit=0, threads=nthreads, time_limit=600)
+        t_setup = time.perf_counter() - a
+        a = time.perf_counter()
+        o.solve_box(q, tolerance=KKT_TOL, iteration_limit=sample_iterations, threads=nthreads, time_limit=600)
+        t_run = time.perf_counter() - a
+        per_iter = max(t_run - t_setup, 0.0) / sample_iterations
+        full = t_setup + per_iter * iterations_needed
+        text = (f"{source}, {nthreads} OpenMP thread{'s' if nthreads > 1 else ''}: setup+check {t_setup:.2f}s, "
+                f"setup+{sample_iterations} iterations {t_run:.2f}s; full solve = setup + {iterations_needed} "
+                f"iterations x {per_iter * 1e3:.1f} ms (checks every 25 included) = {full:.2f}s")
+        return full, t_setup, per_iter, text
+
+    full, t_setup, per_iter, text = sample(threads, 100)
+    out = {"value": 1.0 / full, "unit": "solves/s", "cores": threads,
+           "kind": "reference" if kind == "reference" else "port", "sample": text,
+           "setup_seconds": t_setup, "ms_per_iteration": per_iter * 1e3}
+    if threads > 1:
+        full1, setup1, per1, text1 = sample(1, 25)
+        out["single_thread"] = {"value": 1.0 / full1, "unit": "solves/s", "cores": 1, "sample": text1,
+                                "setup_seconds": setup1, "ms_per_iteration": per1 * 1e3}
+    return out
 
 
 def batch_workload(args):
