@@ -23,6 +23,8 @@
 // matrix-vector multiplication on GPUs using the CSR storage format", SC 2014),
 // written for wave64 with the solver step fused into the row epilogue.
 #include "box_kernels.hpp"
+#include "slab_pass.hpp"
+#include "slabs.hpp"
 #include "sparse_pass.hpp"
 
 #include <cmath>
@@ -134,6 +136,72 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
             for (int w = 0; w < kWavesPerGroup; ++w) {
                 s += wave_dot[w];
                 r = fmax(r, wave_res[w]);
+            }
+            partials[blockIdx.x] = s;
+            partials[gridDim.x + blockIdx.x] = r;
+        }
+    }
+}
+
+// One FP32 step over column slabs (slab_pass.hpp): the row block's sums come
+// from slab_rows, then each row applies the step epilogue of step_kernel. The
+// restart partials are one per row block.
+template <int MOM>
+__global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int slabs, const int* __restrict__ rb,
+                                                                 const int* __restrict__ tc,
+                                                                 const uint4* __restrict__ row,
+                                                                 const uint4* __restrict__ col,
+                                                                 const float* __restrict__ val,
+                                                                 StepBuffers<float> bufs, Control* ctrl,
+                                                                 double* partials, int parity, double beta_imm) {
+    __shared__ __attribute__((aligned(16))) float ws0[kSlabWidth];
+    __shared__ __attribute__((aligned(16))) float ws1[kSlabWidth];
+    __shared__ __attribute__((aligned(16))) float ws2[kSlabWidth];
+    __shared__ float acc[kSlabRows];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    double beta = beta_imm, next_t = 0;
+    int use_x = 0;
+    if (MOM != kMomentumImmediate) {
+        if (ctrl->done) return;  // grid-uniform
+        momentum_of(ctrl->t[parity], &next_t, &beta);
+        use_x = ctrl->use_x[parity];
+    }
+    StepEpilogue<float> epi;
+    epi.gat = use_x ? bufs.x[parity] : bufs.w[parity];
+    epi.xin = bufs.x[parity];
+    epi.c = bufs.c;
+    epi.lo = bufs.lo;
+    epi.up = bufs.up;
+    epi.xout = bufs.x[parity ^ 1];
+    epi.wout = bufs.w[parity ^ 1];
+    epi.lipschitz = bufs.lipschitz;
+    epi.dstep = bufs.dstep;
+    epi.beta = float(beta);
+    epi.dot = 0;
+    epi.resid = 0;
+    const SlabArgs m{blocks, slabs, rb, tc, row, col, val};
+    slab_rows(m, epi.gat, ws0, ws1, ws2, acc);
+    const int r0 = rb[blockIdx.x], rows = rb[blockIdx.x + 1] - r0;
+    for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(r0 + r, acc[r]);
+    if (MOM == kMomentumReference) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctrl->t[parity ^ 1] = next_t;
+            ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
+            ctrl->use_x[parity ^ 1] = 0;
+        }
+    } else if (MOM == kMomentumRestart) {
+        const double d = wave_sum(epi.dot), rr = wave_max(epi.resid);
+        double* red = reinterpret_cast<double*>(ws0);  // free: every thread is past slab_rows' last barrier
+        if (lane == 0) {
+            red[wave] = d;
+            red[kSlabWaves + wave] = rr;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0, r = 0;
+            for (int w = 0; w < kSlabWaves; ++w) {
+                s += red[w];
+                r = fmax(r, red[kSlabWaves + w]);
             }
             partials[blockIdx.x] = s;
             partials[gridDim.x + blockIdx.x] = r;
@@ -511,6 +579,21 @@ template void launch_step<double>(const PanelMatrix<double>&, const StepBuffers<
                                   int, bool, double, hipStream_t, double);
 template void launch_step<float>(const PanelMatrix<float>&, const StepBuffers<float>&, Control*, double*, int, int,
                                  bool, double, hipStream_t, double);
+
+void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
+                      int momentum, double beta, hipStream_t stream, double switch_stat) {
+    dim3 grid(m.blocks), block(kSlabThreads);
+#define HGQP_SLAB_STEP(MOM)                                                                                  \
+    hipLaunchKernelGGL(slab_step_kernel<MOM>, grid, block, 0, stream, m.blocks, m.slabs, m.rb, m.tc, m.row, m.col, \
+                       m.val, b, ctrl, partials, parity, beta)
+    if (momentum == kMomentumImmediate) HGQP_SLAB_STEP(kMomentumImmediate);
+    else if (momentum == kMomentumReference) HGQP_SLAB_STEP(kMomentumReference);
+    else HGQP_SLAB_STEP(kMomentumRestart);
+#undef HGQP_SLAB_STEP
+    if (momentum == kMomentumRestart)
+        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity,
+                           switch_stat);
+}
 
 void launch_check(const PanelMatrix<double>& pm, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream) {

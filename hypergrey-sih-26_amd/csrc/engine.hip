@@ -5,6 +5,7 @@
 #include "internal.hpp"
 #include "device_util.hpp"
 #include "panels.hpp"
+#include "slabs.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -133,6 +134,7 @@ struct hgqp_box {
         int* blk[kMaxPanels] = {};
     };
     Layout l64, l32;
+    SlabStore slab;  // FP32 step layout when slab.m.blocks > 0
     double* val = nullptr;
     float* val32 = nullptr;
     double* carry64 = nullptr;  // running row sums between panel passes
@@ -257,6 +259,9 @@ void launch_one_step(hgqp_box* b, int precision, int momentum, const CheckParams
     const bool ord = b->ordered();
     if (precision == HGQP_PRECISION_FP64)
         launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+    else if (b->slab.m.blocks)
+        launch_slab_step(b->slab.m, b->b32(), b->ctrl, b->partials, b->parity, momentum, 0.0, b->stream,
+                         residual_phase(precision, momentum) ? cp.switch_stat : 0.0);
     else
         launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream,
                            residual_phase(precision, momentum) ? cp.switch_stat : 0.0);
@@ -418,6 +423,7 @@ void hgqp_box_options_default(hgqp_box_options* o) {
     o->diagonal_hessian = -1;
     o->step_metric = HGQP_STEP_SCALAR;
     o->column_panels = 0;
+    o->slabs = 0;
 }
 
 void hgqp_solve_options_default(hgqp_solve_options* o) {
@@ -475,6 +481,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
     if (opt.column_panels < 0 || opt.column_panels > kMaxPanels)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column_panels must be in [0, 16]");
+    if (opt.slabs < 0 || opt.slabs > 2) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: slabs must be 0, 1 or 2");
     require_device();
 
     auto b = std::make_unique<hgqp_box>();
@@ -496,8 +503,12 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     // the size rule (n 8 bytes >= 3 MiB, so at least two 2 MiB panels) or the
     // caller gives more than one; otherwise the host CSR goes straight into the
     // handle's own arrays, with no survey and no temporaries.
-    const bool may_panel = !b->diagonal && nnz > 0 &&
-                           (opt.column_panels > 1 || (opt.column_panels == 0 && n * 8 >= 3 * kPanelBytes / 2));
+    // Slabs for the FP32 steps are considered under the same size rule (or when
+    // asked for); the survey then decides as for panels.
+    const bool slab_candidate = f32 && nnz > 0 && !b->diagonal &&
+                                (opt.slabs == 2 || (opt.slabs == 0 && n * 8 >= 3 * kPanelBytes / 2));
+    const bool may_panel = slab_candidate || (!b->diagonal && nnz > 0 &&
+                           (opt.column_panels > 1 || (opt.column_panels == 0 && n * 8 >= 3 * kPanelBytes / 2)));
     // the plain CSR on the device: the ingest's arrays, or an upload into
     // temporaries for the survey and the split
     DeviceTemp<int> tstart, tcol;
@@ -522,8 +533,10 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     // Column panels for a matrix that gathers far from its diagonal. A diagonal
     // Hessian keeps one panel: its separable bound reads each row's entry.
     int p64 = 1, p32 = 1;
+    bool use_slabs = false;
     if (may_panel) {
         const PanelSurvey sv = survey_panels(n, dstart, dcol, b->stream);
+        use_slabs = slab_candidate && sv.sorted && (opt.slabs == 2 || sv.far >= 8 * n);
         if (opt.column_panels == 0) {
             p64 = panels_for(n, sv, 8);
             p32 = f32 ? panels_for(n, sv, 4) : 1;
@@ -532,11 +545,13 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         }
     }
     const bool own32 = f32 && p32 != p64;
+    // the FP32 iterates hold whole slabs when the steps may read them slab by slab
+    const size_t un32 = use_slabs ? static_cast<size_t>(slab_padded(n)) : un;
 
     size_t bytes = Arena::span(size_t(p64) * (un + 1), 4) + Arena::span(unnz, 4) + Arena::span(unnz, 8) +
                    3 * Arena::span(un, 8) + 4 * Arena::span(un, 8) + Arena::span(1, sizeof(Control));
     if (p64 > 1) bytes += Arena::span(un, 8);
-    if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un, 4);
+    if (f32) bytes += Arena::span(unnz, 4) + 3 * Arena::span(un, 4) + 4 * Arena::span(un32, 4);
     if (own32) bytes += Arena::span(size_t(p32) * (un + 1), 4) + Arena::span(unnz, 4);
     if (f32 && p32 > 1) bytes += Arena::span(un, 4);
     if (refine) bytes += Arena::span(un, 8);
@@ -561,8 +576,8 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         b->lo32 = b->arena.take<float>(un);
         b->up32 = b->arena.take<float>(un);
         for (int p = 0; p < 2; ++p) {
-            b->x32[p] = b->arena.take<float>(un);
-            b->w32[p] = b->arena.take<float>(un);
+            b->x32[p] = b->arena.take<float>(un32);
+            b->w32[p] = b->arena.take<float>(un32);
         }
         b->l32.count = p32;
         if (own32) {
@@ -617,6 +632,14 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     }
     launch_init<double>(n, b->lo, b->up, b->x[0], b->w[0], b->stream);
     HIPCHECK(hipGetLastError());
+    if (use_slabs) {
+        // one row block per CU: a slab step holds 160 KiB of LDS per workgroup
+        int dev = 0, cus = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const std::vector<int> rb = slab_blocks(n, start, std::max(cus, 1));
+        (void)build_slabs(n, nnz, rb, dstart, dcol, dval, b->slab, b->stream);  // false: the CSR passes remain
+    }
 
     // row blocks of every panel, from the host copy of its starts; the greedy
     // blocking is a sequential scan over n rows, so each panel gets a host thread
@@ -637,7 +660,9 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     std::vector<Job> jobs;
     for (int p = 0; p < b->l64.count; ++p)
         jobs.push_back({&blk64[p], b->l64.count == 1 ? start : hs64.data() + size_t(p) * (un + 1)});
-    for (size_t p = 0; p < blk32.size(); ++p) jobs.push_back({&blk32[p], hs32.data() + p * (un + 1)});
+    // a one-panel FP32 layout of its own (FP64 split, FP32 not) is the plain CSR: the host starts
+    for (size_t p = 0; p < blk32.size(); ++p)
+        jobs.push_back({&blk32[p], b->l32.count == 1 ? start : hs32.data() + p * (un + 1)});
     std::vector<std::exception_ptr> failed(jobs.size());
     {
         std::vector<std::thread> workers;
@@ -665,6 +690,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
             bbytes += Arena::span(v.size(), 4);
             groups = std::max(groups, groups_for(int(v.size()) - 1));
         }
+    groups = std::max(groups, b->slab.m.blocks);
     b->groups = groups;
     bbytes += Arena::span(size_t(groups) * 10 + 16, 8);
     b->blocks.reserve(bbytes);
@@ -920,9 +946,11 @@ int hgqp_box_info_get(const hgqp_box* b, hgqp_box_info* info) {
         info->n = b->n;
         info->nnz = b->nnz;
         info->row_blocks = b->nblocks;
-        info->device_bytes = int64_t(b->arena.size + b->blocks.size);
+        info->device_bytes = int64_t(b->arena.size + b->blocks.size + b->slab.bytes);
         info->column_panels = b->l64.count;
         info->column_panels32 = b->val32 ? b->l32.count : 0;
+        info->slabs32 = b->slab.m.blocks ? b->slab.m.slabs : 0;
+        info->reserved = 0;
         info->lipschitz = b->lipschitz;
     });
 }

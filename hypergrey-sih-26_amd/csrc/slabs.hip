@@ -1,0 +1,271 @@
+// Slab layout of a single instance's CSR (slabs.hpp), built on the device once
+// per handle:
+//   1. one thread per row counts the row's runs (its entries in one slab) and
+//      checks ascending columns and runs of at most kSlabChunk entries;
+//   2. an exclusive sum gives each row the offset of its run records; one
+//      thread per row writes them: key = tile (block, slab), value = first
+//      entry, row within the block and run length;
+//   3. a stable radix sort by tile puts each tile's runs in row order;
+//   4. one thread per tile finds its runs, then counts its chunks with the
+//      packing rule (a run that does not fit the open chunk starts a new one);
+//   5. an exclusive sum gives the tiles' first chunks; one thread per tile
+//      writes its entries and padding.
+#include "slabs.hpp"
+
+#include "device_util.hpp"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace hgqp {
+namespace {
+
+constexpr int kThreads = 256;
+
+int grid_for(int64_t items) {
+    const int64_t g = (items + kThreads - 1) / kThreads;
+    return int(std::min<int64_t>(std::max<int64_t>(g, 1), 1 << 20));
+}
+
+__device__ inline int64_t thread_id() { return int64_t(blockIdx.x) * kThreads + threadIdx.x; }
+__device__ inline int64_t thread_count() { return int64_t(gridDim.x) * kThreads; }
+
+// Runs of row i; flags a descending column or a run longer than a chunk.
+__global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* start, const int* col, int* runs,
+                                                       int* bad) {
+    for (int64_t i = thread_id(); i < n; i += thread_count()) {
+        const int a = start[i], e = start[i + 1];
+        int count = 0, len = 0, prev_slab = -1, prev_col = -1;
+        for (int k = a; k < e; ++k) {
+            const int c = col[k];
+            if (c < prev_col) atomicOr(bad, 1);
+            prev_col = c;
+            const int s = c / kSlabWidth;
+            if (s != prev_slab) {
+                ++count;
+                len = 0;
+                prev_slab = s;
+            }
+            if (++len > kSlabChunk) atomicOr(bad, 2);
+        }
+        runs[i] = count;
+    }
+}
+
+// Block of row i: the last b with rb[b] <= i.
+__device__ inline int block_of(const int* rb, int blocks, int64_t i) {
+    int lo = 0, hi = blocks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (rb[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Run records in row order: key = b slabs + s, value = (first entry << 32) |
+// (row within the block << 8) | length.
+__global__ __launch_bounds__(kThreads) void write_runs(int64_t n, const int* start, const int* col, const int* rb,
+                                                       int blocks, int slabs, const int* offset, unsigned* key,
+                                                       uint64_t* value) {
+    for (int64_t i = thread_id(); i < n; i += thread_count()) {
+        const int a = start[i], e = start[i + 1];
+        if (a == e) continue;
+        const int b = block_of(rb, blocks, i);
+        const uint64_t local = uint64_t(i - rb[b]);
+        int o = offset[i];
+        int k = a;
+        while (k < e) {
+            const int s = col[k] / kSlabWidth;
+            int q = k + 1;
+            while (q < e && col[q] / kSlabWidth == s) ++q;
+            key[o] = unsigned(b * slabs + s);
+            value[o] = (uint64_t(uint32_t(k)) << 32) | (local << 8) | uint64_t(q - k);
+            ++o;
+            k = q;
+        }
+    }
+}
+
+// first[t] = first sorted record of tile t (lower bound), t in [0, tiles]
+__global__ __launch_bounds__(kThreads) void tile_first(const unsigned* key, int records, int tiles, int* first) {
+    for (int64_t t = thread_id(); t <= tiles; t += thread_count()) {
+        int lo = 0, hi = records;
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (key[mid] < unsigned(t)) lo = mid + 1;
+            else hi = mid;
+        }
+        first[t] = lo;
+    }
+}
+
+// Chunks of tile t under the packing rule.
+__global__ __launch_bounds__(kThreads) void count_chunks(const int* first, const uint64_t* value, int tiles,
+                                                         int* chunks) {
+    for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
+        int count = 0, fill = 0;
+        for (int j = first[t]; j < first[t + 1]; ++j) {
+            const int len = int(value[j] & 0xFF);
+            if (fill == 0 || fill + len > kSlabChunk) {
+                ++count;
+                fill = 0;
+            }
+            fill += len;
+        }
+        chunks[t] = count;
+    }
+}
+
+// Writes tile t's entries from chunk tc[t] on, padding each chunk's unused slots.
+__global__ __launch_bounds__(kThreads) void pack_tiles(const int* first, const uint64_t* value, const int* tc,
+                                                       int tiles, int slabs, const int* col, const double* val,
+                                                       uint16_t* out_row, uint16_t* out_col, float* out_val) {
+    for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
+        const int base = (int(t % slabs)) * kSlabWidth;
+        int64_t chunk = int64_t(tc[t]) - 1;
+        int fill = 0;
+        auto pad = [&]() {
+            for (; fill > 0 && fill < kSlabChunk; ++fill) {
+                const int64_t at = chunk * kSlabChunk + fill;
+                out_row[at] = uint16_t(kSlabPad);
+                out_col[at] = 0;
+                out_val[at] = 0.0f;
+            }
+        };
+        for (int j = first[t]; j < first[t + 1]; ++j) {
+            const uint64_t v = value[j];
+            const int len = int(v & 0xFF);
+            const int local = int((v >> 8) & 0xFFFFFF);
+            const int k = int(v >> 32);
+            if (fill == 0 || fill + len > kSlabChunk) {
+                pad();
+                ++chunk;
+                fill = 0;
+            }
+            for (int q = 0; q < len; ++q) {
+                const int64_t at = chunk * kSlabChunk + fill + q;
+                out_row[at] = uint16_t(local);
+                out_col[at] = uint16_t(col[k + q] - base);
+                out_val[at] = float(val[k + q]);  // round to nearest
+            }
+            fill += len;
+        }
+        pad();
+    }
+}
+
+int bits_for(int64_t v) {
+    int b = 1;
+    while ((int64_t(1) << b) <= v) ++b;
+    return b;
+}
+
+}  // namespace
+
+std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target) {
+    const int64_t nnz = start[n];
+    std::vector<int> rb{0};
+    int64_t r = 0;
+    while (r < n) {
+        const int64_t rs = r;
+        const int64_t goal = nnz * int64_t(rb.size()) / std::max(target, 1);
+        ++r;  // every block takes at least one row
+        while (r < n && r - rs < kSlabRows && start[r + 1] <= goal) ++r;
+        rb.push_back(int(r));
+    }
+    return rb;
+}
+
+bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const int* start, const int* col,
+                 const double* val, SlabStore& out, hipStream_t stream) {
+    const int nb = int(blocks.size()) - 1;
+    const int64_t P = (n + kSlabWidth - 1) / kSlabWidth;
+    const int64_t tiles64 = int64_t(nb) * P;
+    if (nb < 1 || nnz < 1 || tiles64 + 1 >= (int64_t(1) << 31)) return false;
+    for (int b = 0; b < nb; ++b)
+        if (blocks[b + 1] - blocks[b] > kSlabRows) return false;
+    const int tiles = int(tiles64);
+    const size_t un = size_t(n);
+
+    DeviceTemp<int> rb(size_t(nb) + 1), runs(un + 1), offset(un + 1), flag(1);
+    HIPCHECK(hipMemcpyAsync(rb.p, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemsetAsync(flag.p, 0, 4, stream));
+    HIPCHECK(hipMemsetAsync(runs.p + n, 0, 4, stream));
+    hipLaunchKernelGGL(count_runs, dim3(grid_for(n)), dim3(kThreads), 0, stream, n, start, col, runs.p, flag.p);
+    size_t tmp_bytes = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, runs.p, offset.p, int(n + 1), stream));
+    {
+        DeviceTemp<char> tmp(tmp_bytes);
+        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, runs.p, offset.p, int(n + 1), stream));
+    }
+    int host[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(&host[0], flag.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipMemcpyAsync(&host[1], offset.p + n, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (host[0]) return false;
+    const int records = host[1];
+
+    DeviceTemp<unsigned> key{size_t(records)}, key2{size_t(records)};
+    DeviceTemp<int> first(size_t(tiles) + 1), chunks(size_t(tiles) + 1), tc(size_t(tiles) + 1);
+    DeviceTemp<uint64_t> value{size_t(records)}, value2{size_t(records)};
+    hipLaunchKernelGGL(write_runs, dim3(grid_for(n)), dim3(kThreads), 0, stream, n, start, col, rb.p, nb, int(P),
+                       offset.p, key.p, value.p);
+    const int end_bit = bits_for(tiles);
+    tmp_bytes = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key2.p, value.p, value2.p, records, 0,
+                                                end_bit, stream));
+    {
+        DeviceTemp<char> tmp(tmp_bytes);
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, key.p, key2.p, value.p, value2.p, records, 0,
+                                                    end_bit, stream));
+    }
+    hipLaunchKernelGGL(tile_first, dim3(grid_for(int64_t(tiles) + 1)), dim3(kThreads), 0, stream, key2.p, records,
+                       tiles, first.p);
+    HIPCHECK(hipMemsetAsync(chunks.p + tiles, 0, 4, stream));
+    hipLaunchKernelGGL(count_chunks, dim3(grid_for(tiles)), dim3(kThreads), 0, stream, first.p, value2.p, tiles,
+                       chunks.p);
+    tmp_bytes = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, chunks.p, tc.p, tiles + 1, stream));
+    {
+        DeviceTemp<char> tmp(tmp_bytes);
+        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, chunks.p, tc.p, tiles + 1, stream));
+    }
+    int total = 0;
+    HIPCHECK(hipMemcpyAsync(&total, tc.p + tiles, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (total < 1) return false;
+
+    // one allocation: block starts, tile chunk starts, rows, columns, values
+    auto span = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    const size_t entries = size_t(total) * kSlabChunk;
+    const size_t b_rb = span((size_t(nb) + 1) * 4), b_tc = span((size_t(tiles) + 1) * 4), b_idx = span(entries * 2),
+                 b_val = span(entries * 4);
+    out.bytes = b_rb + b_tc + 2 * b_idx + b_val;
+    HIPCHECK(hipMalloc(reinterpret_cast<void**>(&out.base), out.bytes));
+    char* p = out.base;
+    int* d_rb = reinterpret_cast<int*>(p);
+    int* d_tc = reinterpret_cast<int*>(p + b_rb);
+    uint16_t* d_row = reinterpret_cast<uint16_t*>(p + b_rb + b_tc);
+    uint16_t* d_col = reinterpret_cast<uint16_t*>(p + b_rb + b_tc + b_idx);
+    float* d_val = reinterpret_cast<float*>(p + b_rb + b_tc + 2 * b_idx);
+    HIPCHECK(hipMemcpyAsync(d_rb, rb.p, (size_t(nb) + 1) * 4, hipMemcpyDeviceToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(d_tc, tc.p, (size_t(tiles) + 1) * 4, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(pack_tiles, dim3(grid_for(tiles)), dim3(kThreads), 0, stream, first.p, value2.p, tc.p, tiles,
+                       int(P), col, val, d_row, d_col, d_val);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(stream));  // the temporaries are released on return
+    out.chunks = total;
+    out.m.blocks = nb;
+    out.m.slabs = int(P);
+    out.m.rb = d_rb;
+    out.m.tc = d_tc;
+    out.m.row = reinterpret_cast<const uint4*>(d_row);
+    out.m.col = reinterpret_cast<const uint4*>(d_col);
+    out.m.val = d_val;
+    return true;
+}
+
+}  // namespace hgqp

@@ -1,0 +1,82 @@
+// Column slabs: the FP32 step's layout of a single instance's matrix when its
+// columns are spread over all n, so that gathering the vector from L2 one
+// 128-byte line per entry bounds the step.
+//
+// Rows are split into row blocks (one workgroup each, about nnz / CUs entries
+// and at most kSlabRows rows). Columns are split into slabs of kSlabWidth. The
+// tile (block b, slab s) holds the block's entries whose columns lie in slab s,
+// rows in order and each row's entries in stored column order, packed into
+// chunks of kSlabChunk entries: a row's entries in one slab (its run) never
+// straddle two chunks, and the rest of a chunk is padding. Each entry stores
+// its row within the block and its column within the slab as 16-bit values.
+//
+// A step launch stages slab s of the gathered vector into LDS with direct
+// global-to-LDS copies (every CU reads the vector once per step, in full
+// 1 KiB requests) while it sums the tile's chunks from registers loaded one
+// slab earlier; each row's running sum stays in LDS across the slabs. A thread
+// adds a run's products to the row's sum one by one, so every row sums its
+// entries in stored column order, exactly as the ORDERED CSR pass does.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "box_kernels.hpp"
+
+namespace hgqp {
+
+constexpr int kSlabWidth = 12288;  // columns per slab: 48 KiB of the FP32 vector
+constexpr int kSlabThreads = 512;  // threads of a slab step launch (8 waves)
+constexpr int kSlabRows = 4096;    // rows per row block: its FP32 sums take 16 KiB of LDS
+constexpr int kSlabChunk = 8;      // entries per chunk
+constexpr unsigned kSlabPad = 0xFFFF;  // row value of a padding entry
+
+struct SlabMatrix {
+    int blocks = 0;                 // row blocks (workgroups of a launch); 0: no slabs
+    int slabs = 0;                  // column slabs
+    const int* rb = nullptr;        // [blocks + 1] first row of each block
+    const int* tc = nullptr;        // [blocks * slabs + 1] first chunk of tile (b, s) at b * slabs + s
+    const uint4* row = nullptr;     // [chunks] 8 x uint16: row within the block, kSlabPad for padding
+    const uint4* col = nullptr;     // [chunks] 8 x uint16: column within the slab (0 for padding)
+    const float* val = nullptr;     // [chunks * 8] value (0 for padding)
+};
+
+// Elements a gathered vector must hold so that every slab copy reads inside it.
+inline int64_t slab_padded(int64_t n) { return (n + kSlabWidth - 1) / kSlabWidth * kSlabWidth; }
+
+// Row blocks from the host row starts: block j ends at about (j + 1) nnz / target
+// entries, with at most kSlabRows rows. Returns the first row of every block
+// followed by n.
+std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target);
+
+// Device storage of a slab matrix, owned by a handle.
+struct SlabStore {
+    char* base = nullptr;
+    size_t bytes = 0;
+    int64_t chunks = 0;
+    SlabMatrix m;
+    SlabStore() = default;
+    SlabStore(const SlabStore&) = delete;
+    SlabStore& operator=(const SlabStore&) = delete;
+    ~SlabStore() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+// Builds the slab layout of the device CSR (start, col, val) over the row blocks
+// `blocks` (from slab_blocks). Returns false and allocates nothing when a row's
+// columns are not ascending or a row has more than kSlabChunk entries in one
+// slab. Synchronizes `stream`.
+bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const int* start, const int* col,
+                 const double* val, SlabStore& out, hipStream_t stream);
+
+// One FP32 step over slabs: the step_kernel contract of launch_step<float>
+// (same buffers, momentum sources and restart partials, one partial per row
+// block), with every row summed in stored column order whatever the handle's
+// reduction order. The gathered vectors x[parity] and w[parity] must hold
+// slab_padded(n) elements.
+void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
+                      int momentum, double beta, hipStream_t stream, double switch_stat);
+
+}  // namespace hgqp

@@ -95,6 +95,14 @@ typedef struct hgqp_box_options {
                                 not depend on it: each row still sums its entries
                                 in stored order. Rows must have ascending columns
                                 for more than one; otherwise 1 is used.           */
+    int32_t slabs;           /* FP32 steps over LDS-staged column slabs: 0
+                                (default) uses them for a matrix with ascending
+                                columns that gathers far from its diagonal and
+                                has n >= 393216; 1 never; 2 whenever the matrix
+                                fits (ascending columns, at most 8 entries per
+                                row in a slab of 12288 columns). Slab steps sum
+                                each row in stored column order, so their FP32
+                                iterates equal those of HGQP_REDUCE_ORDERED.   */
 } hgqp_box_options;
 
 void hgqp_box_options_default(hgqp_box_options* options);
@@ -184,14 +192,17 @@ int hgqp_box_reset(hgqp_box* box);
 int hgqp_box_destroy(hgqp_box* box);
 
 /* Static facts of a handle: rows, stored entries, row blocks (over the FP64
- * panels), device bytes, and the column panels of the FP64 and FP32 passes
- * (0 for FP32 when the handle has no FP32 phase). */
+ * panels), device bytes, the column panels of the FP64 and FP32 passes (0 for
+ * FP32 when the handle has no FP32 phase) and the column slabs of the FP32
+ * steps (0 when they do not use slabs). */
 typedef struct hgqp_box_info {
     int64_t n, nnz, row_blocks;
     int64_t device_bytes;
     double lipschitz;
     int32_t column_panels;
     int32_t column_panels32;
+    int32_t slabs32;
+    int32_t reserved;
 } hgqp_box_info;
 int hgqp_box_info_get(const hgqp_box* box, hgqp_box_info* info);
 

@@ -86,7 +86,8 @@ def test_invalid_input_rejected_before_device(hg):
     bad[4][0], bad[5][0] = 1.0, 0.0  # lower > upper
     with pytest.raises(hg.InvalidArgument):
         hg.Box(*bad, 5.0)
-    for opt in ({"column_panels": 17}, {"column_panels": -1}, {"step_metric": 7}, {"precision": 9}):
+    for opt in ({"column_panels": 17}, {"column_panels": -1}, {"step_metric": 7}, {"precision": 9}, {"slabs": 3},
+                {"slabs": -1}):
         with pytest.raises(hg.InvalidArgument):
             hg.Box(*args, 5.0, **opt)
     for kw in ({"threads": 0}, {"tolerance": 0.0}, {"time_limit": -1.0}):

@@ -226,6 +226,24 @@ def test_full_size_panels_chosen_and_bit_exact(hg, oracle):
     assert np.abs(x - d["optimum"]).max() < 1e-4
 
 
+def test_fp32_one_panel_beside_fp64_panels(hg):
+    """Planted n = 4e5: the size rule splits the FP64 passes into 2 column
+    panels and keeps the FP32 passes on one, a layout of its own whose row
+    blocks come from the host row starts; the FP32 solve on it is verified and
+    recovers x*."""
+    n = 400000
+    d = hg.gen_planted(n, 8, 7)
+    s, i, v, c, lo, up = d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"]
+    with hg.Box(s, i, v, c, lo, up, hg.lipschitz(s, v), precision=hg.PRECISION_FP32_REFINE,
+                momentum=hg.MOMENTUM_RESTART, step_metric=hg.STEP_JACOBI, slabs=1) as box:
+        info = box.info()
+        assert info["column_panels"] == 2 and info["column_panels32"] == 1 and info["slabs32"] == 0
+        rep = box.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=120)
+        x = box.primal()
+    assert rep["status"] == "optimal" and rep["optimality_verified"]
+    assert np.abs(x - d["optimum"]).max() < 1e-4
+
+
 def test_column_panels_fall_back_to_one(hg, oracle):
     """Panels need each row's columns ascending (the split keeps stored order
     only then) and are never used for a diagonal Hessian (its separable bound
