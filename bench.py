@@ -53,18 +53,18 @@ def step_bytes(n, nnz, value_bytes, vector_bytes, jacobi=False):
     return nnz * (value_bytes + 4) + 4 * (n + 1) + (8 if jacobi else 7) * n * vector_bytes
 
 
-def measured_traffic(use32, mode, n, draws, nnz, panels):
-    """HBM bytes per FP32 step (all of its column-panel launches) from the
-    committed rocprofv3 PMC summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on
-    plain streaming kernels), when this run's mode, instance and panel count are
-    the ones profiled; otherwise None."""
+def measured_traffic(use32, mode, n, draws, nnz, panels, slabs):
+    """HBM bytes per FP32 step (all of its launches) from the committed
+    rocprofv3 PMC summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on plain
+    streaming kernels), when this run's mode, instance, panel and slab counts
+    are the ones profiled; otherwise None."""
     path = os.path.join(ROOT, "profiles", "traffic_planted_n1e6.json")
     if not use32 or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
     if (t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or t.get("mode") != mode
-            or t.get("column_panels32") != panels):
+            or t.get("column_panels32") != panels or t.get("slabs32", 0) != slabs):
         return None
     return t["traffic_bytes_per_step"]
 
@@ -106,7 +106,7 @@ def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
     finally:
         timed.close()
     return dict(name=name, wall=wall, lat=lat, reps=reps, timed=treps, x=x,
-                panels=(info["column_panels"], info["column_panels32"]))
+                panels=(info["column_panels"], info["column_panels32"]), slabs=info["slabs32"])
 
 
 def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
@@ -620,7 +620,7 @@ def main():
             "max_abs_err_vs_planted_x": float(np.abs(r["x"] - inst["optimum"]).max()),
             "fp64_step_us": f64_ms / f64_n * 1e3 if f64_n else None,
             "fp32_step_us": f32_ms / f32_n * 1e3 if f32_n else None,
-            "column_panels": r["panels"][0], "column_panels32": r["panels"][1],
+            "column_panels": r["panels"][0], "column_panels32": r["panels"][1], "slabs32": r["slabs"],
         }
         if ref_x is not None:
             m["max_abs_err_vs_fp64_reference_x"] = float(np.abs(r["x"] - ref_x).max())
@@ -637,14 +637,19 @@ def main():
     nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8, jacobi=MODES[head][3] == hgqp.STEP_JACOBI)
     achieved = nbytes / (us * 1e-6) / 1e9
     panels = hm["column_panels32"] if use32 else hm["column_panels"]
+    slabs = hm["slabs32"] if use32 else 0
     tname = "float" if use32 else "double"
-    kernel = f"step_kernel<{tname}>" if panels == 1 else \
-        f"step_kernel<{tname}> + {panels - 1} carry_kernel<{tname}> (one step over {panels} column panels)"
+    if slabs:
+        kernel = f"slab_step_kernel (one FP32 step over {slabs} column slabs staged in LDS)"
+    elif panels == 1:
+        kernel = f"step_kernel<{tname}>"
+    else:
+        kernel = f"step_kernel<{tname}> + {panels - 1} carry_kernel<{tname}> (one step over {panels} column panels)"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(use32, head, n, args.draws, nnz, panels),
+                "traffic": measured_traffic(use32, head, n, args.draws, nnz, panels, slabs),
                 "kernel": kernel, "algorithmic_bytes": nbytes, "avg_launch_us": round(us, 2),
-                "column_panels": panels}
+                "column_panels": panels, "slabs": slabs}
     # the FP64 reference-schedule step beside it
     if "fp64_reference" in modes and modes["fp64_reference"]["fp64_step_us"]:
         b64 = step_bytes(n, nnz, 8, 8)

@@ -6,10 +6,11 @@
 //      thread per row writes them: key = tile (block, slab), value = first
 //      entry, row within the block and run length;
 //   3. a stable radix sort by tile puts each tile's runs in row order;
-//   4. one thread per tile finds its runs, then counts its chunks with the
-//      packing rule (a run that does not fit the open chunk starts a new one);
-//   5. an exclusive sum gives the tiles' first chunks; one thread per tile
-//      writes its entries and padding.
+//   4. one thread per tile finds its runs, then walks them with the packing
+//      rule (a run that does not fit the open chunk starts a new one), giving
+//      the tile's chunk count and each run's slot;
+//   5. an exclusive sum gives the tiles' first chunks; the output is filled
+//      with padding and one thread per run writes its entries.
 #include "slabs.hpp"
 
 #include "device_util.hpp"
@@ -102,9 +103,10 @@ __global__ __launch_bounds__(kThreads) void tile_first(const unsigned* key, int 
     }
 }
 
-// Chunks of tile t under the packing rule.
-__global__ __launch_bounds__(kThreads) void count_chunks(const int* first, const uint64_t* value, int tiles,
-                                                         int* chunks) {
+// Chunks of tile t under the packing rule, and each run's slot in its tile
+// (local chunk * kSlabChunk + first entry within the chunk).
+__global__ __launch_bounds__(kThreads) void place_runs(const int* first, const uint64_t* value, int tiles, int* chunks,
+                                                       int* slot) {
     for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
         int count = 0, fill = 0;
         for (int j = first[t]; j < first[t + 1]; ++j) {
@@ -113,47 +115,32 @@ __global__ __launch_bounds__(kThreads) void count_chunks(const int* first, const
                 ++count;
                 fill = 0;
             }
+            slot[j] = (count - 1) * kSlabChunk + fill;
             fill += len;
         }
         chunks[t] = count;
     }
 }
 
-// Writes tile t's entries from chunk tc[t] on, padding each chunk's unused slots.
-__global__ __launch_bounds__(kThreads) void pack_tiles(const int* first, const uint64_t* value, const int* tc,
-                                                       int tiles, int slabs, const int* col, const double* val,
-                                                       uint16_t* out_row, uint16_t* out_col, float* out_val) {
-    for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
-        const int base = (int(t % slabs)) * kSlabWidth;
-        int64_t chunk = int64_t(tc[t]) - 1;
-        int fill = 0;
-        auto pad = [&]() {
-            for (; fill > 0 && fill < kSlabChunk; ++fill) {
-                const int64_t at = chunk * kSlabChunk + fill;
-                out_row[at] = uint16_t(kSlabPad);
-                out_col[at] = 0;
-                out_val[at] = 0.0f;
-            }
-        };
-        for (int j = first[t]; j < first[t + 1]; ++j) {
-            const uint64_t v = value[j];
-            const int len = int(v & 0xFF);
-            const int local = int((v >> 8) & 0xFFFFFF);
-            const int k = int(v >> 32);
-            if (fill == 0 || fill + len > kSlabChunk) {
-                pad();
-                ++chunk;
-                fill = 0;
-            }
-            for (int q = 0; q < len; ++q) {
-                const int64_t at = chunk * kSlabChunk + fill + q;
-                out_row[at] = uint16_t(local);
-                out_col[at] = uint16_t(col[k + q] - base);
-                out_val[at] = float(val[k + q]);  // round to nearest
-            }
-            fill += len;
+// One thread per run: writes its entries at tile tc[t]'s chunks + its slot. The
+// output already holds padding (row kSlabPad, column 0, value 0) everywhere.
+__global__ __launch_bounds__(kThreads) void pack_runs(const unsigned* key, const uint64_t* value, const int* slot,
+                                                      const int* tc, int records, int slabs, const int* col,
+                                                      const double* val, uint16_t* out_row, uint16_t* out_col,
+                                                      float* out_val) {
+    for (int64_t j = thread_id(); j < records; j += thread_count()) {
+        const int t = int(key[j]);
+        const uint64_t v = value[j];
+        const int len = int(v & 0xFF);
+        const int local = int((v >> 8) & 0xFFFFFF);
+        const int k = int(v >> 32);
+        const int base = (t % slabs) * kSlabWidth;
+        const int64_t at = int64_t(tc[t]) * kSlabChunk + slot[j];
+        for (int q = 0; q < len; ++q) {
+            out_row[at + q] = uint16_t(local);
+            out_col[at + q] = uint16_t(col[k + q] - base);
+            out_val[at + q] = float(val[k + q]);  // round to nearest
         }
-        pad();
     }
 }
 
@@ -210,6 +197,7 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
 
     DeviceTemp<unsigned> key{size_t(records)}, key2{size_t(records)};
     DeviceTemp<int> first(size_t(tiles) + 1), chunks(size_t(tiles) + 1), tc(size_t(tiles) + 1);
+    DeviceTemp<int> slot{size_t(records)};
     DeviceTemp<uint64_t> value{size_t(records)}, value2{size_t(records)};
     hipLaunchKernelGGL(write_runs, dim3(grid_for(n)), dim3(kThreads), 0, stream, n, start, col, rb.p, nb, int(P),
                        offset.p, key.p, value.p);
@@ -225,8 +213,8 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     hipLaunchKernelGGL(tile_first, dim3(grid_for(int64_t(tiles) + 1)), dim3(kThreads), 0, stream, key2.p, records,
                        tiles, first.p);
     HIPCHECK(hipMemsetAsync(chunks.p + tiles, 0, 4, stream));
-    hipLaunchKernelGGL(count_chunks, dim3(grid_for(tiles)), dim3(kThreads), 0, stream, first.p, value2.p, tiles,
-                       chunks.p);
+    hipLaunchKernelGGL(place_runs, dim3(grid_for(tiles)), dim3(kThreads), 0, stream, first.p, value2.p, tiles,
+                       chunks.p, slot.p);
     tmp_bytes = 0;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, chunks.p, tc.p, tiles + 1, stream));
     {
@@ -253,8 +241,11 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     float* d_val = reinterpret_cast<float*>(p + b_rb + b_tc + 2 * b_idx);
     HIPCHECK(hipMemcpyAsync(d_rb, rb.p, (size_t(nb) + 1) * 4, hipMemcpyDeviceToDevice, stream));
     HIPCHECK(hipMemcpyAsync(d_tc, tc.p, (size_t(tiles) + 1) * 4, hipMemcpyDeviceToDevice, stream));
-    hipLaunchKernelGGL(pack_tiles, dim3(grid_for(tiles)), dim3(kThreads), 0, stream, first.p, value2.p, tc.p, tiles,
-                       int(P), col, val, d_row, d_col, d_val);
+    HIPCHECK(hipMemsetAsync(d_row, 0xFF, entries * 2, stream));  // kSlabPad
+    HIPCHECK(hipMemsetAsync(d_col, 0, entries * 2, stream));
+    HIPCHECK(hipMemsetAsync(d_val, 0, entries * 4, stream));
+    hipLaunchKernelGGL(pack_runs, dim3(grid_for(records)), dim3(kThreads), 0, stream, key2.p, value2.p, slot.p, tc.p,
+                       records, int(P), col, val, d_row, d_col, d_val);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream));  // the temporaries are released on return
     out.chunks = total;
