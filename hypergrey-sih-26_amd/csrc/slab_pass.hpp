@@ -79,12 +79,7 @@ __device__ inline void load_chunk(SlabChunk& q, const uint4* __restrict__ row, c
     q.v1 = v[1];
 }
 
-// Adds a chunk's products to the running sums of its rows. A row's run sits in
-// one chunk and no other thread touches the row during the tile, so the sums
-// are read up front and each run adds its rounded products in stored order.
-__device__ inline void sum_chunk(const SlabChunk& q, const float* ws, float* acc) {
-#pragma clang fp contract(off)
-    int r[kSlabChunk], c[kSlabChunk];
+__device__ inline void unpack_chunk(const SlabChunk& q, int* r, int* c, float* v) {
     const uint32_t rw[4] = {q.r.x, q.r.y, q.r.z, q.r.w};
     const uint32_t cw[4] = {q.c.x, q.c.y, q.c.z, q.c.w};
 #pragma unroll
@@ -94,25 +89,61 @@ __device__ inline void sum_chunk(const SlabChunk& q, const float* ws, float* acc
         c[2 * j] = int(cw[j] & 0xFFFF);
         c[2 * j + 1] = int(cw[j] >> 16);
     }
-    const float v[kSlabChunk] = {__uint_as_float(q.v0.x), __uint_as_float(q.v0.y), __uint_as_float(q.v0.z),
-                                 __uint_as_float(q.v0.w), __uint_as_float(q.v1.x), __uint_as_float(q.v1.y),
-                                 __uint_as_float(q.v1.z), __uint_as_float(q.v1.w)};
-    float g[kSlabChunk], a[kSlabChunk];
+    const uint32_t vw[8] = {q.v0.x, q.v0.y, q.v0.z, q.v0.w, q.v1.x, q.v1.y, q.v1.z, q.v1.w};
+#pragma unroll
+    for (int j = 0; j < kSlabChunk; ++j) v[j] = __uint_as_float(vw[j]);
+}
+
+// Adds the products of the runs that start in chunk `ch` (registers q) to their
+// rows' running sums. A run sits in one tile, and only the thread of its first
+// chunk touches its row during the tile, so the sums are read up front and
+// each run adds its rounded products in stored order. A run that goes on into
+// the next chunks (its last entry here marked kSlabFollow) is followed there:
+// the entries marked kSlabCont at the head of each later chunk are its own.
+__device__ inline void sum_chunk(const SlabChunk& q, const float* ws, float* acc, const uint4* __restrict__ row,
+                                 const uint4* __restrict__ col, const float* __restrict__ val, int ch) {
+#pragma clang fp contract(off)
+    int r[kSlabChunk], c[kSlabChunk];
+    float v[kSlabChunk], g[kSlabChunk], a[kSlabChunk];
+    unpack_chunk(q, r, c, v);
 #pragma unroll
     for (int j = 0; j < kSlabChunk; ++j) {
-        g[j] = ws[c[j]];                                // padding: column 0, read and unused
-        a[j] = acc[r[j] == int(kSlabPad) ? 0 : r[j]];   // padding: row 0, read and unused
+        g[j] = ws[c[j]];                     // padding: column 0, read and unused
+        a[j] = acc[r[j] & kSlabRowMask];     // padding and continuations: read and unused
     }
-    float s = a[0];
+    float s = 0.0f;
+    int cur = -1;
 #pragma unroll
     for (int j = 0; j < kSlabChunk; ++j) {
-        const bool live = r[j] != int(kSlabPad);
-        if (j > 0 && live && r[j] != r[j - 1]) s = a[j];
+        const bool own = r[j] != int(kSlabPad) && !(r[j] & kSlabCont);
+        const int rr = r[j] & kSlabRowMask;
+        if (own && rr != cur) {
+            if (cur >= 0) acc[cur] = s;
+            cur = rr;
+            s = a[j];
+        }
         const float prod = v[j] * g[j];  // rounded product, as in the CPU sum
-        if (live) s = s + prod;
-        const bool last = live && (j == kSlabChunk - 1 || r[j + 1] != r[j]);
-        if (last) acc[r[j]] = s;
+        if (own) s = s + prod;
     }
+    const int tail = r[kSlabChunk - 1];
+    if (tail != int(kSlabPad) && (tail & kSlabFollow) && !(tail & kSlabCont)) {
+        for (int nc = ch + 1;; ++nc) {
+            SlabChunk n2;
+            load_chunk(n2, row, col, val, nc);
+            int r2[kSlabChunk], c2[kSlabChunk];
+            float v2[kSlabChunk];
+            unpack_chunk(n2, r2, c2, v2);
+            bool more = false;
+            for (int j = 0; j < kSlabChunk; ++j) {
+                if (r2[j] == int(kSlabPad) || !(r2[j] & kSlabCont)) break;
+                const float prod = v2[j] * ws[c2[j]];
+                s = s + prod;
+                more = j == kSlabChunk - 1 && (r2[j] & kSlabFollow);
+            }
+            if (!more) break;
+        }
+    }
+    if (cur >= 0) acc[cur] = s;
 }
 
 // Tile s of block b: waits for slab s and tile s's chunk registers (`cur`),
@@ -130,11 +161,11 @@ __device__ inline void slab_tile(int P, const int* __restrict__ tc, const uint4*
     load_chunk(ahead, row, col, val, min(tc[size_t(b) * P + sa] + tid, last));
     stage_slab(gat + size_t(sa) * kSlabWidth, ahead_buf);
     const int k0 = tc[size_t(b) * P + s], k1 = tc[size_t(b) * P + s + 1];
-    if (k0 + tid < k1) sum_chunk(cur, cur_buf, acc);
+    if (k0 + tid < k1) sum_chunk(cur, cur_buf, acc, row, col, val, k0 + tid);
     for (int ch = k0 + tid + kSlabThreads; ch < k1; ch += kSlabThreads) {  // tiles with more chunks than threads
         SlabChunk extra;
         load_chunk(extra, row, col, val, ch);
-        sum_chunk(extra, cur_buf, acc);
+        sum_chunk(extra, cur_buf, acc, row, col, val, ch);
     }
 }
 

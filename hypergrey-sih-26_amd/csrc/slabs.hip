@@ -1,14 +1,16 @@
 // Slab layout of a single instance's CSR (slabs.hpp), built on the device once
 // per handle:
 //   1. one thread per row counts the row's runs (its entries in one slab) and
-//      checks ascending columns and runs of at most kSlabChunk entries;
+//      checks ascending columns;
 //   2. an exclusive sum gives each row the offset of its run records; one
 //      thread per row writes them: key = tile (block, slab), value = first
 //      entry, row within the block and run length;
 //   3. a stable radix sort by tile puts each tile's runs in row order;
 //   4. one thread per tile finds its runs, then walks them with the packing
-//      rule (a run that does not fit the open chunk starts a new one), giving
-//      the tile's chunk count and each run's slot;
+//      rule, giving the tile's chunk count and each run's slot: a run of at
+//      most kSlabChunk entries that does not fit the open chunk starts a new
+//      one; a longer run starts a new chunk and fills consecutive chunks, its
+//      last one left open for the runs after it;
 //   5. an exclusive sum gives the tiles' first chunks; the output is filled
 //      with padding and one thread per run writes its entries.
 #include "slabs.hpp"
@@ -33,12 +35,12 @@ int grid_for(int64_t items) {
 __device__ inline int64_t thread_id() { return int64_t(blockIdx.x) * kThreads + threadIdx.x; }
 __device__ inline int64_t thread_count() { return int64_t(gridDim.x) * kThreads; }
 
-// Runs of row i; flags a descending column or a run longer than a chunk.
+// Runs of row i; flags a descending column.
 __global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* start, const int* col, int* runs,
                                                        int* bad) {
     for (int64_t i = thread_id(); i < n; i += thread_count()) {
         const int a = start[i], e = start[i + 1];
-        int count = 0, len = 0, prev_slab = -1, prev_col = -1;
+        int count = 0, prev_slab = -1, prev_col = -1;
         for (int k = a; k < e; ++k) {
             const int c = col[k];
             if (c < prev_col) atomicOr(bad, 1);
@@ -46,10 +48,8 @@ __global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* sta
             const int s = c / kSlabWidth;
             if (s != prev_slab) {
                 ++count;
-                len = 0;
                 prev_slab = s;
             }
-            if (++len > kSlabChunk) atomicOr(bad, 2);
         }
         runs[i] = count;
     }
@@ -67,7 +67,7 @@ __device__ inline int block_of(const int* rb, int blocks, int64_t i) {
 }
 
 // Run records in row order: key = b slabs + s, value = (first entry << 32) |
-// (row within the block << 8) | length.
+// (row within the block << 20) | length (a run is at most kSlabWidth long).
 __global__ __launch_bounds__(kThreads) void write_runs(int64_t n, const int* start, const int* col, const int* rb,
                                                        int blocks, int slabs, const int* offset, unsigned* key,
                                                        uint64_t* value) {
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void write_runs(int64_t n, const int* sta
             int q = k + 1;
             while (q < e && col[q] / kSlabWidth == s) ++q;
             key[o] = unsigned(b * slabs + s);
-            value[o] = (uint64_t(uint32_t(k)) << 32) | (local << 8) | uint64_t(q - k);
+            value[o] = (uint64_t(uint32_t(k)) << 32) | (local << 20) | uint64_t(q - k);
             ++o;
             k = q;
         }
@@ -110,7 +110,13 @@ __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const u
     for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
         int count = 0, fill = 0;
         for (int j = first[t]; j < first[t + 1]; ++j) {
-            const int len = int(value[j] & 0xFF);
+            const int len = int(value[j] & 0xFFFFF);
+            if (len > kSlabChunk) {  // whole chunks, then a partial last one left open
+                slot[j] = count * kSlabChunk;
+                count += (len + kSlabChunk - 1) / kSlabChunk;
+                fill = len - (len - 1) / kSlabChunk * kSlabChunk;
+                continue;
+            }
             if (fill == 0 || fill + len > kSlabChunk) {
                 ++count;
                 fill = 0;
@@ -124,6 +130,8 @@ __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const u
 
 // One thread per run: writes its entries at tile tc[t]'s chunks + its slot. The
 // output already holds padding (row kSlabPad, column 0, value 0) everywhere.
+// Entries of a long run past its first chunk carry kSlabCont; the last entry of
+// a chunk whose run goes on into the next chunk carries kSlabFollow.
 __global__ __launch_bounds__(kThreads) void pack_runs(const unsigned* key, const uint64_t* value, const int* slot,
                                                       const int* tc, int records, int slabs, const int* col,
                                                       const double* val, uint16_t* out_row, uint16_t* out_col,
@@ -131,13 +139,14 @@ __global__ __launch_bounds__(kThreads) void pack_runs(const unsigned* key, const
     for (int64_t j = thread_id(); j < records; j += thread_count()) {
         const int t = int(key[j]);
         const uint64_t v = value[j];
-        const int len = int(v & 0xFF);
-        const int local = int((v >> 8) & 0xFFFFFF);
+        const int len = int(v & 0xFFFFF);
+        const int local = int((v >> 20) & 0xFFF);
         const int k = int(v >> 32);
         const int base = (t % slabs) * kSlabWidth;
         const int64_t at = int64_t(tc[t]) * kSlabChunk + slot[j];
         for (int q = 0; q < len; ++q) {
-            out_row[at + q] = uint16_t(local);
+            const bool cont = q >= kSlabChunk, follow = q + 1 < len && (slot[j] + q) % kSlabChunk == kSlabChunk - 1;
+            out_row[at + q] = uint16_t(local | (cont ? kSlabCont : 0u) | (follow ? kSlabFollow : 0u));
             out_col[at + q] = uint16_t(col[k + q] - base);
             out_val[at + q] = float(val[k + q]);  // round to nearest
         }

@@ -6,16 +6,20 @@
 // and at most kSlabRows rows). Columns are split into slabs of kSlabWidth. The
 // tile (block b, slab s) holds the block's entries whose columns lie in slab s,
 // rows in order and each row's entries in stored column order, packed into
-// chunks of kSlabChunk entries: a row's entries in one slab (its run) never
-// straddle two chunks, and the rest of a chunk is padding. Each entry stores
-// its row within the block and its column within the slab as 16-bit values.
+// chunks of kSlabChunk entries. A row's entries in one slab (its run) of at
+// most kSlabChunk never straddle two chunks; a longer run starts a chunk and
+// fills consecutive ones, its entries past the first chunk marked kSlabCont and
+// the last entry of each chunk it continues from marked kSlabFollow. Unused
+// slots are padding. Each entry stores its row within the block (with those
+// marks) and its column within the slab as 16-bit values.
 //
 // A step launch stages slab s of the gathered vector into LDS with direct
 // global-to-LDS copies (every CU reads the vector once per step, in full
 // 1 KiB requests) while it sums the tile's chunks from registers loaded one
-// slab earlier; each row's running sum stays in LDS across the slabs. A thread
-// adds a run's products to the row's sum one by one, so every row sums its
-// entries in stored column order, exactly as the ORDERED CSR pass does.
+// slab earlier; each row's running sum stays in LDS across the slabs. The
+// thread of a run's first chunk adds all of the run's products to the row's
+// sum one by one (following a long run into the next chunks), so every row
+// sums its entries in stored column order, exactly as the ORDERED CSR pass.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,14 +34,17 @@ constexpr int kSlabWidth = 12288;  // columns per slab: 48 KiB of the FP32 vecto
 constexpr int kSlabThreads = 512;  // threads of a slab step launch (8 waves)
 constexpr int kSlabRows = 4096;    // rows per row block: its FP32 sums take 16 KiB of LDS
 constexpr int kSlabChunk = 8;      // entries per chunk
-constexpr unsigned kSlabPad = 0xFFFF;  // row value of a padding entry
+constexpr unsigned kSlabPad = 0xFFFF;     // row value of a padding entry
+constexpr unsigned kSlabCont = 0x8000;    // row mark: an entry of a run whose first chunk is an earlier one
+constexpr unsigned kSlabFollow = 0x4000;  // row mark: the run goes on into the next chunk
+constexpr unsigned kSlabRowMask = 0x0FFF; // row within the block
 
 struct SlabMatrix {
     int blocks = 0;                 // row blocks (workgroups of a launch); 0: no slabs
     int slabs = 0;                  // column slabs
     const int* rb = nullptr;        // [blocks + 1] first row of each block
     const int* tc = nullptr;        // [blocks * slabs + 1] first chunk of tile (b, s) at b * slabs + s
-    const uint4* row = nullptr;     // [chunks] 8 x uint16: row within the block, kSlabPad for padding
+    const uint4* row = nullptr;     // [chunks] 8 x uint16: row within the block and marks, kSlabPad for padding
     const uint4* col = nullptr;     // [chunks] 8 x uint16: column within the slab (0 for padding)
     const float* val = nullptr;     // [chunks * 8] value (0 for padding)
 };
@@ -66,8 +73,7 @@ struct SlabStore {
 
 // Builds the slab layout of the device CSR (start, col, val) over the row blocks
 // `blocks` (from slab_blocks). Returns false and allocates nothing when a row's
-// columns are not ascending or a row has more than kSlabChunk entries in one
-// slab. Synchronizes `stream`.
+// columns are not ascending. Synchronizes `stream`.
 bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const int* start, const int* col,
                  const double* val, SlabStore& out, hipStream_t stream);
 

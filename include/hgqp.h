@@ -99,10 +99,10 @@ typedef struct hgqp_box_options {
                                 (default) uses them for a matrix with ascending
                                 columns that gathers far from its diagonal and
                                 has n >= 393216; 1 never; 2 whenever the matrix
-                                fits (ascending columns, at most 8 entries per
-                                row in a slab of 12288 columns). Slab steps sum
-                                each row in stored column order, so their FP32
-                                iterates equal those of HGQP_REDUCE_ORDERED.   */
+                                fits (ascending columns in every row). Slab
+                                steps sum each row in stored column order, so
+                                their FP32 iterates equal those of
+                                HGQP_REDUCE_ORDERED.                           */
 } hgqp_box_options;
 
 void hgqp_box_options_default(hgqp_box_options* options);

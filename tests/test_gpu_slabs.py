@@ -5,7 +5,8 @@ ORDERED CSR pass does, so a whole FP32 solve with slabs equals the same solve
 without them bit for bit: iterations, restarts, checks, every report field and
 the primal. The cases cover a slab count that is not a multiple of the
 three-buffer ring, a ragged last slab, rows split over several slabs, empty
-rows and tiles with more chunks than a launch has threads. Every call goes
+rows, runs longer than a chunk (followed by the thread of their first chunk)
+and tiles with more chunks than a launch has threads. Every call goes
 through the C ABI of libhgqp.so.
 """
 import numpy as np
@@ -80,6 +81,10 @@ def slab_case(hg, case):
     """(instance, slabs option, slab count expected)"""
     if case == "planted_400k":  # 33 slabs (not a multiple of 3), chosen by the automatic rule
         return planted_qp(hg, 400000, 8, 7)[0], 0, 33
+    if case == "long_runs":  # 3 slabs; rows have up to ~20 entries in one slab, so runs span chunks
+        return planted_qp(hg, 30000, 16, 5)[0], 2, 3
+    if case == "one_slab_rows":  # one slab: every row is one run of up to ~65 entries
+        return planted_qp(hg, 5000, 32, 6)[0], 2, 1
     if case == "planted_ragged":  # 3 slabs, the last 5424 columns wide; forced below the size rule
         return planted_qp(hg, 30000, 1, 3)[0], 2, 3
     if case == "tridiagonal":
@@ -89,7 +94,8 @@ def slab_case(hg, case):
 
 
 @pytest.mark.parametrize("mode", ["refine_restart_jacobi", "polish_reference"])
-@pytest.mark.parametrize("case", ["planted_400k", "planted_ragged", "tridiagonal", "spread_rows"])
+@pytest.mark.parametrize("case", ["planted_400k", "planted_ragged", "tridiagonal", "spread_rows", "long_runs",
+                                  "one_slab_rows"])
 def test_slab_solve_equals_ordered_csr(hg, oracle, case, mode):
     q, slabs, count = slab_case(hg, case)
     data = engine_inputs(oracle, q)
@@ -145,9 +151,10 @@ def test_full_size_slabs(hg):
 
 
 def test_slabs_fall_back(hg, oracle):
-    """Slabs need ascending columns in every row and at most 8 entries of a row
-    in one slab, and serve FP32 steps of a non-diagonal Hessian only; otherwise
-    the handle keeps the CSR passes (slabs32 = 0) and solves as before."""
+    """Slabs need ascending columns in every row and serve FP32 steps of a
+    non-diagonal Hessian only; otherwise the handle keeps the CSR passes
+    (slabs32 = 0) and solves as before. A banded matrix (11 entries of a row in
+    one slab) takes slabs and equals the CSR pass bit for bit."""
     q = planted_qp(hg, 30000, 1, 3)[0]
     s, i, v, c, lo, up, L = engine_inputs(oracle, q)
     fast = dict(modes(hg)["refine_restart_jacobi"], reduce=hg.REDUCE_ORDERED)
@@ -161,7 +168,7 @@ def test_slabs_fall_back(hg, oracle):
     # FP64 handle, and the switch turned off
     assert solve(hg, (s, i, v, c, lo, up, L), slabs=2)[0]["slabs32"] == 0
     assert solve(hg, (s, i, v, c, lo, up, L), slabs=1, **fast)[0]["slabs32"] == 0
-    # a banded matrix: 11 entries of every row in one slab
+    # a banded matrix: 11 entries of almost every row in one slab
     n, hb = 20000, 5
     rng = np.random.default_rng(4)
     hr, hc, hv = [], [], []
@@ -174,8 +181,10 @@ def test_slabs_fall_back(hg, oracle):
     d = np.arange(n)
     band = BoxQP(n, np.concatenate([hr, d]), np.concatenate([hc, d]), np.concatenate([hv, radius + 0.5]),
                  rng.normal(0, 1, size=n), -np.ones(n), np.ones(n))
-    info, rep, _ = solve(hg, engine_inputs(oracle, band), slabs=2, **fast)
-    assert info["slabs32"] == 0 and rep["optimality_verified"]
+    on = solve(hg, engine_inputs(oracle, band), slabs=2, **fast)
+    off = solve(hg, engine_inputs(oracle, band), slabs=1, **fast)
+    assert on[0]["slabs32"] == 2 and on[1]["optimality_verified"]
+    assert_same_solve(on, off)
     # a diagonal Hessian
     dn = 5000
     ds = np.arange(dn + 1, dtype=np.int32)
