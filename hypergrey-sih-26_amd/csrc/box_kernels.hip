@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
 // One FP32 step over column slabs (slab_pass.hpp): the row block's sums come
 // from slab_rows, then each row applies the step epilogue of step_kernel. The
 // restart partials are one per row block.
-template <int MOM>
+template <int MOM, bool LONG>
 __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int slabs, const int* __restrict__ rb,
                                                                  const int* __restrict__ tc,
                                                                  const uint4* __restrict__ row,
@@ -179,9 +179,10 @@ __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int
     epi.beta = float(beta);
     epi.dot = 0;
     epi.resid = 0;
-    const SlabArgs m{blocks, slabs, rb, tc, row, col, val};
-    slab_rows(m, epi.gat, ws0, ws1, ws2, acc);
     const int r0 = rb[blockIdx.x], rows = rb[blockIdx.x + 1] - r0;
+    const int last = tc[size_t(blocks) * slabs] - 1;
+    slab_rows<LONG>(slabs, rows, last, GlobalBounds{tc, size_t(blockIdx.x) * slabs}, row, col, val, epi.gat, ws0, ws1,
+                    ws2, acc);
     for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(r0 + r, acc[r]);
     if (MOM == kMomentumReference) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -583,12 +584,18 @@ template void launch_step<float>(const PanelMatrix<float>&, const StepBuffers<fl
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
                       int momentum, double beta, hipStream_t stream, double switch_stat) {
     dim3 grid(m.blocks), block(kSlabThreads);
-#define HGQP_SLAB_STEP(MOM)                                                                                  \
-    hipLaunchKernelGGL(slab_step_kernel<MOM>, grid, block, 0, stream, m.blocks, m.slabs, m.rb, m.tc, m.row, m.col, \
-                       m.val, b, ctrl, partials, parity, beta)
-    if (momentum == kMomentumImmediate) HGQP_SLAB_STEP(kMomentumImmediate);
-    else if (momentum == kMomentumReference) HGQP_SLAB_STEP(kMomentumReference);
-    else HGQP_SLAB_STEP(kMomentumRestart);
+#define HGQP_SLAB_STEP(MOM, LONG)                                                                              \
+    hipLaunchKernelGGL((slab_step_kernel<MOM, LONG>), grid, block, 0, stream, m.blocks, m.slabs, m.rb, m.tc, m.row, \
+                       m.col, m.val, b, ctrl, partials, parity, beta)
+    if (m.long_runs) {
+        if (momentum == kMomentumImmediate) HGQP_SLAB_STEP(kMomentumImmediate, true);
+        else if (momentum == kMomentumReference) HGQP_SLAB_STEP(kMomentumReference, true);
+        else HGQP_SLAB_STEP(kMomentumRestart, true);
+    } else {
+        if (momentum == kMomentumImmediate) HGQP_SLAB_STEP(kMomentumImmediate, false);
+        else if (momentum == kMomentumReference) HGQP_SLAB_STEP(kMomentumReference, false);
+        else HGQP_SLAB_STEP(kMomentumRestart, false);
+    }
 #undef HGQP_SLAB_STEP
     if (momentum == kMomentumRestart)
         hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity,

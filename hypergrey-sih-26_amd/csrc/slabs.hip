@@ -35,23 +35,27 @@ int grid_for(int64_t items) {
 __device__ inline int64_t thread_id() { return int64_t(blockIdx.x) * kThreads + threadIdx.x; }
 __device__ inline int64_t thread_count() { return int64_t(gridDim.x) * kThreads; }
 
-// Runs of row i; flags a descending column.
+// Runs of row i; flags a descending column (bit 0) and a run longer than a
+// chunk (bit 1).
 __global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* start, const int* col, int* runs,
-                                                       int* bad) {
+                                                       int* flags) {
     for (int64_t i = thread_id(); i < n; i += thread_count()) {
         const int a = start[i], e = start[i + 1];
-        int count = 0, prev_slab = -1, prev_col = -1;
+        int count = 0, len = 0, prev_slab = -1, prev_col = -1, f = 0;
         for (int k = a; k < e; ++k) {
             const int c = col[k];
-            if (c < prev_col) atomicOr(bad, 1);
+            if (c < prev_col) f |= 1;
             prev_col = c;
             const int s = c / kSlabWidth;
             if (s != prev_slab) {
                 ++count;
+                len = 0;
                 prev_slab = s;
             }
+            if (++len > kSlabChunk) f |= 2;
         }
         runs[i] = count;
+        if (f) atomicOr(flags, f);
     }
 }
 
@@ -201,7 +205,8 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     HIPCHECK(hipMemcpyAsync(&host[0], flag.p, 4, hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipMemcpyAsync(&host[1], offset.p + n, 4, hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    if (host[0]) return false;
+    if (host[0] & 1) return false;
+    const int long_runs = (host[0] & 2) ? 1 : 0;
     const int records = host[1];
 
     DeviceTemp<unsigned> key{size_t(records)}, key2{size_t(records)};
@@ -260,6 +265,7 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     out.chunks = total;
     out.m.blocks = nb;
     out.m.slabs = int(P);
+    out.m.long_runs = long_runs;
     out.m.rb = d_rb;
     out.m.tc = d_tc;
     out.m.row = reinterpret_cast<const uint4*>(d_row);

@@ -42,6 +42,7 @@ constexpr unsigned kSlabRowMask = 0x0FFF; // row within the block
 struct SlabMatrix {
     int blocks = 0;                 // row blocks (workgroups of a launch); 0: no slabs
     int slabs = 0;                  // column slabs
+    int long_runs = 0;              // 1: some run is longer than a chunk (marked entries present)
     const int* rb = nullptr;        // [blocks + 1] first row of each block
     const int* tc = nullptr;        // [blocks * slabs + 1] first chunk of tile (b, s) at b * slabs + s
     const uint4* row = nullptr;     // [chunks] 8 x uint16: row within the block and marks, kSlabPad for padding
