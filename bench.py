@@ -205,6 +205,7 @@ def batch_workload(args):
     Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in batch]
     print(f"# batch: {len(batch)} instances, rows {rows}, nnz {nnz}", file=sys.stderr, flush=True)
     pool = hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
+                     slabs=args.pool_slabs,
                      check_every=args.check_every, step_metric=STEP[args.pool_step])
     times, results = [], None
     tuned = None
@@ -253,10 +254,13 @@ def batch_workload(args):
                                f"draws/row uniform in 4..32, bounds l~U(-2,-0.5), u~U(0.5,2), KKT<=1e-6, "
                                f"one pool, graph-replayed check periods; one step = the whole batch",
                    "rows": rows, "nnz": nnz, "autotuned_block_loads": tuned[0], "autotuned_block_rows": tuned[1],
-                   "check_every": args.check_every, "step_metric": args.pool_step},
+                   "check_every": args.check_every, "step_metric": args.pool_step,
+                   "slabs": args.pool_slabs},
         "roofline": {"bound": "hbm", "achieved": round(step_bytes_all / step_us / 1e3, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(step_bytes_all / step_us / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"pool_step<{tuned[0]}> (FP32 step, all {len(batch)} instances active)",
+                     "kernel": (f"pool_slab_step (FP32 step over per-instance column slabs, all {len(batch)} "
+                                f"instances active)" if args.pool_slabs else
+                                f"pool_step<{tuned[0]}> (FP32 step, all {len(batch)} instances active)"),
                      "algorithmic_bytes": step_bytes_all, "avg_launch_us": round(step_us, 1)},
         "all_verified": ok, "worst_max_abs_err_vs_planted_x": worst_x, "worst_objective_rel_err_vs_planted": worst_obj,
         "iterations": {"min": int(iters.min()), "median": float(np.median(iters)), "max": int(iters.max())},
@@ -550,6 +554,8 @@ def main():
     ap.add_argument("--budget-rows", type=int, default=8_000_000, help="serve: pool row budget")
     ap.add_argument("--budget-nnz", type=int, default=160_000_000, help="serve: pool entry budget")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-at-a-time FP64 loop")
+    ap.add_argument("--pool-slabs", type=int, choices=(0, 1), default=1,
+                    help="batch: 1 = FP32 steps over per-instance column slabs, 0 = row blocks")
     ap.add_argument("--pool-step", choices=sorted(STEP), default="jacobi",
                     help="batch/serve: step metric of the pool (per-row Jacobi or the scalar 1/L)")
     ap.add_argument("--no-graph", action="store_true",

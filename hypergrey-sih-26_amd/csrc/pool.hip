@@ -4,6 +4,7 @@
 #include "hgqp.h"
 #include "internal.hpp"
 #include "pool_kernels.hpp"
+#include "slabs.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -62,6 +63,12 @@ struct SlotHost {
     int64_t tag = 0;
     int64_t n = 0, nnz = 0;
     int64_t row0 = -1, ent0 = -1, blk0 = -1, nblk = 0;
+    int64_t rows_len = 0;              // rows held in the row range
+    int64_t sblk0 = -1, nsblk = 0;     // slab blocks (pools with slabs)
+    std::vector<int> srb;              // their first rows (local), then n
+    std::vector<int32_t> srb_stage;    // host staging of their slot and local indices
+    std::shared_ptr<SlabStore> slab;   // the instance's slab layout
+    PoolSlab slab_desc{};              // host copy of its device descriptor
     double* x_out = nullptr;
     int32_t gen = 0;             // admission number, mirrored in the device slot entry
     std::vector<int32_t> start;  // local row starts (kept for re-blocking)
@@ -85,12 +92,18 @@ struct hgqp_pool {
     int* blk_r0 = nullptr;
     int* blk_r1 = nullptr;
     int* blk_inst = nullptr;
+    int* sblk_inst = nullptr;        // slab blocks: owning slot and local block (pools with slabs)
+    int* sblk_local = nullptr;
+    PoolSlab* slab_dev = nullptr;    // per slot
     PoolInst* table_host = nullptr;  // pinned mirror of the slot table
     int32_t* free_phase = nullptr;   // pinned constant kSlotFree (source of slot releases)
     int32_t next_gen = 1;
     hipEvent_t table_event = nullptr;
     bool table_pending = false;
-    RangeAlloc rows, ents, blks;
+    RangeAlloc rows, ents, blks, sblks;
+    // slab layouts of released slots: freed at the next admission (which waits
+    // for the stream anyway), so a release never stalls the queued periods
+    std::vector<std::shared_ptr<SlabStore>> retired;
     std::vector<SlotHost> slots;
     std::vector<int> free_slots;
     hipGraphExec_t graph = nullptr;
@@ -114,18 +127,32 @@ void check_pool(const hgqp_pool* p) {
 }
 
 // Upper bound of row blocks for `rows` rows and `nnz` entries at the smallest
-// block shape (cap 256 entries, 64 rows), plus one per instance.
+// block shape (cap 256 entries, 64 rows), plus one per instance. It also bounds
+// the slab blocks of a pool with slabs (at most 4000 rows and about 64K
+// entries each, plus one per instance).
 int64_t block_budget(int64_t rows, int64_t nnz, int64_t inst) { return nnz / 256 + rows / 64 + 2 * inst + 16; }
 
+// Rows the row allocator hands out: with slabs every instance starts at a
+// multiple of 4 rows (16-byte aligned vectors for the slab copies).
+int64_t row_capacity(const hgqp_pool_options& o) {
+    return o.max_rows + (o.slabs ? 3 * int64_t(o.max_instances) : 0);
+}
+
+// Length of the row vectors: with slabs a slab of padding follows the last
+// row, so every slab copy of an instance stays inside the vector.
+int64_t vector_rows(const hgqp_pool_options& o) { return row_capacity(o) + (o.slabs ? kSlabWidth : 0); }
+
 size_t pool_bytes(const hgqp_pool_options& o) {
-    const size_t R = size_t(o.max_rows), E = size_t(std::max<int64_t>(o.max_nnz, 1));
+    const size_t Ra = size_t(row_capacity(o)), R = size_t(vector_rows(o)), E = size_t(std::max<int64_t>(o.max_nnz, 1));
     const size_t B = size_t(block_budget(o.max_rows, o.max_nnz, o.max_instances)), S = size_t(o.max_instances);
-    return Arena::span(R + 1, 4) + Arena::span(E, 4) + Arena::span(E, 8) + Arena::span(E, 4) +  // start col val val32
-           3 * Arena::span(B, 4) +                                                            // blocks
-           5 * Arena::span(R, 8) +                                                            // c lo up x g
-           8 * Arena::span(R, 4) +                                                            // c32 lo32 up32 d[2] w[2] dstep
-           Arena::span(S, sizeof(PoolInst)) + 2 * Arena::span(B, 4) + Arena::span(4, 4) + Arena::span(S, 4) +
-           Arena::span(B * kPartial, 8) + Arena::span(B, 8);
+    size_t bytes = Arena::span(Ra + 1, 4) + Arena::span(E, 4) + Arena::span(E, 8) + Arena::span(E, 4) +  // start col val val32
+                   3 * Arena::span(B, 4) +                                                             // blocks
+                   5 * Arena::span(R, 8) +                                                             // c lo up x g
+                   8 * Arena::span(R, 4) +                                             // c32 lo32 up32 d[2] w[2] dstep
+                   Arena::span(S, sizeof(PoolInst)) + 2 * Arena::span(B, 4) + Arena::span(4, 4) + Arena::span(S, 4) +
+                   Arena::span(B * kPartial, 8) + Arena::span(B, 8);
+    if (o.slabs) bytes += 2 * Arena::span(B, 4) + Arena::span(S, sizeof(PoolSlab));
+    return bytes;
 }
 
 void validate_options(const hgqp_pool_options& o) {
@@ -145,6 +172,7 @@ void validate_options(const hgqp_pool_options& o) {
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: fp32_switch must be positive");
     if (o.step_metric != HGQP_STEP_SCALAR && o.step_metric != HGQP_STEP_JACOBI)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: unknown step metric");
+    if (o.slabs != 0 && o.slabs != 1) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: slabs must be 0 or 1");
 }
 
 // Computes one slot's row blocks for the pool's block shape into `stage`
@@ -265,6 +293,16 @@ void autotune(hgqp_pool* p) {
     const bool tune_loads = p->opt.block_loads == 0, tune_rows = p->opt.block_rows == 0;
     if (!tune_loads && !tune_rows) return;
     if (!p->resident) return;  // nothing to time: keep the defaults
+    if (p->opt.slabs) {
+        // the steps run over slabs; the row blocks serve the FP64 checks, whose
+        // best shape on the batched workload was 16 loads and 256 rows
+        p->loads = tune_loads ? 16 : p->opt.block_loads;
+        p->rows_per_block = tune_rows ? 256 : p->opt.block_rows;
+        reblock_all(p);
+        launch_pool_lists(p->v, p->stream);
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        return;
+    }
     // Steps are timed only: they run on whatever c32/lo32/up32 and d/w the slots
     // hold, and time_all_active restores the slot table afterwards.
     std::vector<int> loads_c, rows_c;
@@ -300,9 +338,10 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     p->max_blocks = block_budget(o.max_rows, o.max_nnz, o.max_instances);
     HIPCHECK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     p->arena.reserve(pool_bytes(o));
-    const size_t R = size_t(o.max_rows), E = size_t(o.max_nnz), B = size_t(p->max_blocks), S = size_t(o.max_instances);
+    const size_t Ra = size_t(row_capacity(o)), R = size_t(vector_rows(o)), E = size_t(o.max_nnz),
+                 B = size_t(p->max_blocks), S = size_t(o.max_instances);
     PoolView& v = p->v;
-    int* start = p->arena.take<int>(R + 1);
+    int* start = p->arena.take<int>(Ra + 1);
     int* col = p->arena.take<int>(E);
     double* val = p->arena.take<double>(E);
     float* val32 = p->arena.take<float>(E);
@@ -343,6 +382,19 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     v.resid = p->arena.take<double>(B);
     v.switch_stat = o.fp32_switch;
     v.restart = o.momentum == HGQP_MOMENTUM_RESTART ? 1 : 0;
+    v.slabs = o.slabs;
+    if (o.slabs) {
+        p->sblk_inst = p->arena.take<int>(B);
+        p->sblk_local = p->arena.take<int>(B);
+        p->slab_dev = p->arena.take<PoolSlab>(S);
+        v.sblk_inst = p->sblk_inst;
+        v.sblk_local = p->sblk_local;
+        v.slab = p->slab_dev;
+        int dev = 0, cus = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        v.slab_groups = std::max(cus, 1);  // 160 KiB of LDS per workgroup: one per CU
+    }
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->table_host), sizeof(PoolInst) * S, hipHostMallocDefault));
     std::memset(p->table_host, 0, sizeof(PoolInst) * S);  // all kSlotFree
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->free_phase), sizeof(int32_t), hipHostMallocDefault));
@@ -352,9 +404,10 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     HIPCHECK(hipMemcpyAsync(v.counts, zero, sizeof zero, hipMemcpyHostToDevice, p->stream));
     HIPCHECK(hipEventCreateWithFlags(&p->table_event, hipEventDisableTiming));
     HIPCHECK(hipStreamSynchronize(p->stream));
-    p->rows.init(o.max_rows);
+    p->rows.init(row_capacity(o));
     p->ents.init(o.max_nnz);
     p->blks.init(p->max_blocks);
+    p->sblks.init(o.slabs ? p->max_blocks : 0);
     p->slots.resize(S);
     for (int s = int(S) - 1; s >= 0; --s) p->free_slots.push_back(s);
     return p.release();
@@ -400,11 +453,12 @@ bool validate_instance(int64_t n, int64_t nnz, const int32_t* start, const int32
 // budget is free (nothing is held then).
 int reserve(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, int32_t* blk) {
     if (p->free_slots.empty()) return -1;
-    const int64_t r0 = p->rows.alloc(n + 1);
+    const int64_t rows_len = p->opt.slabs ? (n + 1 + 3) / 4 * 4 : n + 1;
+    const int64_t r0 = p->rows.alloc(rows_len);
     if (r0 < 0) return -1;
     const int64_t e0 = p->ents.alloc(std::max<int64_t>(nnz, 1));
     if (e0 < 0) {
-        p->rows.release(r0, n + 1);
+        p->rows.release(r0, rows_len);
         return -1;
     }
     const int s = p->free_slots.back();
@@ -413,10 +467,25 @@ int reserve(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, int32_t*
     h.n = n;
     h.nnz = nnz;
     h.row0 = r0;
+    h.rows_len = rows_len;
     h.ent0 = e0;
+    if (p->opt.slabs) {
+        // slab blocks of at most kPoolSlabRows rows and about 64K entries
+        const int target = int(std::max<int64_t>(1, (nnz + 65535) / 65536));
+        h.srb = slab_blocks(n, start, target, kPoolSlabRows);
+        h.nsblk = int64_t(h.srb.size()) - 1;
+        h.sblk0 = p->sblks.alloc(h.nsblk);
+        if (h.sblk0 < 0) {
+            p->rows.release(r0, rows_len);
+            p->ents.release(e0, std::max<int64_t>(nnz, 1));
+            h = SlotHost();
+            return -1;
+        }
+    }
     if (!place_blocks(p, s, start, blk)) {
-        p->rows.release(r0, n + 1);
+        p->rows.release(r0, rows_len);
         p->ents.release(e0, std::max<int64_t>(nnz, 1));
+        if (h.nsblk) p->sblks.release(h.sblk0, h.nsblk);
         h = SlotHost();
         return -1;
     }
@@ -450,6 +519,8 @@ PoolInst slot_entry(const SlotHost& h, double lipschitz, double tol, uint64_t li
     t.n = int(h.n);
     t.gen = h.gen;
     t.zero_start = zero_start ? 1 : 0;
+    t.sb0 = int(h.sblk0);
+    t.nsblk = int(h.nsblk);
     return t;
 }
 
@@ -468,6 +539,26 @@ void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, con
         HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.val) + h.ent0, value, h.nnz * 8, hipMemcpyHostToDevice, st));
         launch_convert<float, double>(h.nnz, p->v.val + h.ent0, const_cast<float*>(p->v.val32) + h.ent0, st);
     }
+    if (p->opt.slabs) {
+        // the slab layout from the instance's own (local) CSR, before the move to pool numbering
+        SlotHost& hs = p->slots[s];
+        hs.slab = std::make_shared<SlabStore>();
+        if (!build_slabs(h.n, h.nnz, h.srb, dstart, dcol, p->v.val + h.ent0, *hs.slab, st))
+            throw Error(HGQP_DEVICE_ERROR, "internal: slab layout of an admitted instance");
+        const SlabMatrix& m = hs.slab->m;
+        hs.slab_desc = PoolSlab{m.rb, m.tc, m.row, m.col, m.val, m.blocks, m.slabs, int(hs.slab->chunks - 1),
+                                m.long_runs};
+        HIPCHECK(hipMemcpyAsync(p->slab_dev + s, &hs.slab_desc, sizeof(PoolSlab), hipMemcpyHostToDevice, st));
+        std::vector<int32_t>& stage = hs.srb_stage;
+        stage.resize(size_t(2 * h.nsblk));
+        for (int64_t k = 0; k < h.nsblk; ++k) {
+            stage[size_t(k)] = s;
+            stage[size_t(h.nsblk + k)] = int32_t(k);
+        }
+        HIPCHECK(hipMemcpyAsync(p->sblk_inst + h.sblk0, stage.data(), h.nsblk * 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(p->sblk_local + h.sblk0, stage.data() + h.nsblk, h.nsblk * 4, hipMemcpyHostToDevice,
+                                st));
+    }
     launch_pool_offset(dstart, h.n, int(h.ent0), dcol, h.nnz, int(h.row0), st);
     if (p->v.dstep) launch_row_abs(h.n, dstart, p->v.val, entry->lipschitz, nullptr, p->v.dstep + h.row0, st);
     HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.c) + h.row0, cost, h.n * 8, hipMemcpyHostToDevice, st));
@@ -485,6 +576,20 @@ int admit(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, const int3
     bool zero = false;
     const bool diag = validate_instance(n, nnz, start, index, value, cost, lower, upper, x_out, lipschitz, tol,
                                         "hgqp_pool_admit", &zero);
+    if (p->opt.slabs) {
+        if (nnz < 1 || (n + kSlabWidth - 1) / kSlabWidth > kPoolSlabMax)
+            throw Error(HGQP_UNSUPPORTED, "hgqp_pool_admit: a pool with slabs takes instances with stored entries "
+                                          "and n <= 786432");
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t k = start[i] + 1; k < start[i + 1]; ++k)
+                if (index[k] < index[k - 1])
+                    throw Error(HGQP_UNSUPPORTED, "hgqp_pool_admit: a pool with slabs needs ascending columns in "
+                                                  "every row");
+    }
+    if (!p->retired.empty()) {
+        HIPCHECK(hipStreamSynchronize(p->stream));  // no queued launch reads a retired layout any more
+        p->retired.clear();
+    }
     std::vector<int32_t> blk(3 * (n + 1));
     const int s = reserve(p, n, nnz, start, blk.data());
     if (s < 0) return HGQP_BUSY;
@@ -525,9 +630,11 @@ hgqp_pool_result result_of(const SlotHost& h, const PoolInst& t, int s) {
 void release_slot(hgqp_pool* p, int s) {
     SlotHost& h = p->slots[s];
     HIPCHECK(hipMemcpyAsync(&p->v.inst[s].phase, p->free_phase, sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
-    p->rows.release(h.row0, h.n + 1);
+    p->rows.release(h.row0, h.rows_len);
     p->ents.release(h.ent0, std::max<int64_t>(h.nnz, 1));
     p->blks.release(h.blk0, h.nblk);
+    if (h.nsblk) p->sblks.release(h.sblk0, h.nsblk);
+    if (h.slab) p->retired.push_back(std::move(h.slab));  // queued launches may still read it
     h = SlotHost();
     p->free_slots.push_back(s);
     --p->resident;
@@ -916,6 +1023,8 @@ int hgqp_server_start(const hgqp_pool_options* options, hgqp_server** out) {
         hgqp_pool_options o;
         if (options) o = *options;
         else hgqp_pool_options_default(&o);
+        if (o.slabs)
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_start: the server's pool runs without slabs (slabs = 0)");
         auto s = std::make_unique<hgqp_server>();
         s->pool = create_pool(o);
         HIPCHECK(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));

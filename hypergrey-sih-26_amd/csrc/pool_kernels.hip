@@ -16,6 +16,7 @@
 #include "pool_kernels.hpp"
 
 #include <algorithm>
+#include "slab_pass.hpp"
 #include "sparse_pass.hpp"
 
 #include "hgqp.h"
@@ -83,6 +84,69 @@ __global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
     }
 }
 
+// ---- FP32 correction step over the active slab blocks (pools with slabs) ----------
+// One workgroup per slab block in turn (list entries strided over the grid):
+// the block's tile bounds go to LDS, slab_rows sums its rows over the
+// instance's vector (aligned: instances start at a multiple of 4 rows; the
+// vectors hold a slab of padding past the last row), and each row applies the
+// step epilogue of pool_step. The block's restart dot and residual maximum go
+// to its own partial slots.
+__global__ __launch_bounds__(kSlabThreads) void pool_slab_step(PoolView v, int parity) {
+    __shared__ __attribute__((aligned(16))) float ws0[kSlabWidth];
+    __shared__ __attribute__((aligned(16))) float ws1[kSlabWidth];
+    __shared__ __attribute__((aligned(16))) float ws2[kSlabWidth];
+    __shared__ float acc[kPoolSlabRows];
+    __shared__ int tcs[kPoolSlabMax + 1];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int len = v.counts[0];
+    for (int idx = blockIdx.x; idx < len; idx += gridDim.x) {
+        const int sb = uniform(v.list_active[idx]);
+        const int i = uniform(v.sblk_inst[sb]);
+        const PoolInst* s = v.inst + i;
+        if (s->phase != kSlotActive) continue;  // met its target earlier in this period (workgroup-uniform)
+        const PoolSlab D = v.slab[i];
+        const int lb = uniform(v.sblk_local[sb]);
+        double next_t, beta;
+        momentum_of(s->t[parity], &next_t, &beta);
+        StepEpilogue<float> epi;
+        epi.gat = s->use_x[parity] ? v.d[parity] : v.w[parity];
+        epi.xin = v.d[parity];
+        epi.c = v.c32;
+        epi.lo = v.lo32;
+        epi.up = v.up32;
+        epi.xout = v.d[parity ^ 1];
+        epi.wout = v.w[parity ^ 1];
+        epi.lipschitz = float(s->lipschitz);
+        epi.dstep = v.dstep;
+        epi.beta = float(beta);
+        epi.dot = 0;
+        epi.resid = 0;
+        const int lr0 = D.rb[lb], rows = D.rb[lb + 1] - lr0;
+        for (int t = threadIdx.x; t <= D.slabs; t += kSlabThreads) tcs[t] = D.tc[size_t(lb) * D.slabs + t];
+        __syncthreads();  // tile bounds in place; the previous block's reads of ws0 are done
+        slab_rows<true>(D.slabs, rows, D.last, LdsBounds{tcs}, D.row, D.col, D.val, epi.gat + s->r0, ws0, ws1, ws2,
+                        acc);
+        const int g0 = s->r0 + lr0;
+        for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(g0 + r, acc[r]);
+        const double d = wave_sum(epi.dot), res = wave_max(epi.resid);
+        double* red = reinterpret_cast<double*>(ws0);  // free: every thread is past slab_rows' last barrier
+        if (lane == 0) {
+            red[wave] = d;
+            red[kSlabWaves + wave] = res;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0, mx = 0;
+            for (int w = 0; w < kSlabWaves; ++w) {
+                tot += red[w];
+                mx = fmax(mx, red[kSlabWaves + w]);
+            }
+            v.partials[sb] = tot;
+            v.resid[sb] = mx;
+        }
+    }
+}
+
 // Sum over an instance's blocks, in block order, by one workgroup.
 __device__ inline double slot_sum(const double* part, int b0, int nblk, double* red) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -117,8 +181,9 @@ __global__ __launch_bounds__(kGroup) void pool_finalize_step(PoolView v, int par
     __shared__ double red2[kWavesPerGroup];
     PoolInst* s = v.inst + blockIdx.x;
     if (s->phase != kSlotActive) return;
-    const double tot = slot_sum(v.partials, s->b0, s->nblk, red);
-    const double res = slot_max(v.resid, s->b0, s->nblk, red2);
+    const int b0 = v.slabs ? s->sb0 : s->b0, nb = v.slabs ? s->nsblk : s->nblk;
+    const double tot = slot_sum(v.partials, b0, nb, red);
+    const double res = slot_max(v.resid, b0, nb, red2);
     if (threadIdx.x != 0) return;
     double next_t, beta;
     momentum_of(s->t[parity], &next_t, &beta);
@@ -142,6 +207,12 @@ __device__ inline bool selected(int phase, int which) {
     return which == 0 ? phase == kSlotActive : (phase == kSlotInit || phase == kSlotVerify);
 }
 
+// The blocks a list names for slot s: slab blocks for the step list of a pool
+// with slabs, row blocks otherwise.
+__device__ inline int list_blocks(const PoolView& v, const PoolInst& s, int which) {
+    return which == 0 && v.slabs ? s.nsblk : s.nblk;
+}
+
 // One workgroup: exclusive prefix of the selected slots' block counts in slot order.
 __global__ __launch_bounds__(kListThreads) void pool_list_count(PoolView v, int which) {
     __shared__ int sums[kListThreads];
@@ -149,7 +220,7 @@ __global__ __launch_bounds__(kListThreads) void pool_list_count(PoolView v, int 
     const int per = (v.max_inst + kListThreads - 1) / kListThreads;
     const int a = t * per, e = min(v.max_inst, a + per);
     int local = 0;
-    for (int i = a; i < e; ++i) local += selected(v.inst[i].phase, which) ? v.inst[i].nblk : 0;
+    for (int i = a; i < e; ++i) local += selected(v.inst[i].phase, which) ? list_blocks(v, v.inst[i], which) : 0;
     sums[t] = local;
     __syncthreads();
     for (int off = 1; off < kListThreads; off <<= 1) {  // inclusive Hillis-Steele scan
@@ -161,7 +232,7 @@ __global__ __launch_bounds__(kListThreads) void pool_list_count(PoolView v, int 
     int run = t ? sums[t - 1] : 0;
     for (int i = a; i < e; ++i) {
         v.offsets[i] = run;
-        run += selected(v.inst[i].phase, which) ? v.inst[i].nblk : 0;
+        run += selected(v.inst[i].phase, which) ? list_blocks(v, v.inst[i], which) : 0;
     }
     if (t == kListThreads - 1) v.counts[which] = sums[t];
 }
@@ -171,7 +242,8 @@ __global__ __launch_bounds__(kGroup) void pool_list_expand(PoolView v, int which
     if (!selected(s->phase, which)) return;
     int* list = which == 0 ? v.list_active : v.list_verify;
     const int o = v.offsets[blockIdx.x];
-    for (int k = threadIdx.x; k < s->nblk; k += kGroup) list[o + k] = s->b0 + k;
+    const int b0 = which == 0 && v.slabs ? s->sb0 : s->b0, nb = list_blocks(v, *s, which);
+    for (int k = threadIdx.x; k < nb; k += kGroup) list[o + k] = b0 + k;
 }
 
 // ---- FP64 verification of new and converging slots ----------------------------------
@@ -324,7 +396,10 @@ void period_impl(const PoolView& v, int max_blocks, int steps, hipStream_t strea
     const dim3 grid(grid_for(max_blocks)), block(kGroup), slots(v.max_inst);
     int parity = 0;
     for (int k = 0; k < steps; ++k) {
-        hipLaunchKernelGGL(pool_step<LOADS>, grid, block, 0, stream, v, parity);
+        if (v.slabs)
+            hipLaunchKernelGGL(pool_slab_step, dim3(v.slab_groups), dim3(kSlabThreads), 0, stream, v, parity);
+        else
+            hipLaunchKernelGGL(pool_step<LOADS>, grid, block, 0, stream, v, parity);
         hipLaunchKernelGGL(pool_finalize_step, slots, block, 0, stream, v, parity);
         parity ^= 1;
     }
@@ -359,7 +434,9 @@ void launch_pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, in
 void launch_pool_steps(const PoolView& v, int loads, int max_blocks, int reps, hipStream_t stream) {
     const dim3 grid(grid_for(max_blocks)), block(kGroup);
     for (int k = 0; k < reps; ++k) {
-        if (loads == 4) hipLaunchKernelGGL(pool_step<4>, grid, block, 0, stream, v, k & 1);
+        if (v.slabs)
+            hipLaunchKernelGGL(pool_slab_step, dim3(v.slab_groups), dim3(kSlabThreads), 0, stream, v, k & 1);
+        else if (loads == 4) hipLaunchKernelGGL(pool_step<4>, grid, block, 0, stream, v, k & 1);
         else if (loads == 16) hipLaunchKernelGGL(pool_step<16>, grid, block, 0, stream, v, k & 1);
         else hipLaunchKernelGGL(pool_step<8>, grid, block, 0, stream, v, k & 1);
     }

@@ -41,7 +41,23 @@ struct PoolInst {
     int32_t frozen;    // parity of the correction a waiting slot stopped at
     int32_t gen;       // admission number of the slot's instance (host bookkeeping)
     int32_t zero_start;  // 0 lies in every box: x0 = 0, so the first check needs no matrix pass
+    int32_t sb0, nsblk;  // slab block range (pools with slabs)
 };
+
+// Slab layout of one resident instance (pools with slabs; slabs.hpp): rows and
+// columns local to the instance. The FP32 steps of such a pool run over these
+// instead of the row blocks; the FP64 checks keep the row blocks.
+struct PoolSlab {
+    const int* rb;
+    const int* tc;
+    const uint4* row;
+    const uint4* col;
+    const float* val;
+    int blocks, slabs, last, long_runs;  // last: the instance's last chunk
+};
+
+constexpr int kPoolSlabRows = 4000;  // rows per slab block in a pool (LDS also holds the block's tile bounds)
+constexpr int kPoolSlabMax = 64;     // slabs per pool instance: n <= 64 * 12288
 
 struct PoolView {
     // matrix (block-diagonal concatenation; columns carry the instance row offset)
@@ -76,6 +92,12 @@ struct PoolView {
     double* resid;     // per block: the step's residual maximum
     double switch_stat;
     int restart;       // 1: gradient restart, 0: reference momentum
+    // slab mode: the active list holds slab blocks (instance, local block)
+    int slabs;
+    int slab_groups;          // workgroups of a slab step launch (one per CU)
+    const int* sblk_inst;
+    const int* sblk_local;
+    const PoolSlab* slab;     // [max_inst]
 };
 
 // One check period: `steps` FP32 correction steps of every active slot (a slot

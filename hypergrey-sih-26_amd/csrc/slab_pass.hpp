@@ -58,13 +58,25 @@ struct SlabChunk {
     uint4 r, c, v0, v1;
 };
 
+// Chunk loads go through global-space pointers: a pointer read from memory
+// (the pool's per-instance layout) would otherwise become a flat load, which
+// also counts against lgkmcnt and would be drained by every barrier.
+struct alignas(16) Words4 {
+    uint32_t x, y, z, w;
+};
+typedef const __attribute__((address_space(1))) Words4* GlobalWords4;
+
+__device__ __forceinline__ uint4 load_words(const void* base, size_t i) {
+    GlobalWords4 p = (GlobalWords4)base + i;
+    return make_uint4(p->x, p->y, p->z, p->w);  // one 16-byte global load
+}
+
 __device__ __forceinline__ void load_chunk(SlabChunk& q, const uint4* __restrict__ row, const uint4* __restrict__ col,
                                   const float* __restrict__ val, int ch) {
-    q.r = row[ch];
-    q.c = col[ch];
-    const uint4* v = reinterpret_cast<const uint4*>(val) + 2 * size_t(ch);
-    q.v0 = v[0];
-    q.v1 = v[1];
+    q.r = load_words(row, size_t(ch));
+    q.c = load_words(col, size_t(ch));
+    q.v0 = load_words(val, 2 * size_t(ch));
+    q.v1 = load_words(val, 2 * size_t(ch) + 1);
 }
 
 __device__ __forceinline__ void unpack_chunk(const SlabChunk& q, int* r, int* c, float* v) {
@@ -141,7 +153,7 @@ __device__ __forceinline__ void sum_chunk(const SlabChunk& q, const float* ws, f
 #pragma unroll
     for (int j = 0; j < kSlabChunk; ++j) {
         g[j] = ws[c[j]];                     // padding: column 0, read and unused
-        a[j] = acc[r[j] & kSlabRowMask];     // padding and continuations: read and unused
+        a[j] = acc[r[j] == int(kSlabPad) ? 0 : (r[j] & kSlabRowMask)];  // padding, continuations: read, unused
     }
     float s = 0.0f;
     int cur = -1;
@@ -202,8 +214,9 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
 
 // Row sums of one row block over gat (the block's instance vector, holding
 // whole slabs): P slabs, R rows, `last` the matrix's last chunk, tb the block's
-// tile bounds; LONG when some run of the matrix is longer than a chunk. On return acc[r] holds row r's sum for r < R, every copy has
-// landed and all threads have passed a barrier.
+// tile bounds; LONG when some run of the matrix is longer than a chunk. On
+// return acc[r] holds row r's sum for r < R, every copy has landed and all
+// threads have passed a barrier.
 template <bool LONG, class Bounds>
 __device__ __forceinline__ void slab_rows(int P, int R, int last, const Bounds& tb, const uint4* __restrict__ row,
                                  const uint4* __restrict__ col, const float* __restrict__ val,

@@ -165,7 +165,7 @@ int bits_for(int64_t v) {
 
 }  // namespace
 
-std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target) {
+std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target, int max_rows) {
     const int64_t nnz = start[n];
     std::vector<int> rb{0};
     int64_t r = 0;
@@ -173,7 +173,7 @@ std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target) {
         const int64_t rs = r;
         const int64_t goal = nnz * int64_t(rb.size()) / std::max(target, 1);
         ++r;  // every block takes at least one row
-        while (r < n && r - rs < kSlabRows && start[r + 1] <= goal) ++r;
+        while (r < n && r - rs < max_rows && start[r + 1] <= goal) ++r;
         rb.push_back(int(r));
     }
     return rb;

@@ -54,9 +54,9 @@ struct SlabMatrix {
 inline int64_t slab_padded(int64_t n) { return (n + kSlabWidth - 1) / kSlabWidth * kSlabWidth; }
 
 // Row blocks from the host row starts: block j ends at about (j + 1) nnz / target
-// entries, with at most kSlabRows rows. Returns the first row of every block
-// followed by n.
-std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target);
+// entries, with at most max_rows (<= kSlabRows) rows. Returns the first row of
+// every block followed by n.
+std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target, int max_rows = kSlabRows);
 
 // Device storage of a slab matrix, owned by a handle.
 struct SlabStore {

@@ -391,7 +391,12 @@ typedef struct hgqp_pool_options {
     double fp32_switch;      /* first FP32 cycle ends at this multiple of the
                                 tolerance (default 10)                           */
     int32_t step_metric;     /* HGQP_STEP_JACOBI (default) or _SCALAR            */
-    int32_t reserved;
+    int32_t slabs;           /* 1: FP32 steps over per-instance column slabs
+                                staged in LDS (each admission builds the
+                                instance's layout on the device and waits for
+                                it; instances need ascending columns, stored
+                                entries and n <= 786432); 0 (default): row
+                                blocks. The server needs 0.                      */
 } hgqp_pool_options;
 
 void hgqp_pool_options_default(hgqp_pool_options* options);

@@ -90,6 +90,9 @@ def test_invalid_input_rejected_before_device(hg):
                 {"slabs": -1}):
         with pytest.raises(hg.InvalidArgument):
             hg.Box(*args, 5.0, **opt)
+    for opt in ({"slabs": 2}, {"slabs": -1}, {"check_every": 3}):
+        with pytest.raises(hg.InvalidArgument):
+            hg.Pool(**opt)
     for kw in ({"threads": 0}, {"tolerance": 0.0}, {"time_limit": -1.0}):
         with pytest.raises(hg.InvalidArgument):
             hg.solve_qp(2, [0, 1], [0, 1], [2.0, 2.0], [-2.0, -4.0], [-10, -10], [10, 10], **kw)
