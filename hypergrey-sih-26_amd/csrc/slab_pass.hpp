@@ -96,26 +96,33 @@ __device__ __forceinline__ void unpack_chunk(const SlabChunk& q, int* r, int* c,
 
 // The rest of a run that goes on past its first chunk: adds the entries marked
 // kSlabCont at the head of chunk nc and of each later chunk the run reaches,
-// in stored order, to the running sum s. Rarely taken, so it stays out of line
-// and its loads and registers do not enter the pipelined tile loop.
+// in stored order, to the running sum s. The loads of four chunks are issued
+// together (indices clamped to the matrix's last chunk; those past the run go
+// unused). Rarely taken in a matrix of short runs, so it stays out of line and
+// its loads and registers do not enter the pipelined tile loop.
 __device__ __attribute__((noinline)) float follow_run(float s, const float* ws, const uint4* __restrict__ row,
                                                       const uint4* __restrict__ col, const float* __restrict__ val,
-                                                      int nc) {
+                                                      int nc, int last) {
 #pragma clang fp contract(off)
-    for (;; ++nc) {
-        SlabChunk q;
-        load_chunk(q, row, col, val, nc);
-        int r[kSlabChunk], c[kSlabChunk];
-        float v[kSlabChunk];
-        unpack_chunk(q, r, c, v);
-        bool more = false;
-        for (int j = 0; j < kSlabChunk; ++j) {
-            if (r[j] == int(kSlabPad) || !(r[j] & kSlabCont)) break;
-            const float prod = v[j] * ws[c[j]];
-            s = s + prod;
-            more = j == kSlabChunk - 1 && (r[j] & kSlabFollow);
+    constexpr int kAhead = 4;
+    for (;; nc += kAhead) {
+        SlabChunk q[kAhead];
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a) load_chunk(q[a], row, col, val, min(nc + a, last));
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a) {
+            int r[kSlabChunk], c[kSlabChunk];
+            float v[kSlabChunk];
+            unpack_chunk(q[a], r, c, v);
+            bool more = false;
+            for (int j = 0; j < kSlabChunk; ++j) {
+                if (r[j] == int(kSlabPad) || !(r[j] & kSlabCont)) break;
+                const float prod = v[j] * ws[c[j]];
+                s = s + prod;
+                more = j == kSlabChunk - 1 && (r[j] & kSlabFollow);
+            }
+            if (!more) return s;
         }
-        if (!more) return s;
     }
 }
 
@@ -127,7 +134,7 @@ __device__ __attribute__((noinline)) float follow_run(float s, const float* ws, 
 // the entries marked kSlabCont at the head of each later chunk are its own.
 template <bool LONG>
 __device__ __forceinline__ void sum_chunk(const SlabChunk& q, const float* ws, float* acc, const uint4* __restrict__ row,
-                                 const uint4* __restrict__ col, const float* __restrict__ val, int ch) {
+                                 const uint4* __restrict__ col, const float* __restrict__ val, int ch, int last) {
 #pragma clang fp contract(off)
     int r[kSlabChunk], c[kSlabChunk];
     float v[kSlabChunk], g[kSlabChunk], a[kSlabChunk];
@@ -171,7 +178,7 @@ __device__ __forceinline__ void sum_chunk(const SlabChunk& q, const float* ws, f
     }
     const int tail = r[kSlabChunk - 1];
     if (tail != int(kSlabPad) && (tail & kSlabFollow) && !(tail & kSlabCont))
-        s = follow_run(s, ws, row, col, val, ch + 1);
+        s = follow_run(s, ws, row, col, val, ch + 1, last);
     if (cur >= 0) acc[cur] = s;
 }
 
@@ -204,11 +211,21 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
     load_chunk(ahead, row, col, val, min(tb(sa) + tid, last));
     stage_slab(gat + size_t(sa) * kSlabWidth, ahead_buf);
     const int k0 = tb(s), k1 = tb(s + 1);
-    if (k0 + tid < k1) sum_chunk<LONG>(cur, cur_buf, acc, row, col, val, k0 + tid);
-    for (int ch = k0 + tid + kSlabThreads; ch < k1; ch += kSlabThreads) {  // tiles with more chunks than threads
-        SlabChunk extra;
-        load_chunk(extra, row, col, val, ch);
-        sum_chunk<LONG>(extra, cur_buf, acc, row, col, val, ch);
+    if (k0 + tid < k1) sum_chunk<LONG>(cur, cur_buf, acc, row, col, val, k0 + tid, last);
+    // tiles with more chunks than threads: each further chunk's loads are issued
+    // before the previous one is summed
+    int ch = k0 + tid + kSlabThreads;
+    if (ch < k1) {
+        SlabChunk nx;
+        load_chunk(nx, row, col, val, ch);
+        for (;;) {
+            const SlabChunk here = nx;
+            const int nch = ch + kSlabThreads;
+            if (nch < k1) load_chunk(nx, row, col, val, nch);
+            sum_chunk<LONG>(here, cur_buf, acc, row, col, val, ch, last);
+            if (nch >= k1) break;
+            ch = nch;
+        }
     }
 }
 
