@@ -207,12 +207,14 @@ def batch_workload(args):
     pool = hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
                      slabs=args.pool_slabs,
                      check_every=args.check_every, step_metric=STEP[args.pool_step])
-    times, results = [], None
+    times, results, admits = [], None, []
     tuned = None
     try:
         for rep in range(args.warmup + args.steps):
+            a = time.perf_counter()
             for k, d in enumerate(batch):
                 assert pool.admit(d, tag=k, tolerance=KKT_TOL, lipschitz_value=Ls[k]) is not None
+            admits.append(time.perf_counter() - a)
             if tuned is None:
                 tuned = pool.autotune()
                 step_us, step_bytes_all = pool.time_step(6)  # all 512 instances active, before the solve
@@ -264,6 +266,7 @@ def batch_workload(args):
                      "algorithmic_bytes": step_bytes_all, "avg_launch_us": round(step_us, 1)},
         "all_verified": ok, "worst_max_abs_err_vs_planted_x": worst_x, "worst_objective_rel_err_vs_planted": worst_obj,
         "iterations": {"min": int(iters.min()), "median": float(np.median(iters)), "max": int(iters.max())},
+        "admission_ms": round(float(np.median(admits)) * 1e3, 1),
         "plain_fp64_single_instance": {"solves_per_s": round(plain_rate, 2), "instances_timed": plain_n,
                                        "all_verified": all(p["status"] == "optimal" for p in plain)},
         "speedup_vs_plain_fp64": round(len(batch) / wall / plain_rate, 2),

@@ -111,22 +111,33 @@ __global__ __launch_bounds__(kThreads) void tile_first(const unsigned* key, int 
 // (local chunk * kSlabChunk + first entry within the chunk).
 __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const uint64_t* value, int tiles, int* chunks,
                                                        int* slot) {
+    constexpr int kBatch = 16;  // run lengths read per round, so the walk does not wait on each load
     for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
         int count = 0, fill = 0;
-        for (int j = first[t]; j < first[t + 1]; ++j) {
-            const int len = int(value[j] & 0xFFFFF);
-            if (len > kSlabChunk) {  // whole chunks, then a partial last one left open
-                slot[j] = count * kSlabChunk;
-                count += (len + kSlabChunk - 1) / kSlabChunk;
-                fill = len - (len - 1) / kSlabChunk * kSlabChunk;
-                continue;
+        const int j0 = first[t], j1 = first[t + 1];
+        for (int jb = j0; jb < j1; jb += kBatch) {
+            int lens[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) lens[u] = jb + u < j1 ? int(value[jb + u] & 0xFFFFF) : 0;
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                if (jb + u >= j1) break;
+                const int len = lens[u];
+                int at;
+                if (len > kSlabChunk) {  // whole chunks, then a partial last one left open
+                    at = count * kSlabChunk;
+                    count += (len + kSlabChunk - 1) / kSlabChunk;
+                    fill = len - (len - 1) / kSlabChunk * kSlabChunk;
+                } else {
+                    if (fill == 0 || fill + len > kSlabChunk) {
+                        ++count;
+                        fill = 0;
+                    }
+                    at = (count - 1) * kSlabChunk + fill;
+                    fill += len;
+                }
+                slot[jb + u] = at;
             }
-            if (fill == 0 || fill + len > kSlabChunk) {
-                ++count;
-                fill = 0;
-            }
-            slot[j] = (count - 1) * kSlabChunk + fill;
-            fill += len;
         }
         chunks[t] = count;
     }
