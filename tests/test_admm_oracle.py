@@ -6,7 +6,7 @@ mps.cpp:279-304), pinned:
      (infinite row and variable bounds, zero duals, maximization, offset, a
      diagonal Hessian with its separable bound);
   2. against a hand-computed KKT point (closed form).
-The GPU ADMM's results are judged by this check in test_gpu_admm.py."""
+The GPU ADMM's results are checked by it in test_gpu_admm.py."""
 import numpy as np
 import pytest
 

@@ -1,6 +1,6 @@
 """GPU ADMM for convex QPs with linear rows (hgqp_solve_qp_admm, restating
 solve_qp_admm of qp_admm.cpp:118-281 without its polish and diagonal-equality
-fast path), judged by the oracle's restated check_qp_solution with rows (pinned
+fast path), checked by the oracle's restated check_qp_solution with rows (pinned
 bit for bit to the compiled reference in test_admm_oracle.py) on the returned
 point and row duals, and compared with the reference's own ADMM where
 oracle/_ref is built."""
