@@ -89,13 +89,15 @@ def slab_case(hg, case):
         return planted_qp(hg, 30000, 1, 3)[0], 2, 3
     if case == "tridiagonal":
         return tridiagonal_qp(hg, 2000)[0], 2, 1
+    if case == "whole_slabs":  # n = 2 x 12288: no ragged slab
+        return planted_qp(hg, 2 * SLAB_WIDTH, 2, 8)[0], 2, 2
     # rows with entries in every slab, empty rows, 4 slabs
     return spread_rows_qp(40000, (5, 20000, 39999), 3001, (10, 11, 30000), 9), 2, 4
 
 
 @pytest.mark.parametrize("mode", ["refine_restart_jacobi", "polish_reference"])
 @pytest.mark.parametrize("case", ["planted_400k", "planted_ragged", "tridiagonal", "spread_rows", "long_runs",
-                                  "one_slab_rows"])
+                                  "one_slab_rows", "whole_slabs"])
 def test_slab_solve_equals_ordered_csr(hg, oracle, case, mode):
     q, slabs, count = slab_case(hg, case)
     data = engine_inputs(oracle, q)
