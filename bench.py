@@ -53,7 +53,7 @@ def step_bytes(n, nnz, value_bytes, vector_bytes, jacobi=False):
     return nnz * (value_bytes + 4) + 4 * (n + 1) + (8 if jacobi else 7) * n * vector_bytes
 
 
-def measured_traffic(use32, mode, n, draws, nnz, panels, slabs):
+def measured_traffic(use32, mode, n, draws, nnz, panels, slabs, groups):
     """HBM bytes per FP32 step (all of its launches) from the committed
     rocprofv3 PMC summary (FETCH_SIZE x 2 + WRITE_SIZE, calibrated on plain
     streaming kernels), when this run's mode, instance, panel and slab counts
@@ -64,7 +64,8 @@ def measured_traffic(use32, mode, n, draws, nnz, panels, slabs):
     with open(path) as f:
         t = json.load(f)
     if (t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or t.get("mode") != mode
-            or t.get("column_panels32") != panels or t.get("slabs32", 0) != slabs):
+            or t.get("column_panels32") != panels or t.get("slabs32", 0) != slabs
+            or t.get("slab_groups", 1 if slabs else 0) != groups):
         return None
     return t["traffic_bytes_per_step"]
 
@@ -106,7 +107,8 @@ def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
     finally:
         timed.close()
     return dict(name=name, wall=wall, lat=lat, reps=reps, timed=treps, x=x,
-                panels=(info["column_panels"], info["column_panels32"]), slabs=info["slabs32"])
+                panels=(info["column_panels"], info["column_panels32"]), slabs=info["slabs32"],
+                slab_groups=info["slab_groups"])
 
 
 def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
@@ -630,6 +632,7 @@ def main():
             "fp64_step_us": f64_ms / f64_n * 1e3 if f64_n else None,
             "fp32_step_us": f32_ms / f32_n * 1e3 if f32_n else None,
             "column_panels": r["panels"][0], "column_panels32": r["panels"][1], "slabs32": r["slabs"],
+            "slab_groups": r["slab_groups"],
         }
         if ref_x is not None:
             m["max_abs_err_vs_fp64_reference_x"] = float(np.abs(r["x"] - ref_x).max())
@@ -647,8 +650,12 @@ def main():
     achieved = nbytes / (us * 1e-6) / 1e9
     panels = hm["column_panels32"] if use32 else hm["column_panels"]
     slabs = hm["slabs32"] if use32 else 0
+    groups = hm["slab_groups"] if use32 else 0
     tname = "float" if use32 else "double"
-    if slabs:
+    if slabs and groups > 1:
+        kernel = (f"slab_sum_kernel + slab_finish_kernel (one FP32 step over {slabs} column slabs staged in LDS, "
+                  f"two workgroups per row block)")
+    elif slabs:
         kernel = f"slab_step_kernel (one FP32 step over {slabs} column slabs staged in LDS)"
     elif panels == 1:
         kernel = f"step_kernel<{tname}>"
@@ -656,9 +663,9 @@ def main():
         kernel = f"step_kernel<{tname}> + {panels - 1} carry_kernel<{tname}> (one step over {panels} column panels)"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(use32, head, n, args.draws, nnz, panels, slabs),
+                "traffic": measured_traffic(use32, head, n, args.draws, nnz, panels, slabs, groups),
                 "kernel": kernel, "algorithmic_bytes": nbytes, "avg_launch_us": round(us, 2),
-                "column_panels": panels, "slabs": slabs}
+                "column_panels": panels, "slabs": slabs, "slab_groups": groups}
     # the FP64 reference-schedule step beside it
     if "fp64_reference" in modes and modes["fp64_reference"]["fp64_step_us"]:
         b64 = step_bytes(n, nnz, 8, 8)

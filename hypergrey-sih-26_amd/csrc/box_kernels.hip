@@ -143,29 +143,9 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     }
 }
 
-// One FP32 step over column slabs (slab_pass.hpp): the row block's sums come
-// from slab_rows, then each row applies the step epilogue of step_kernel. The
-// restart partials are one per row block.
-template <int MOM, bool LONG>
-__global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int slabs, const int* __restrict__ rb,
-                                                                 const int* __restrict__ tc,
-                                                                 const uint4* __restrict__ row,
-                                                                 const uint4* __restrict__ col,
-                                                                 const float* __restrict__ val,
-                                                                 StepBuffers<float> bufs, Control* ctrl,
-                                                                 double* partials, int parity, double beta_imm) {
-    __shared__ __attribute__((aligned(16))) float ws0[kSlabWidth];
-    __shared__ __attribute__((aligned(16))) float ws1[kSlabWidth];
-    __shared__ __attribute__((aligned(16))) float ws2[kSlabWidth];
-    __shared__ float acc[kSlabRows];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    double beta = beta_imm, next_t = 0;
-    int use_x = 0;
-    if (MOM != kMomentumImmediate) {
-        if (ctrl->done) return;  // grid-uniform
-        momentum_of(ctrl->t[parity], &next_t, &beta);
-        use_x = ctrl->use_x[parity];
-    }
+// Step epilogue of an FP32 slab step at the step's source point.
+__device__ inline StepEpilogue<float> slab_epilogue(const StepBuffers<float>& bufs, int parity, int use_x,
+                                                    double beta) {
     StepEpilogue<float> epi;
     epi.gat = use_x ? bufs.x[parity] : bufs.w[parity];
     epi.xin = bufs.x[parity];
@@ -179,11 +159,16 @@ __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int
     epi.beta = float(beta);
     epi.dot = 0;
     epi.resid = 0;
-    const int r0 = rb[blockIdx.x], rows = rb[blockIdx.x + 1] - r0;
-    const int last = tc[size_t(blocks) * slabs] - 1;
-    slab_rows<LONG>(slabs, rows, last, GlobalBounds{tc, size_t(blockIdx.x) * slabs}, row, col, val, epi.gat, ws0, ws1,
-                    ws2, acc);
-    for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(r0 + r, acc[r]);
+    return epi;
+}
+
+// End of a slab step's epilogue workgroup (kSlabThreads threads): the momentum
+// state of the next step (reference schedule), or this workgroup's restart
+// partials (the dot product and the residual maximum, which finalize_restart
+// combines). `red` is free LDS for 2 kSlabWaves doubles.
+template <int MOM>
+__device__ inline void slab_step_end(const StepEpilogue<float>& epi, Control* ctrl, double* partials, int parity,
+                                     double next_t, double* red) {
     if (MOM == kMomentumReference) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctrl->t[parity ^ 1] = next_t;
@@ -191,8 +176,8 @@ __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int
             ctrl->use_x[parity ^ 1] = 0;
         }
     } else if (MOM == kMomentumRestart) {
+        const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
         const double d = wave_sum(epi.dot), rr = wave_max(epi.resid);
-        double* red = reinterpret_cast<double*>(ws0);  // free: every thread is past slab_rows' last barrier
         if (lane == 0) {
             red[wave] = d;
             red[kSlabWaves + wave] = rr;
@@ -210,24 +195,92 @@ __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int
     }
 }
 
-// Restart decision of a step launch: the sum of the step's workgroup partials in
-// workgroup order (deterministic), then the t / extrapolation state of the next
-// step. One workgroup; runs after the step launch on the same stream.
+// One FP32 step over column slabs (slab_pass.hpp): the row block's sums come
+// from slab_rows, then each row applies the step epilogue of step_kernel. The
+// restart partials are one per row block.
+template <int MOM, bool LONG>
+__global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int slabs, const int* __restrict__ rb,
+                                                                 const int* __restrict__ tc,
+                                                                 const uint4* __restrict__ row,
+                                                                 const uint4* __restrict__ col,
+                                                                 const float* __restrict__ val,
+                                                                 StepBuffers<float> bufs, Control* ctrl,
+                                                                 double* partials, int parity, double beta_imm) {
+    __shared__ __attribute__((aligned(16))) float ws0[kSlabWidth];
+    __shared__ __attribute__((aligned(16))) float ws1[kSlabWidth];
+    __shared__ __attribute__((aligned(16))) float ws2[kSlabWidth];
+    __shared__ float acc[kSlabRows];
+    double beta = beta_imm, next_t = 0;
+    int use_x = 0;
+    if (MOM != kMomentumImmediate) {
+        if (ctrl->done) return;  // grid-uniform
+        momentum_of(ctrl->t[parity], &next_t, &beta);
+        use_x = ctrl->use_x[parity];
+    }
+    StepEpilogue<float> epi = slab_epilogue(bufs, parity, use_x, beta);
+    const int r0 = rb[blockIdx.x], rows = rb[blockIdx.x + 1] - r0;
+    const int last = tc[size_t(blocks) * slabs] - 1;
+    slab_rows<kSlabWidth, LONG>(slabs, rows, last, GlobalBounds{tc, size_t(blockIdx.x) * slabs}, row, col, val,
+                                epi.gat, ws0, ws1, ws2, acc);
+    for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(r0 + r, acc[r]);
+    // ws0 is free: every thread is past slab_rows' last barrier
+    slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, reinterpret_cast<double*>(ws0));
+}
+
+// Split layout, first launch of a step: workgroup G b + h sums row block b's
+// entries in the h-th range of its slabs (slab_range_start) and stores the
+// block's partial row sums at part[h stride + row]. Workgroups are dealt to
+// the XCDs in turn, so the G ranges run on different XCDs and each XCD's L2
+// serves a G-th of the vector. `follow` = 1: the step's source point and early
+// exit come from ctrl.
+template <int G, bool LONG>
+__global__ __launch_bounds__(kSlabThreads) void slab_sum_kernel(int slabs, const int* __restrict__ rb,
+                                                                const int* __restrict__ tc,
+                                                                const uint4* __restrict__ row,
+                                                                const uint4* __restrict__ col,
+                                                                const float* __restrict__ val, const float* x,
+                                                                const float* w, const Control* ctrl, int parity,
+                                                                int follow, float* __restrict__ part,
+                                                                int64_t stride) {
+    constexpr int W = slab_width(G);
+    __shared__ __attribute__((aligned(16))) float ws0[W];
+    __shared__ __attribute__((aligned(16))) float ws1[W];
+    __shared__ __attribute__((aligned(16))) float ws2[W];
+    __shared__ float acc[slab_rows_max(G)];
+    int use_x = 0;
+    if (follow) {
+        if (ctrl->done) return;  // grid-uniform
+        use_x = ctrl->use_x[parity];
+    }
+    const float* gat = use_x ? x : w;
+    const int b = blockIdx.x / G, h = blockIdx.x % G;
+    const int s0 = slab_range_start(h, slabs, G), P = slab_range_start(h + 1, slabs, G) - s0;
+    const int r0 = rb[b], rows = rb[b + 1] - r0;
+    const int last = tc[size_t(gridDim.x / G) * slabs] - 1;
+    slab_rows<W, LONG>(P, rows, last, GlobalBounds{tc, size_t(b) * slabs + s0}, row, col, val,
+                       gat + size_t(s0) * W, ws0, ws1, ws2, acc);
+    float* out = part + h * stride + r0;
+    for (int r = threadIdx.x; r < rows; r += kSlabThreads) out[r] = acc[r];
+}
+
 constexpr int kFinal = 1024;  // threads of a finalize launch
 
-__global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const double* partials, int groups,
-                                                           int parity, double switch_stat) {
-    __shared__ double wave_dot[kFinal / kWave];
-    __shared__ double wave_res[kFinal / kWave];
-    if (ctrl->done) return;  // the step returned early
+// Restart decision of a step: the sum of the step's `groups` workgroup
+// partials (dot products at [0, groups), residual maxima at [groups, 2 groups))
+// in a fixed order, then the t / extrapolation state of the next step. Run by
+// one workgroup of NT threads.
+template <int NT>
+__device__ void restart_decision(Control* ctrl, const double* partials, int groups, int parity, double switch_stat) {
+    __shared__ double wave_dot[NT / kWave];
+    __shared__ double wave_res[NT / kWave];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // thread t owns partials t, t + kFinal, ...; 8 loads in flight per batch
+    // thread t owns partials t, t + NT, ...; 8 loads in flight per batch
     double s = 0, res = 0;
-    for (int g0 = threadIdx.x; g0 < groups; g0 += 8 * kFinal) {
+    for (int g0 = threadIdx.x; g0 < groups; g0 += 8 * NT) {
         double v[8], r[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int g = g0 + u * kFinal;
+            const int g = g0 + u * NT;
             v[u] = g < groups ? partials[g] : 0.0;
             r[u] = g < groups ? partials[groups + g] : 0.0;
         }
@@ -246,7 +299,7 @@ __global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const 
     __syncthreads();
     if (threadIdx.x == 0) {
         double tot = 0, rmax = 0;
-        for (int w = 0; w < kFinal / kWave; ++w) {
+        for (int w = 0; w < NT / kWave; ++w) {
             tot += wave_dot[w];
             rmax = fmax(rmax, wave_res[w]);
         }
@@ -271,6 +324,57 @@ __global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const 
             }
         }
     }
+}
+
+// Split layout, second launch of a step: each row's sum is its ranges' partials
+// added in range order, then the step epilogue; thread t of workgroup j takes
+// rows 4 kSlabThreads j + t + k kSlabThreads, k < 4, with all their operands
+// loaded before the first is applied. The restart partials are one per
+// workgroup of this launch.
+template <int MOM>
+__global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, int groups, const float* __restrict__ part,
+                                                                   int64_t stride, StepBuffers<float> bufs,
+                                                                   Control* ctrl, double* partials, int parity,
+                                                                   double beta_imm) {
+#pragma clang fp contract(off)
+    constexpr int K = 4;
+    __shared__ double red[2 * kSlabWaves];
+    double beta = beta_imm, next_t = 0;
+    int use_x = 0;
+    if (MOM != kMomentumImmediate) {
+        if (ctrl->done) return;  // grid-uniform
+        momentum_of(ctrl->t[parity], &next_t, &beta);
+        use_x = ctrl->use_x[parity];
+    }
+    StepEpilogue<float> epi = slab_epilogue(bufs, parity, use_x, beta);
+    const float* __restrict__ dstep = bufs.dstep;
+    for (int r0 = blockIdx.x * K * kSlabThreads + threadIdx.x; r0 < n; r0 += gridDim.x * K * kSlabThreads) {
+        float g[K], wv[K], xv[K], den[K], cr[K], lr[K], ur[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int r = min(r0 + k * kSlabThreads, n - 1);  // rows past n repeat row n - 1, unused
+            g[k] = part[r];
+            for (int h = 1; h < groups; ++h) g[k] = g[k] + part[h * stride + r];
+            wv[k] = epi.gat[r];
+            xv[k] = epi.xin[r];
+            den[k] = dstep ? dstep[r] : epi.lipschitz;
+            cr[k] = epi.c[r];
+            lr[k] = epi.lo[r];
+            ur[k] = epi.up[r];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (r0 + k * kSlabThreads < n) epi.apply(r0 + k * kSlabThreads, g[k], wv[k], xv[k], den[k], cr[k], lr[k], ur[k]);
+    }
+    slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, red);
+}
+
+// Restart decision of a step launch (restart_decision), as a launch of its own
+// after the step launch on the same stream.
+__global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const double* partials, int groups,
+                                                           int parity, double switch_stat) {
+    if (ctrl->done) return;  // the step returned early
+    restart_decision<kFinal>(ctrl, partials, groups, parity, switch_stat);
 }
 
 template <bool ORDERED>
@@ -583,7 +687,35 @@ template void launch_step<float>(const PanelMatrix<float>&, const StepBuffers<fl
 
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
                       int momentum, double beta, hipStream_t stream, double switch_stat) {
-    dim3 grid(m.blocks), block(kSlabThreads);
+    dim3 block(kSlabThreads);
+    if (m.groups > 1) {
+        const int follow = momentum != kMomentumImmediate;
+        const dim3 sum(m.groups * m.blocks);
+#define HGQP_SLAB_SUM(G, LONG)                                                                                   \
+    hipLaunchKernelGGL((slab_sum_kernel<G, LONG>), sum, block, 0, stream, m.slabs, m.rb, m.tc, m.row, m.col, m.val, \
+                       b.x[parity], b.w[parity], ctrl, parity, follow, m.part, m.part_stride)
+        if (m.groups == 4) {
+            if (m.long_runs) HGQP_SLAB_SUM(4, true);
+            else HGQP_SLAB_SUM(4, false);
+        } else {
+            if (m.long_runs) HGQP_SLAB_SUM(2, true);
+            else HGQP_SLAB_SUM(2, false);
+        }
+#undef HGQP_SLAB_SUM
+        const dim3 fin(slab_finish_groups(m.n));
+#define HGQP_SLAB_FINISH(MOM)                                                                                      \
+    hipLaunchKernelGGL((slab_finish_kernel<MOM>), fin, block, 0, stream, m.n, m.groups, m.part, m.part_stride, b, \
+                       ctrl, partials, parity, beta)
+        if (momentum == kMomentumImmediate) HGQP_SLAB_FINISH(kMomentumImmediate);
+        else if (momentum == kMomentumReference) HGQP_SLAB_FINISH(kMomentumReference);
+        else HGQP_SLAB_FINISH(kMomentumRestart);
+#undef HGQP_SLAB_FINISH
+        if (momentum == kMomentumRestart)
+            hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(fin.x), parity,
+                               switch_stat);
+        return;
+    }
+    dim3 grid(m.blocks);
 #define HGQP_SLAB_STEP(MOM, LONG)                                                                              \
     hipLaunchKernelGGL((slab_step_kernel<MOM, LONG>), grid, block, 0, stream, m.blocks, m.slabs, m.rb, m.tc, m.row, \
                        m.col, m.val, b, ctrl, partials, parity, beta)

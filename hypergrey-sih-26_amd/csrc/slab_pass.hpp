@@ -4,11 +4,12 @@
 //
 // Pipeline of one row block (workgroup) over the slabs: LDS holds a ring of
 // three slab buffers and the block's row sums. While the threads sum tile s
-// from buffer s % 3 (with chunk registers loaded two slabs earlier), the copies
-// of slab s + 1 and the chunk loads of tile s + 1 are in flight and those of
-// s + 2 are issued. The barrier before tile s waits for this wave's LDS work
-// and for every vector-memory operation but the newest slab's group, so two
-// slabs of copy latency hide behind the sums.
+// from buffer s % 3 (with chunk registers loaded three tiles earlier), the
+// copies of slabs s + 1 and s + 2 and the chunk loads of tiles s + 1 and s + 2
+// are in flight; tile s + 3's chunk loads follow the sum. The barrier before
+// tile s waits for this wave's LDS work and for every vector-memory operation
+// but the newest two chunk groups and one slab's copies, so two tiles of
+// latency hide behind the sums.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,8 +23,12 @@ typedef __attribute__((address_space(3))) void* LdsPtr;
 typedef __attribute__((address_space(1))) void* GlobalPtr;
 
 constexpr int kSlabWaves = kSlabThreads / kWave;
-constexpr int kSlabCopies = kSlabWidth * 4 / 1024 / kSlabWaves;  // 1 KiB copies per wave per slab
-static_assert(kSlabWidth * 4 % (1024 * kSlabWaves) == 0, "a slab splits into whole 1 KiB copies per wave");
+// 1 KiB copies per wave of a slab W columns wide
+template <int W>
+constexpr int slab_copies() {
+    static_assert(W * 4 % (1024 * kSlabWaves) == 0, "a slab splits into whole 1 KiB copies per wave");
+    return W * 4 / 1024 / kSlabWaves;
+}
 constexpr int kChunkLoads = 4;  // vector loads per chunk: rows, columns, two of values
 static_assert(kSlabChunk == 8, "a chunk is one uint4 of rows, one of columns and two of FP32 values");
 
@@ -41,14 +46,16 @@ __device__ __forceinline__ void barrier_vm() {
     __asm__ volatile("" ::: "memory");
 }
 
-// Copies kSlabWidth floats from src into the LDS buffer dst: wave v copies its
-// kSlabCopies KiB, one 1 KiB global_load_lds_dwordx4 per step, lane l's 16
+// Copies W floats from src into the LDS buffer dst: wave v copies its
+// slab_copies<W>() KiB, one 1 KiB global_load_lds_dwordx4 per step, lane l's 16
 // bytes landing at +16 l of that KiB (M0 holds the wave's LDS address).
+template <int W>
 __device__ __forceinline__ void stage_slab(const float* src, float* dst) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    constexpr int kCopies = slab_copies<W>();
 #pragma unroll
-    for (int i = 0; i < kSlabCopies; ++i) {
-        const int off = (wave * kSlabCopies + i) * 1024;
+    for (int i = 0; i < kCopies; ++i) {
+        const int off = (wave * kCopies + i) * 1024;
         __builtin_amdgcn_global_load_lds((GlobalPtr)(reinterpret_cast<const char*>(src) + off + lane * 16),
                                          (LdsPtr)(reinterpret_cast<char*>(dst) + off), 16, 0, 0);
     }
@@ -195,23 +202,29 @@ struct LdsBounds {
     __device__ int operator()(int s) const { return tcs[s]; }
 };
 
-// Tile s: waits for slab s and tile s's chunk registers (`cur`), issues tile
-// s + 2's chunk loads into `ahead` and its slab copies into `ahead_buf`, then
-// sums tile s from `cur_buf`. Indices past the last tile are clamped, so every
-// load is unconditional (the repeated copies land in a buffer no later tile
-// reads).
-template <bool LONG, class Bounds>
-__device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* __restrict__ row, const uint4* __restrict__ col,
-                                 const float* __restrict__ val, int s, int last, const float* __restrict__ gat,
-                                 const float* cur_buf, float* ahead_buf, float* acc, const SlabChunk& cur,
-                                 SlabChunk& ahead) {
+// Tile s: waits for slab s and tile s's chunk registers `q`, issues slab
+// s + 2's copies into `ahead_buf`, sums tile s from `cur_buf`, then issues
+// tile s + 3's chunk loads into `q`. Per tile the vector-memory operations go
+// out in that order (copies, then chunk loads), so the barrier may leave the
+// newest 2 kChunkLoads + slab_copies<W>() in flight: tile s + 1's and s + 2's
+// chunk loads and slab s + 1's copies, while everything older (slab s, tile
+// s's chunks) has landed. The empty asm statements keep the compiler from
+// moving the chunk loads across the copies or the sums. Indices past the last
+// tile are clamped, so every load is unconditional (the repeated copies land
+// in a buffer no later tile reads). The tiles after the three-tile loop wait
+// for everything instead (ALL): loads into registers no later tile reads may
+// be dropped by the compiler, which would shift the count.
+template <int W, bool LONG, bool ALL, class Bounds>
+__device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* __restrict__ row,
+                                          const uint4* __restrict__ col, const float* __restrict__ val, int s,
+                                          int last, const float* __restrict__ gat, const float* cur_buf,
+                                          float* ahead_buf, float* acc, SlabChunk& q) {
     const int tid = threadIdx.x;
-    barrier_vm<kChunkLoads + kSlabCopies>();  // tile s's group landed; tile s + 1's may still be in flight
-    const int sa = min(s + 2, P - 1);
-    load_chunk(ahead, row, col, val, min(tb(sa) + tid, last));
-    stage_slab(gat + size_t(sa) * kSlabWidth, ahead_buf);
+    barrier_vm<ALL ? 0 : 2 * kChunkLoads + slab_copies<W>()>();
+    stage_slab<W>(gat + size_t(min(s + 2, P - 1)) * W, ahead_buf);
+    __asm__ volatile("" ::: "memory");
     const int k0 = tb(s), k1 = tb(s + 1);
-    if (k0 + tid < k1) sum_chunk<LONG>(cur, cur_buf, acc, row, col, val, k0 + tid, last);
+    if (k0 + tid < k1) sum_chunk<LONG>(q, cur_buf, acc, row, col, val, k0 + tid, last);
     // tiles with more chunks than threads: each further chunk's loads are issued
     // before the previous one is summed
     int ch = k0 + tid + kSlabThreads;
@@ -227,34 +240,43 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
             ch = nch;
         }
     }
+    __asm__ volatile("" ::: "memory");
+    load_chunk(q, row, col, val, min(tb(min(s + 3, P - 1)) + tid, last));
+    __asm__ volatile("" ::: "memory");
 }
 
-// Row sums of one row block over gat (the block's instance vector, holding
-// whole slabs): P slabs, R rows, `last` the matrix's last chunk, tb the block's
-// tile bounds; LONG when some run of the matrix is longer than a chunk. On
-// return acc[r] holds row r's sum for r < R, every copy has landed and all
-// threads have passed a barrier.
-template <bool LONG, class Bounds>
+// Row sums of one row block over gat (the vector from the block's first slab
+// on, holding whole slabs W columns wide): P slabs, R rows, `last` the
+// matrix's last chunk, tb the block's tile bounds; LONG when some run of the
+// matrix is longer than a chunk. On return acc[r] holds row r's sum for r < R,
+// every copy and load has landed and all threads have passed a barrier.
+template <int W, bool LONG, class Bounds>
 __device__ __forceinline__ void slab_rows(int P, int R, int last, const Bounds& tb, const uint4* __restrict__ row,
-                                 const uint4* __restrict__ col, const float* __restrict__ val,
-                                 const float* __restrict__ gat, float* ws0, float* ws1, float* ws2, float* acc) {
+                                          const uint4* __restrict__ col, const float* __restrict__ val,
+                                          const float* __restrict__ gat, float* ws0, float* ws1, float* ws2,
+                                          float* acc) {
     const int tid = threadIdx.x;
     for (int r = tid; r < R; r += kSlabThreads) acc[r] = 0.0f;
     SlabChunk q0, q1, q2;
-    const int s1 = min(1, P - 1);
+    const int s1 = min(1, P - 1), s2 = min(2, P - 1);
+    // the order of a tile's operations: copies of slab 0, chunks of tile 0, ...
     load_chunk(q0, row, col, val, min(tb(0) + tid, last));
-    stage_slab(gat, ws0);
+    stage_slab<W>(gat, ws0);
+    __asm__ volatile("" ::: "memory");
     load_chunk(q1, row, col, val, min(tb(s1) + tid, last));
-    stage_slab(gat + size_t(s1) * kSlabWidth, ws1);
+    __asm__ volatile("" ::: "memory");
+    stage_slab<W>(gat + size_t(s1) * W, ws1);
+    __asm__ volatile("" ::: "memory");
+    load_chunk(q2, row, col, val, min(tb(s2) + tid, last));
     // three tiles per trip with fixed buffer roles; the one or two left follow
     int s = 0;
     for (; s + 3 <= P; s += 3) {
-        slab_tile<LONG>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0, q2);
-        slab_tile<LONG>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1, q0);
-        slab_tile<LONG>(P, tb, row, col, val, s + 2, last, gat, ws2, ws1, acc, q2, q1);
+        slab_tile<W, LONG, false>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
+        slab_tile<W, LONG, false>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
+        slab_tile<W, LONG, false>(P, tb, row, col, val, s + 2, last, gat, ws2, ws1, acc, q2);
     }
-    if (s < P) slab_tile<LONG>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0, q2);
-    if (s + 1 < P) slab_tile<LONG>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1, q0);
+    if (s < P) slab_tile<W, LONG, true>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
+    if (s + 1 < P) slab_tile<W, LONG, true>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
     barrier_vm<0>();
 }
 

@@ -37,8 +37,8 @@ __device__ inline int64_t thread_count() { return int64_t(gridDim.x) * kThreads;
 
 // Runs of row i; flags a descending column (bit 0) and a run longer than a
 // chunk (bit 1).
-__global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* start, const int* col, int* runs,
-                                                       int* flags) {
+__global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* start, const int* col, int width,
+                                                       int* runs, int* flags) {
     for (int64_t i = thread_id(); i < n; i += thread_count()) {
         const int a = start[i], e = start[i + 1];
         int count = 0, len = 0, prev_slab = -1, prev_col = -1, f = 0;
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* sta
             const int c = col[k];
             if (c < prev_col) f |= 1;
             prev_col = c;
-            const int s = c / kSlabWidth;
+            const int s = c / width;
             if (s != prev_slab) {
                 ++count;
                 len = 0;
@@ -71,10 +71,10 @@ __device__ inline int block_of(const int* rb, int blocks, int64_t i) {
 }
 
 // Run records in row order: key = b slabs + s, value = (first entry << 32) |
-// (row within the block << 20) | length (a run is at most kSlabWidth long).
+// (row within the block << 16) | length (a run is at most one slab, < 2^16).
 __global__ __launch_bounds__(kThreads) void write_runs(int64_t n, const int* start, const int* col, const int* rb,
-                                                       int blocks, int slabs, const int* offset, unsigned* key,
-                                                       uint64_t* value) {
+                                                       int blocks, int slabs, int width, const int* offset,
+                                                       unsigned* key, uint64_t* value) {
     for (int64_t i = thread_id(); i < n; i += thread_count()) {
         const int a = start[i], e = start[i + 1];
         if (a == e) continue;
@@ -83,11 +83,11 @@ __global__ __launch_bounds__(kThreads) void write_runs(int64_t n, const int* sta
         int o = offset[i];
         int k = a;
         while (k < e) {
-            const int s = col[k] / kSlabWidth;
+            const int s = col[k] / width;
             int q = k + 1;
-            while (q < e && col[q] / kSlabWidth == s) ++q;
+            while (q < e && col[q] / width == s) ++q;
             key[o] = unsigned(b * slabs + s);
-            value[o] = (uint64_t(uint32_t(k)) << 32) | (local << 20) | uint64_t(q - k);
+            value[o] = (uint64_t(uint32_t(k)) << 32) | (local << 16) | uint64_t(q - k);
             ++o;
             k = q;
         }
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const u
         for (int jb = j0; jb < j1; jb += kBatch) {
             int lens[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) lens[u] = jb + u < j1 ? int(value[jb + u] & 0xFFFFF) : 0;
+            for (int u = 0; u < kBatch; ++u) lens[u] = jb + u < j1 ? int(value[jb + u] & 0xFFFF) : 0;
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 if (jb + u >= j1) break;
@@ -148,16 +148,16 @@ __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const u
 // Entries of a long run past its first chunk carry kSlabCont; the last entry of
 // a chunk whose run goes on into the next chunk carries kSlabFollow.
 __global__ __launch_bounds__(kThreads) void pack_runs(const unsigned* key, const uint64_t* value, const int* slot,
-                                                      const int* tc, int records, int slabs, const int* col,
-                                                      const double* val, uint16_t* out_row, uint16_t* out_col,
-                                                      float* out_val) {
+                                                      const int* tc, int records, int slabs, int width,
+                                                      const int* col, const double* val, uint16_t* out_row,
+                                                      uint16_t* out_col, float* out_val) {
     for (int64_t j = thread_id(); j < records; j += thread_count()) {
         const int t = int(key[j]);
         const uint64_t v = value[j];
-        const int len = int(v & 0xFFFFF);
-        const int local = int((v >> 20) & 0xFFF);
+        const int len = int(v & 0xFFFF);
+        const int local = int((v >> 16) & 0xFFFF);
         const int k = int(v >> 32);
-        const int base = (t % slabs) * kSlabWidth;
+        const int base = (t % slabs) * width;
         const int64_t at = int64_t(tc[t]) * kSlabChunk + slot[j];
         for (int q = 0; q < len; ++q) {
             const bool cont = q >= kSlabChunk, follow = q + 1 < len && (slot[j] + q) % kSlabChunk == kSlabChunk - 1;
@@ -190,14 +190,17 @@ std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target, int ma
     return rb;
 }
 
-bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const int* start, const int* col,
-                 const double* val, SlabStore& out, hipStream_t stream) {
+bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int groups, const int* start,
+                 const int* col, const double* val, SlabStore& out, hipStream_t stream) {
+    if (groups != 1 && groups != 2 && groups != 4) return false;
+    const int width = slab_width(groups);
+    const int max_rows = slab_rows_max(groups);
     const int nb = int(blocks.size()) - 1;
-    const int64_t P = (n + kSlabWidth - 1) / kSlabWidth;
+    const int64_t P = (n + width - 1) / width;
     const int64_t tiles64 = int64_t(nb) * P;
-    if (nb < 1 || nnz < 1 || tiles64 + 1 >= (int64_t(1) << 31)) return false;
+    if (nb < 1 || nnz < 1 || tiles64 + 1 >= (int64_t(1) << 31) || P < groups) return false;
     for (int b = 0; b < nb; ++b)
-        if (blocks[b + 1] - blocks[b] > kSlabRows) return false;
+        if (blocks[b + 1] - blocks[b] > max_rows) return false;
     const int tiles = int(tiles64);
     const size_t un = size_t(n);
 
@@ -205,7 +208,8 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     HIPCHECK(hipMemcpyAsync(rb.p, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemsetAsync(flag.p, 0, 4, stream));
     HIPCHECK(hipMemsetAsync(runs.p + n, 0, 4, stream));
-    hipLaunchKernelGGL(count_runs, dim3(grid_for(n)), dim3(kThreads), 0, stream, n, start, col, runs.p, flag.p);
+    hipLaunchKernelGGL(count_runs, dim3(grid_for(n)), dim3(kThreads), 0, stream, n, start, col, width, runs.p,
+                       flag.p);
     size_t tmp_bytes = 0;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, runs.p, offset.p, int(n + 1), stream));
     {
@@ -225,7 +229,7 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     DeviceTemp<int> slot{size_t(records)};
     DeviceTemp<uint64_t> value{size_t(records)}, value2{size_t(records)};
     hipLaunchKernelGGL(write_runs, dim3(grid_for(n)), dim3(kThreads), 0, stream, n, start, col, rb.p, nb, int(P),
-                       offset.p, key.p, value.p);
+                       width, offset.p, key.p, value.p);
     const int end_bit = bits_for(tiles);
     tmp_bytes = 0;
     HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key2.p, value.p, value2.p, records, 0,
@@ -251,12 +255,15 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     HIPCHECK(hipStreamSynchronize(stream));
     if (total < 1) return false;
 
-    // one allocation: block starts, tile chunk starts, rows, columns, values
+    // one allocation: block starts, tile chunk starts, rows, columns, values and,
+    // split, the two halves' partial sums
     auto span = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
     const size_t entries = size_t(total) * kSlabChunk;
     const size_t b_rb = span((size_t(nb) + 1) * 4), b_tc = span((size_t(tiles) + 1) * 4), b_idx = span(entries * 2),
                  b_val = span(entries * 4);
-    out.bytes = b_rb + b_tc + 2 * b_idx + b_val;
+    const int64_t stride = int64_t(span(un * 4) / 4);
+    const size_t b_part = groups > 1 ? size_t(groups) * size_t(stride) * 4 : 0;
+    out.bytes = b_rb + b_tc + 2 * b_idx + b_val + b_part;
     HIPCHECK(hipMalloc(reinterpret_cast<void**>(&out.base), out.bytes));
     char* p = out.base;
     int* d_rb = reinterpret_cast<int*>(p);
@@ -270,13 +277,18 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const i
     HIPCHECK(hipMemsetAsync(d_col, 0, entries * 2, stream));
     HIPCHECK(hipMemsetAsync(d_val, 0, entries * 4, stream));
     hipLaunchKernelGGL(pack_runs, dim3(grid_for(records)), dim3(kThreads), 0, stream, key2.p, value2.p, slot.p, tc.p,
-                       records, int(P), col, val, d_row, d_col, d_val);
+                       records, int(P), width, col, val, d_row, d_col, d_val);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(stream));  // the temporaries are released on return
     out.chunks = total;
     out.m.blocks = nb;
     out.m.slabs = int(P);
     out.m.long_runs = long_runs;
+    out.m.groups = groups;
+    out.m.width = width;
+    out.m.n = int(n);
+    out.m.part_stride = groups > 1 ? stride : 0;
+    out.m.part = groups > 1 ? reinterpret_cast<float*>(p + b_rb + b_tc + 2 * b_idx + b_val) : nullptr;
     out.m.rb = d_rb;
     out.m.tc = d_tc;
     out.m.row = reinterpret_cast<const uint4*>(d_row);

@@ -20,9 +20,20 @@
 // thread of a run's first chunk adds all of the run's products to the row's
 // sum one by one (following a long run into the next chunks), so every row
 // sums its entries in stored column order, exactly as the ORDERED CSR pass.
+//
+// Split layouts (groups G = 2 or 4): with one workgroup per row block every CU
+// reads all of the vector once per step, so the vector's L2-to-LDS traffic is
+// n^2 4 / (rows per block) bytes. Row blocks G times as tall (slab_rows_max(G)
+// rows, narrower slab_width(G) slabs to leave LDS for the sums) cut it by G: G
+// workgroups share a row block, workgroup h over the h-th of G consecutive
+// ranges of the slabs, each writing its rows' partial sums; a finishing launch
+// adds the G partials in range order and applies the step epilogue. A row's
+// sum is then a sum of per-range ordered sums, not the ORDERED sum, so the
+// split serves handles whose reduction order is free.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -33,16 +44,29 @@ namespace hgqp {
 constexpr int kSlabWidth = 12288;  // columns per slab: 48 KiB of the FP32 vector
 constexpr int kSlabThreads = 512;  // threads of a slab step launch (8 waves)
 constexpr int kSlabRows = 4096;    // rows per row block: its FP32 sums take 16 KiB of LDS
+// Columns per slab and rows per row block with G workgroups per block: three
+// slabs in the LDS ring plus the block's FP32 sums fit 160 KiB.
+constexpr int slab_width(int groups) { return groups >= 4 ? 8192 : groups == 2 ? 10240 : kSlabWidth; }
+constexpr int slab_rows_max(int groups) { return groups >= 4 ? 16384 : groups == 2 ? 8192 : kSlabRows; }
+// First slab of range h when `slabs` slabs are split over G workgroups.
+__host__ __device__ inline int slab_range_start(int h, int slabs, int groups) {
+    return (h * slabs + groups - 1) / groups;
+}
 constexpr int kSlabChunk = 8;      // entries per chunk
 constexpr unsigned kSlabPad = 0xFFFF;     // row value of a padding entry
 constexpr unsigned kSlabCont = 0x8000;    // row mark: an entry of a run whose first chunk is an earlier one
 constexpr unsigned kSlabFollow = 0x4000;  // row mark: the run goes on into the next chunk
-constexpr unsigned kSlabRowMask = 0x0FFF; // row within the block
+constexpr unsigned kSlabRowMask = 0x3FFF; // row within the block
 
 struct SlabMatrix {
-    int blocks = 0;                 // row blocks (workgroups of a launch); 0: no slabs
+    int blocks = 0;                 // row blocks; 0: no slabs
     int slabs = 0;                  // column slabs
     int long_runs = 0;              // 1: some run is longer than a chunk (marked entries present)
+    int groups = 1;                 // workgroups per row block: 1, or 2 / 4 for a split layout
+    int width = kSlabWidth;         // columns per slab: slab_width(groups)
+    int n = 0;                      // rows of the matrix
+    int64_t part_stride = 0;        // split: elements between two ranges' partial sums
+    float* part = nullptr;          // split: [groups][part_stride] partial row sums
     const int* rb = nullptr;        // [blocks + 1] first row of each block
     const int* tc = nullptr;        // [blocks * slabs + 1] first chunk of tile (b, s) at b * slabs + s
     const uint4* row = nullptr;     // [chunks] 8 x uint16: row within the block and marks, kSlabPad for padding
@@ -50,12 +74,26 @@ struct SlabMatrix {
     const float* val = nullptr;     // [chunks * 8] value (0 for padding)
 };
 
-// Elements a gathered vector must hold so that every slab copy reads inside it.
-inline int64_t slab_padded(int64_t n) { return (n + kSlabWidth - 1) / kSlabWidth * kSlabWidth; }
+// Elements a gathered vector must hold so that every copy of a slab `width`
+// wide reads inside it.
+inline int64_t slab_padded(int64_t n, int width = kSlabWidth) { return (n + width - 1) / width * width; }
+
+// Workgroups of the launch that finishes a split step (four rows per thread up
+// to this cap, then grid-stride).
+constexpr int kSlabFinishMax = 2048;
+inline int slab_finish_groups(int64_t n) {
+    const int64_t rows = 4 * int64_t(kSlabThreads);
+    return int(std::min<int64_t>(std::max<int64_t>((n + rows - 1) / rows, 1), kSlabFinishMax));
+}
+// Restart partials a step over m writes (one pair per workgroup that applies
+// the epilogue).
+inline int slab_step_groups(const SlabMatrix& m) {
+    return m.groups > 1 ? slab_finish_groups(m.n) : m.blocks;
+}
 
 // Row blocks from the host row starts: block j ends at about (j + 1) nnz / target
-// entries, with at most max_rows (<= kSlabRows) rows. Returns the first row of
-// every block followed by n.
+// entries, with at most max_rows rows. Returns the first row of every block
+// followed by n.
 std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target, int max_rows = kSlabRows);
 
 // Device storage of a slab matrix, owned by a handle.
@@ -73,16 +111,20 @@ struct SlabStore {
 };
 
 // Builds the slab layout of the device CSR (start, col, val) over the row blocks
-// `blocks` (from slab_blocks). Returns false and allocates nothing when a row's
-// columns are not ascending. Synchronizes `stream`.
-bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, const int* start, const int* col,
-                 const double* val, SlabStore& out, hipStream_t stream);
+// `blocks` (from slab_blocks, at most slab_rows_max(groups) rows each) with
+// `groups` workgroups per block (1, 2 or 4; split layouts also get their partial
+// sums), slabs slab_width(groups) wide. Returns false and allocates nothing when
+// a row's columns are not ascending or a split layout has fewer slabs than
+// groups. Synchronizes `stream`.
+bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int groups, const int* start,
+                 const int* col, const double* val, SlabStore& out, hipStream_t stream);
 
 // One FP32 step over slabs: the step_kernel contract of launch_step<float>
-// (same buffers, momentum sources and restart partials, one partial per row
-// block), with every row summed in stored column order whatever the handle's
-// reduction order. The gathered vectors x[parity] and w[parity] must hold
-// slab_padded(n) elements.
+// (same buffers, momentum sources and restart partials, slab_step_groups(m) of
+// them). With one group every row sums in stored column order whatever the
+// handle's reduction order; split, as its ranges' ordered sums added in range
+// order. The gathered vectors x[parity] and w[parity] must hold
+// slab_padded(n, m.width) elements.
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
                       int momentum, double beta, hipStream_t stream, double switch_stat);
 

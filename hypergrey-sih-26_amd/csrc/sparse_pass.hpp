@@ -218,10 +218,12 @@ struct StepEpilogue {
     double dot;    // restart test: sum (w - next)(next - x)
     double resid;  // max D |w - next| / (1 + |next|): the step's own gradient-mapping residual
     __device__ inline void operator()(int r, T g) {
+        apply(r, g, gat[r], xin[r], dstep ? dstep[r] : lipschitz, c[r], lo[r], up[r]);
+    }
+    // the epilogue of row r from its operands, loaded by the caller
+    __device__ inline void apply(int r, T g, T wv, T xv, T den, T cr, T lr, T ur) {
 #pragma clang fp contract(off)
-        const T wv = gat[r], xv = xin[r];
-        const T den = dstep ? dstep[r] : lipschitz;
-        const T next = clip(wv - (g + c[r]) / den, lo[r], up[r]);
+        const T next = clip(wv - (g + cr) / den, lr, ur);
         wout[r] = next + beta * (next - xv);
         xout[r] = next;
         dot += double(wv - next) * double(next - xv);

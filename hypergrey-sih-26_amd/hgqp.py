@@ -39,7 +39,7 @@ class BoxOptions(C.Structure):
     _fields_ = [("precision", C.c_int32), ("momentum", C.c_int32), ("reduce", C.c_int32),
                 ("use_graph", C.c_int32), ("timing", C.c_int32), ("maximize", C.c_int32),
                 ("offset", C.c_double), ("diagonal_hessian", C.c_int32), ("step_metric", C.c_int32),
-                ("column_panels", C.c_int32), ("slabs", C.c_int32)]
+                ("column_panels", C.c_int32), ("slabs", C.c_int32), ("slab_groups", C.c_int32)]
 
 
 class CheckReport(C.Structure):
@@ -76,7 +76,7 @@ class BoxReport(C.Structure):
 class BoxInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("row_blocks", C.c_int64), ("device_bytes", C.c_int64),
                 ("lipschitz", C.c_double), ("column_panels", C.c_int32), ("column_panels32", C.c_int32),
-                ("slabs32", C.c_int32), ("reserved", C.c_int32)]
+                ("slabs32", C.c_int32), ("slab_groups", C.c_int32)]
 
 
 class Problem(C.Structure):

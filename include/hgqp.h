@@ -102,7 +102,19 @@ typedef struct hgqp_box_options {
                                 fits (ascending columns in every row). Slab
                                 steps sum each row in stored column order, so
                                 their FP32 iterates equal those of
-                                HGQP_REDUCE_ORDERED.                           */
+                                HGQP_REDUCE_ORDERED (with one slab group).     */
+    int32_t slab_groups;     /* workgroups per slab row block: 0 (default) 2
+                                under HGQP_REDUCE_TREE when the matrix has at
+                                least 4 slabs of 10240 columns, else 1; 1 one
+                                workgroup per block over all slabs (rows summed
+                                in stored column order); 2 or 4: that many
+                                workgroups per block of that many times the
+                                rows, each over a consecutive range of the
+                                slabs, so every CU stages that fraction of the
+                                vector; a row's sum is then its ranges' ordered
+                                sums added in range order. 2 or 4 with
+                                HGQP_REDUCE_ORDERED is an error; a matrix with
+                                fewer slabs than groups keeps the CSR passes. */
 } hgqp_box_options;
 
 void hgqp_box_options_default(hgqp_box_options* options);
@@ -202,7 +214,7 @@ typedef struct hgqp_box_info {
     int32_t column_panels;
     int32_t column_panels32;
     int32_t slabs32;
-    int32_t reserved;
+    int32_t slab_groups;     /* workgroups per slab row block (0: no slabs)    */
 } hgqp_box_info;
 int hgqp_box_info_get(const hgqp_box* box, hgqp_box_info* info);
 

@@ -87,7 +87,8 @@ def test_invalid_input_rejected_before_device(hg):
     with pytest.raises(hg.InvalidArgument):
         hg.Box(*bad, 5.0)
     for opt in ({"column_panels": 17}, {"column_panels": -1}, {"step_metric": 7}, {"precision": 9}, {"slabs": 3},
-                {"slabs": -1}):
+                {"slabs": -1}, {"slab_groups": 3}, {"slab_groups": -1},
+                {"slab_groups": 2, "reduce": hg.REDUCE_ORDERED}):
         with pytest.raises(hg.InvalidArgument):
             hg.Box(*args, 5.0, **opt)
     for opt in ({"slabs": 2}, {"slabs": -1}, {"check_every": 3}):

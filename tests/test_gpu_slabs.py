@@ -6,8 +6,13 @@ without them bit for bit: iterations, restarts, checks, every report field and
 the primal. The cases cover a slab count that is not a multiple of the
 three-buffer ring, a ragged last slab, rows split over several slabs, empty
 rows, runs longer than a chunk (followed by the thread of their first chunk)
-and tiles with more chunks than a launch has threads. Every call goes
-through the C ABI of libhgqp.so.
+and tiles with more chunks than a launch has threads.
+
+The split layout (two workgroups per row block, each over half of the slabs)
+sums each row as its first half's ordered sum plus its second half's. On a
+matrix where no row has entries in both halves that equals the one-group sum
+bit for bit; elsewhere the split solve is checked against the contract.
+Every call goes through the C ABI of libhgqp.so.
 """
 import numpy as np
 import pytest
@@ -18,7 +23,9 @@ from oracle import BoxQP
 pytestmark = pytest.mark.gpu
 
 KKT_TOL = 1e-6
+OBJ_REL = 1e-8
 SLAB_WIDTH = 12288
+SPLIT_WIDTH = 10240
 
 REPORT_KEYS = ("status", "restarts", "iterations", "fp32_iterations", "checks", "objective", "primal_residual",
                "stationarity_residual", "dual_bound", "relative_gap", "optimality_verified", "finite")
@@ -129,10 +136,11 @@ def test_tiles_wider_than_a_launch(hg, oracle):
 
 
 def test_full_size_slabs(hg):
-    """Planted n = 1e6, 16 draws: the handle picks 82 slabs for its FP32 steps by
-    itself (and keeps 4 / 2 column panels for the FP64 passes and FP32 checks);
-    the headline solve is verified and recovers x*, and with the ORDERED
-    reduction it equals the solve without slabs bit for bit."""
+    """Planted n = 1e6, 16 draws: the handle picks slabs for its FP32 steps by
+    itself (and keeps 4 / 2 column panels for the FP64 passes and FP32 checks):
+    under the TREE reduction the split layout (98 slabs in two groups), whose
+    solve is verified and recovers x*; under ORDERED one group of 82 slabs,
+    whose solve equals the one without slabs bit for bit."""
     n = 1000000
     d = hg.gen_planted(n, 16, 2026)
     s, i, v, c, lo, up = d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"]
@@ -140,7 +148,8 @@ def test_full_size_slabs(hg):
     opt = modes(hg)["refine_restart_jacobi"]
     print("case: n=1e6 headline solve with slabs", flush=True)
     on = solve(hg, data, reduce=hg.REDUCE_TREE, **opt)
-    assert on[0]["slabs32"] == 82 and on[0]["column_panels"] == 4 and on[0]["column_panels32"] == 2
+    assert on[0]["slabs32"] == 98 and on[0]["slab_groups"] == 2
+    assert on[0]["column_panels"] == 4 and on[0]["column_panels32"] == 2
     print(on[1], flush=True)
     assert on[1]["status"] == "optimal" and on[1]["optimality_verified"]
     assert on[1]["stationarity_residual"] <= KKT_TOL
@@ -148,7 +157,7 @@ def test_full_size_slabs(hg):
     print("case: n=1e6 ORDERED with and without slabs", flush=True)
     a = solve(hg, data, reduce=hg.REDUCE_ORDERED, **opt)
     b = solve(hg, data, reduce=hg.REDUCE_ORDERED, slabs=1, **opt)
-    assert a[0]["slabs32"] == 82 and b[0]["slabs32"] == 0
+    assert a[0]["slabs32"] == 82 and a[0]["slab_groups"] == 1 and b[0]["slabs32"] == 0
     assert_same_solve(a, b)
 
 
@@ -193,3 +202,94 @@ def test_slabs_fall_back(hg, oracle):
     info, rep, _ = solve(hg, (ds, np.arange(dn, dtype=np.int32), np.full(dn, 2.0), np.ones(dn), -np.ones(dn),
                               np.ones(dn), 2.0), slabs=2, **fast)
     assert info["slabs32"] == 0 and rep["optimality_verified"]
+
+
+def cut_band_qp(n, half_band, cuts, seed):
+    """Banded diagonally dominant instance (11 entries per row in one slab:
+    runs longer than a chunk) whose couplings across the columns `cuts` are
+    dropped, so no row has entries on both sides of a cut."""
+    rng = np.random.default_rng(seed)
+    hr, hc, hv = [], [], []
+    for k in range(1, half_band + 1):
+        r = np.arange(k, n)
+        keep = np.ones(len(r), dtype=bool)
+        for cut in cuts:
+            keep &= (r < cut) | (r - k >= cut)
+        r = r[keep]
+        hr.append(r); hc.append(r - k); hv.append(rng.normal(0, 0.2, size=len(r)))
+    hr = np.concatenate(hr); hc = np.concatenate(hc); hv = np.concatenate(hv)
+    radius = np.zeros(n)
+    np.add.at(radius, hr, np.abs(hv)); np.add.at(radius, hc, np.abs(hv))
+    d = np.arange(n)
+    return BoxQP(n, np.concatenate([hr, d]), np.concatenate([hc, d]), np.concatenate([hv, radius + 0.5]),
+                 rng.normal(0, 1, size=n), -np.ones(n), np.ones(n))
+
+
+# slab width and first column of each range after the first, per group count,
+# for n = 45000 (ranges start at slab ceil(h slabs / groups))
+CUT_LAYOUTS = {2: (SPLIT_WIDTH, 5, (3 * 10240,)), 4: (8192, 6, (2 * 8192, 3 * 8192, 5 * 8192))}
+
+
+@pytest.mark.parametrize("groups", [2, 4])
+@pytest.mark.parametrize("mode", ["refine_restart_jacobi", "polish_reference"])
+def test_split_equals_one_group_without_straddling_rows(hg, oracle, mode, groups):
+    """n = 45000 split over 2 groups (5 slabs of 10240 columns, the last 4040
+    wide: ranges 0-2 and 3-4) or 4 (6 slabs of 8192: ranges 0-1, 2, 3-4, 5).
+    With no row coupled across a range boundary every row's sum is one range's
+    ordered sum plus +0s, so under the same (TREE) reduction the split solve
+    equals the one-group solve bit for bit."""
+    width, count, cuts = CUT_LAYOUTS[groups]
+    q = cut_band_qp(45000, 5, cuts, 12)
+    data = engine_inputs(oracle, q)
+    opt = dict(modes(hg)[mode], reduce=hg.REDUCE_TREE)
+    print(f"case=cut_band groups={groups} mode={mode} n={q.n} nnz={len(data[2])}", flush=True)
+    split = solve(hg, data, slabs=2, slab_groups=groups, **opt)
+    one = solve(hg, data, slabs=2, slab_groups=1, **opt)
+    assert split[0]["slabs32"] == count and split[0]["slab_groups"] == groups
+    assert one[0]["slabs32"] == 4 and one[0]["slab_groups"] == 1
+    print(split[1], flush=True)
+    assert split[1]["status"] == "optimal" and split[1]["optimality_verified"]
+    assert split[1]["fp32_iterations"] > 0
+    assert_same_solve(split, one)
+
+
+@pytest.mark.parametrize("n,groups", [(200000, 0), (61440 + 777, 0), (200000, 4)])
+def test_split_solve_meets_contract(hg, oracle, n, groups):
+    """Planted instances whose rows reach every range: the split solve (two
+    groups by the automatic choice, or four) is verified on the device and by
+    the oracle, lands on the one-group optimum and x*, and repeats bit for bit
+    (a fixed summation order)."""
+    q, d = planted_qp(hg, n, 8, 21)
+    data = engine_inputs(oracle, q)
+    opt = dict(modes(hg)["refine_restart_jacobi"], reduce=hg.REDUCE_TREE)
+    print(f"case=planted_split n={n} groups={groups}", flush=True)
+    split = solve(hg, data, slabs=2, slab_groups=groups, **opt)
+    again = solve(hg, data, slabs=2, slab_groups=groups, **opt)
+    one = solve(hg, data, slabs=2, slab_groups=1, **opt)
+    width = 8192 if groups == 4 else SPLIT_WIDTH
+    assert split[0]["slab_groups"] == (groups or 2) and split[0]["slabs32"] == -(-n // width)
+    assert one[0]["slab_groups"] == 1
+    print(split[1], flush=True)
+    for r in (split, one):
+        assert r[1]["status"] == "optimal" and r[1]["optimality_verified"]
+        assert np.abs(r[2] - d["optimum"]).max() < 1e-4
+    chk = oracle.check(q, split[2], KKT_TOL)
+    assert chk["optimality_verified"] and chk["stationarity_residual"] <= KKT_TOL
+    assert abs(split[1]["objective"] - one[1]["objective"]) <= OBJ_REL * (1 + abs(one[1]["objective"]))
+    assert_same_solve(split, again)
+
+
+def test_split_choice(hg, oracle):
+    """The automatic choice: two groups under TREE once there are 4 slabs of the
+    split width, one under ORDERED or below that size; two groups asked for on a
+    matrix of one split slab leave the CSR passes in place."""
+    q = planted_qp(hg, 30000, 1, 3)[0]  # 3 split slabs
+    data = engine_inputs(oracle, q)
+    fast = modes(hg)["refine_restart_jacobi"]
+    assert solve(hg, data, slabs=2, reduce=hg.REDUCE_TREE, **fast)[0]["slab_groups"] == 1
+    big = engine_inputs(oracle, planted_qp(hg, 4 * SPLIT_WIDTH, 1, 3)[0])
+    assert solve(hg, big, slabs=2, reduce=hg.REDUCE_TREE, **fast)[0]["slab_groups"] == 2
+    assert solve(hg, big, slabs=2, reduce=hg.REDUCE_ORDERED, **fast)[0]["slab_groups"] == 1
+    small = engine_inputs(oracle, planted_qp(hg, 9000, 4, 3)[0])
+    info, rep, _ = solve(hg, small, slabs=2, slab_groups=2, reduce=hg.REDUCE_TREE, **fast)
+    assert info["slabs32"] == 0 and info["slab_groups"] == 0 and rep["optimality_verified"]
