@@ -327,17 +327,18 @@ __device__ void restart_decision(Control* ctrl, const double* partials, int grou
 }
 
 // Split layout, second launch of a step: each row's sum is its ranges' partials
-// added in range order, then the step epilogue; thread t of workgroup j takes
-// rows 4 kSlabThreads j + t + k kSlabThreads, k < 4, with all their operands
-// loaded before the first is applied. The restart partials are one per
-// workgroup of this launch.
+// added in range order, then the step epilogue. Thread t of workgroup j takes
+// rows 4 (kSlabThreads j + t) to 4 (kSlabThreads j + t) + 3 with 16-byte loads
+// and stores (every vector starts 256-byte aligned; the partial stride is a
+// multiple of 64); workgroup 0 also takes the rows past the last whole group
+// of four, one by one. The restart partials are one per workgroup of this
+// launch.
 template <int MOM>
 __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, int groups, const float* __restrict__ part,
                                                                    int64_t stride, StepBuffers<float> bufs,
                                                                    Control* ctrl, double* partials, int parity,
                                                                    double beta_imm) {
 #pragma clang fp contract(off)
-    constexpr int K = 4;
     __shared__ double red[2 * kSlabWaves];
     double beta = beta_imm, next_t = 0;
     int use_x = 0;
@@ -347,25 +348,37 @@ __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, int gr
         use_x = ctrl->use_x[parity];
     }
     StepEpilogue<float> epi = slab_epilogue(bufs, parity, use_x, beta);
-    const float* __restrict__ dstep = bufs.dstep;
-    for (int r0 = blockIdx.x * K * kSlabThreads + threadIdx.x; r0 < n; r0 += gridDim.x * K * kSlabThreads) {
-        float g[K], wv[K], xv[K], den[K], cr[K], lr[K], ur[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int r = min(r0 + k * kSlabThreads, n - 1);  // rows past n repeat row n - 1, unused
-            g[k] = part[r];
-            for (int h = 1; h < groups; ++h) g[k] = g[k] + part[h * stride + r];
-            wv[k] = epi.gat[r];
-            xv[k] = epi.xin[r];
-            den[k] = dstep ? dstep[r] : epi.lipschitz;
-            cr[k] = epi.c[r];
-            lr[k] = epi.lo[r];
-            ur[k] = epi.up[r];
+    const int quads = n / 4;
+    for (int q = blockIdx.x * kSlabThreads + threadIdx.x; q < quads; q += gridDim.x * kSlabThreads) {
+        float4 g = reinterpret_cast<const float4*>(part)[q];
+        for (int h = 1; h < groups; ++h) {
+            const float4 p = reinterpret_cast<const float4*>(part + h * stride)[q];
+            g.x = g.x + p.x;
+            g.y = g.y + p.y;
+            g.z = g.z + p.z;
+            g.w = g.w + p.w;
         }
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (r0 + k * kSlabThreads < n) epi.apply(r0 + k * kSlabThreads, g[k], wv[k], xv[k], den[k], cr[k], lr[k], ur[k]);
+        const float4 wv = reinterpret_cast<const float4*>(epi.gat)[q];
+        const float4 xv = reinterpret_cast<const float4*>(epi.xin)[q];
+        const float4 den = epi.dstep ? reinterpret_cast<const float4*>(epi.dstep)[q]
+                                     : make_float4(epi.lipschitz, epi.lipschitz, epi.lipschitz, epi.lipschitz);
+        const float4 cr = reinterpret_cast<const float4*>(epi.c)[q];
+        const float4 lr = reinterpret_cast<const float4*>(epi.lo)[q];
+        const float4 ur = reinterpret_cast<const float4*>(epi.up)[q];
+        float4 nx, nw;
+        epi.compute(g.x, wv.x, xv.x, den.x, cr.x, lr.x, ur.x, nx.x, nw.x);
+        epi.compute(g.y, wv.y, xv.y, den.y, cr.y, lr.y, ur.y, nx.y, nw.y);
+        epi.compute(g.z, wv.z, xv.z, den.z, cr.z, lr.z, ur.z, nx.z, nw.z);
+        epi.compute(g.w, wv.w, xv.w, den.w, cr.w, lr.w, ur.w, nx.w, nw.w);
+        reinterpret_cast<float4*>(epi.wout)[q] = nw;
+        reinterpret_cast<float4*>(epi.xout)[q] = nx;
     }
+    if (blockIdx.x == 0)
+        for (int r = 4 * quads + int(threadIdx.x); r < n; r += kSlabThreads) {
+            float g = part[r];
+            for (int h = 1; h < groups; ++h) g = g + part[h * stride + r];
+            epi(r, g);
+        }
     slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, red);
 }
 

@@ -78,8 +78,8 @@ struct SlabMatrix {
 // wide reads inside it.
 inline int64_t slab_padded(int64_t n, int width = kSlabWidth) { return (n + width - 1) / width * width; }
 
-// Workgroups of the launch that finishes a split step (four rows per thread up
-// to this cap, then grid-stride).
+// Workgroups of the launch that finishes a split step (four consecutive rows per
+// thread up to this cap, then grid-stride).
 constexpr int kSlabFinishMax = 2048;
 inline int slab_finish_groups(int64_t n) {
     const int64_t rows = 4 * int64_t(kSlabThreads);

@@ -218,14 +218,18 @@ struct StepEpilogue {
     double dot;    // restart test: sum (w - next)(next - x)
     double resid;  // max D |w - next| / (1 + |next|): the step's own gradient-mapping residual
     __device__ inline void operator()(int r, T g) {
-        apply(r, g, gat[r], xin[r], dstep ? dstep[r] : lipschitz, c[r], lo[r], up[r]);
-    }
-    // the epilogue of row r from its operands, loaded by the caller
-    __device__ inline void apply(int r, T g, T wv, T xv, T den, T cr, T lr, T ur) {
-#pragma clang fp contract(off)
-        const T next = clip(wv - (g + cr) / den, lr, ur);
-        wout[r] = next + beta * (next - xv);
+        T next, wnext;
+        compute(g, gat[r], xin[r], dstep ? dstep[r] : lipschitz, c[r], lo[r], up[r], next, wnext);
+        wout[r] = wnext;
         xout[r] = next;
+    }
+    // one row's step from its operands (gradient sum g, source point wv, iterate
+    // xv, denominator, cost, bounds): the new iterate and extrapolated point,
+    // with the row's restart terms added to dot and resid
+    __device__ inline void compute(T g, T wv, T xv, T den, T cr, T lr, T ur, T& next, T& wnext) {
+#pragma clang fp contract(off)
+        next = clip(wv - (g + cr) / den, lr, ur);
+        wnext = next + beta * (next - xv);
         dot += double(wv - next) * double(next - xv);
         const double rr = double(den) * fabs(double(wv) - double(next)) / (1 + fabs(double(next)));
         resid = isfinite(rr) ? fmax(resid, rr) : INFINITY;
