@@ -747,6 +747,16 @@ void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control*
                            switch_stat);
 }
 
+// Writes a whole control block from a kernel argument: no host buffer and no
+// wait, ordered on the stream with the launches around it.
+__global__ __launch_bounds__(64) void set_control_kernel(Control* ctrl, Control value) {
+    if (threadIdx.x == 0) *ctrl = value;
+}
+
+void launch_set_control(Control* ctrl, const Control& value, hipStream_t stream) {
+    hipLaunchKernelGGL(set_control_kernel, dim3(1), dim3(64), 0, stream, ctrl, value);
+}
+
 void launch_check(const PanelMatrix<double>& pm, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream) {
     // at x = 0 the check reads no matrix entries, so the panel passes are skipped

@@ -112,6 +112,8 @@ template <class T>
 void launch_step(const PanelMatrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
                  int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat = 0.0);
 
+// Sets *ctrl to `value` in stream order (a one-thread launch).
+void launch_set_control(Control* ctrl, const Control& value, hipStream_t stream);
 void launch_check(const PanelMatrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
                   int parity, bool ordered, const CheckParams& params, hipStream_t stream);
 

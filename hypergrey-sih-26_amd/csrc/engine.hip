@@ -200,7 +200,14 @@ void check_box(const hgqp_box* b) {
     if (!b) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_box handle");
 }
 
-// Writes the host view of the control block; used outside captured work only.
+// Sets the control block in stream order without waiting; used outside captured
+// work only.
+void set_control(hgqp_box* b, const Control& c) {
+    launch_set_control(b->ctrl, c, b->stream);
+    HIPCHECK(hipGetLastError());
+}
+
+// Writes the host view of the control block and waits for it.
 void put_control(hgqp_box* b, const Control& c) {
     std::memcpy(&b->host_slots[0], &c, sizeof(Control));
     HIPCHECK(hipMemcpyAsync(b->ctrl, &b->host_slots[0], sizeof(Control), hipMemcpyHostToDevice, b->stream));
@@ -392,16 +399,10 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
             }
         }
     }
-    HIPCHECK(hipStreamSynchronize(b->stream));
+    // no wait here: every caller reads the control block next, which waits for
+    // the periods still queued
     out.iterations = it;
     return out;
-}
-
-// Small host-side edits of the control block between phases (not captured).
-void edit_control(hgqp_box* b, void (*edit)(Control&)) {
-    Control c = get_control(b);
-    edit(c);
-    put_control(b, c);
 }
 
 }  // namespace
@@ -814,7 +815,7 @@ int hgqp_box_reset(hgqp_box* b) {
         b->parity = 0;
         launch_init<double>(b->n, b->lo, b->up, b->x[0], b->w[0], b->stream);
         HIPCHECK(hipGetLastError());
-        put_control(b, fresh_control());
+        set_control(b, fresh_control());
         b->at_zero = b->zero_start;
     });
 }
@@ -843,7 +844,7 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         std::vector<TimedPeriod>* timed_ptr = b->opt.timing ? &timed : nullptr;
 
         // fresh momentum state at the handle's current iterate (accelerated_qp.cpp:37)
-        put_control(b, fresh_control());
+        set_control(b, fresh_control());
         uint64_t it = 0;
         bool time_up = false;
         // check at iteration 0 (accelerated_qp.cpp:40); at x0 = 0 the gradient is c, no matrix pass
@@ -864,7 +865,7 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             cc.iter[b->parity] = int64_t(at);
             cc.best_stat = std::numeric_limits<double>::infinity();
             cc.stall = 0;
-            put_control(b, cc);
+            set_control(b, cc);
         };
         // One FP32 phase from the current parity; on return x32/w32[b->parity]
         // hold the phase's last checked iterate.
