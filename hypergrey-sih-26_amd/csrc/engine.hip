@@ -354,11 +354,11 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
                       const CheckParams& cp, std::chrono::steady_clock::time_point t0, double time_limit,
                       std::vector<TimedPeriod>* timed) {
     LoopOutcome out;
-    // Periods enqueued ahead of the control read-back. Two hide the host round
-    // trip behind short periods; one suffices once a period's steps take far
-    // longer than the round trip (large instances), and then at most one period
-    // of launches is wasted after the deciding check.
-    const int lag = b->nnz >= (int64_t(1) << 22) ? 1 : 2;
+    // Periods enqueued ahead of the control read-back: one, so that at most one
+    // period of launches is wasted after the deciding check. Two hid more of the
+    // host round trip on small instances, but their wasted periods cost more
+    // (n = 1e4: 2,594 solves/s with two, 2,957 with one).
+    const int lag = 1;
     std::vector<int> pending;  // slot indices of enqueued read-backs, oldest first
     int next_slot = 1;         // slot 0 is kept for synchronous transfers
     bool stop = false;
