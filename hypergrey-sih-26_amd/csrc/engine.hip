@@ -352,7 +352,7 @@ struct LoopOutcome {
 // or the time limit. `it` counts iterations of the whole solve.
 LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, uint64_t limit, int check_every,
                       const CheckParams& cp, std::chrono::steady_clock::time_point t0, double time_limit,
-                      std::vector<TimedPeriod>* timed) {
+                      std::vector<TimedPeriod>* timed, double prior_stat) {
     LoopOutcome out;
     // Periods enqueued ahead of the control read-back: one, so that at most one
     // period of launches is wasted after the deciding check. Two hid more of the
@@ -369,8 +369,39 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
     // launches queued past the end are then a few short periods, not two full
     // ones. Phases decided by a check in every period keep their cadence.
     const bool shrink = residual_phase(precision, momentum);
+    // Phases decided by a check in every period: when the period in flight will
+    // likely verify (the stationarity of its check, extrapolated geometrically
+    // from the last two checks read, at most the tolerance), its read-back is
+    // awaited before another period is queued, so no period follows the
+    // deciding one. A wrong guess costs one host round trip with the GPU idle;
+    // the device check still decides.
+    double last_stat = prior_stat, prev_stat = std::numeric_limits<double>::quiet_NaN();
+    auto likely_last = [&] {
+        if (shrink || !(last_stat > 0) || !std::isfinite(prev_stat) || !(prev_stat > last_stat)) return false;
+        return last_stat * (last_stat / prev_stat) <= cp.tol;
+    };
+    // waits for read-back slot s; true when the phase is over
+    auto read = [&](int s) {
+        HIPCHECK(hipEventSynchronize(b->slot_event[s]));
+        if (b->host_slots[s].done != kRunning) return true;
+        prev_stat = last_stat;
+        last_stat = b->host_slots[s].stationarity;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= time_limit) {
+            out.time_up = true;
+            return true;
+        }
+        return false;
+    };
     int period = check_every;
     while (it < limit && !stop) {
+        if (!pending.empty() && likely_last()) {
+            while (!pending.empty() && !stop) {
+                const int s = pending.front();
+                pending.erase(pending.begin());
+                stop = read(s);
+            }
+            if (stop) break;
+        }
         const int steps = static_cast<int>(std::min<uint64_t>(period, limit - it));
         if (shrink) period = std::max(2, (period + 1) / 2);
         if (timed) enqueue_period_timed(b, precision, steps, momentum, cp, it, timed);
@@ -387,16 +418,10 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
         HIPCHECK(hipMemcpyAsync(&b->host_slots[slot], b->ctrl, sizeof(Control), hipMemcpyDeviceToHost, b->stream));
         HIPCHECK(hipEventRecord(b->slot_event[slot], b->stream));
         pending.push_back(slot);
-        while ((int)pending.size() > lag || (it >= limit && !pending.empty())) {
+        while (!stop && ((int)pending.size() > lag || (it >= limit && !pending.empty()))) {
             const int s = pending.front();
             pending.erase(pending.begin());
-            HIPCHECK(hipEventSynchronize(b->slot_event[s]));
-            if (b->host_slots[s].done != kRunning) { stop = true; break; }
-            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= time_limit) {
-                out.time_up = true;
-                stop = true;
-                break;
-            }
+            stop = read(s);
         }
     }
     // no wait here: every caller reads the control block next, which waits for
@@ -873,7 +898,7 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             CheckParams cp32 = check_params(b, so.tolerance, 1, stat_target);
             const size_t from = timed.size();
             auto o = run_phase(b, HGQP_PRECISION_FP32_POLISH, momentum, it, so.iteration_limit, so.check_every, cp32,
-                               t0, so.time_limit, timed_ptr);
+                               t0, so.time_limit, timed_ptr, c.stationarity);
             time_up = o.time_up;
             c = get_control(b);
             if (c.done != kRunning) {  // steps after the deciding check returned early
@@ -919,7 +944,7 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             new_phase(it);
             const size_t from = timed.size();
             auto o = run_phase(b, HGQP_PRECISION_FP64, momentum, it, so.iteration_limit, every, cp64, t0,
-                               so.time_limit, timed_ptr);
+                               so.time_limit, timed_ptr, c.stationarity);
             time_up = o.time_up;
             c = get_control(b);
             if (c.done != kRunning) b->parity = c.check_parity;
