@@ -862,7 +862,11 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         const bool f32 = precision != HGQP_PRECISION_FP64;
         const bool refine = precision == HGQP_PRECISION_FP32_REFINE;
         const int polish_every = so.polish_every > 0 ? so.polish_every : so.check_every;
-        const double switch_stat = so.fp32_switch > 0 ? so.fp32_switch : 10 * so.tolerance;
+        // The first FP32 phase runs down to twice the tolerance by default: on the
+        // 1e6-row planted instance its FP64 check then verifies (KKT 8.3e-7) and
+        // the second cycle is not needed (466 solves/s against 372 at ten times
+        // the tolerance); where it does not verify, the next cycle runs as before.
+        const double switch_stat = so.fp32_switch > 0 ? so.fp32_switch : 2 * so.tolerance;
         CheckParams cp64 = check_params(b, so.tolerance, 1, 0.0);
         if (refine) cp64.gout = b->g;  // the anchor gradient of the next correction cycle
         std::vector<TimedPeriod> timed;

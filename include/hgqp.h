@@ -136,8 +136,8 @@ typedef struct hgqp_solve_options {
     uint64_t iteration_limit;/* default 200000                                    */
     int32_t check_every;     /* check cadence, 25 in the reference (accelerated_qp.cpp:40) */
     int32_t polish_every;    /* cadence of FP64 checks after an FP32 phase (0 = check_every) */
-    double fp32_switch;      /* FP32_POLISH: switch to FP64 once the FP32 check's
-                                stationarity falls below this (0 = 10*tolerance)  */
+    double fp32_switch;      /* FP32 modes: the first FP32 phase ends once its
+                                stationarity falls below this (0 = 2*tolerance)   */
 } hgqp_solve_options;
 
 void hgqp_solve_options_default(hgqp_solve_options* options);
