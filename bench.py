@@ -63,7 +63,8 @@ def measured_traffic(use32, mode, n, draws, nnz, panels, slabs, groups):
         return None
     with open(path) as f:
         t = json.load(f)
-    if (t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or t.get("mode") != mode
+    modes = t.get("modes", [t.get("mode")])  # modes whose FP32 step is the profiled one
+    if (t.get("workload") != f"planted n={n} draws={draws} nnz={nnz}" or mode not in modes
             or t.get("column_panels32") != panels or t.get("slabs32", 0) != slabs
             or t.get("slab_groups", 1 if slabs else 0) != groups):
         return None
