@@ -765,6 +765,10 @@ void launch_check(const PanelMatrix<double>& pm, const StepBuffers<double>& b, C
                        stream);
     const Matrix<double>& m = pm.pass[pm.count - 1];
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
+    // at x = 0 the check is elementwise; a capped grid-stride grid leaves the
+    // finalize launch fewer partials to combine
+    constexpr unsigned kZeroCheckGroups = 512;
+    if (params.zero_x) grid.x = std::min(grid.x, kZeroCheckGroups);
     if (ordered)
         hipLaunchKernelGGL(check_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params, carry);
     else
