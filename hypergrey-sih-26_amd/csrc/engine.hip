@@ -156,10 +156,11 @@ struct hgqp_box {
     static constexpr int kSlots = 8;
     hipEvent_t slot_event[kSlots] = {};
     int parity = 0;
-    // captured check periods, keyed by (precision, start parity, steps)
+    // captured check periods, keyed by (precision, start parity, steps,
+    // momentum) and every check parameter the captured launches hold by value
     struct Graph {
         int precision, parity, steps, momentum;
-        double switch_stat;
+        CheckParams cp;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -324,22 +325,44 @@ void enqueue_period_timed(hgqp_box* b, int precision, int steps, int momentum, c
     launch_period_check(b, precision, momentum, cp);
 }
 
+// The captured kernels take their CheckParams by value, so a graph is reused
+// only for the same parameters; a solve with another tolerance captures anew.
+bool same_check(const CheckParams& a, const CheckParams& b) {
+    return a.tol == b.tol && a.sign == b.sign && a.offset == b.offset && a.diagonal == b.diagonal &&
+           a.set_done == b.set_done && a.switch_stat == b.switch_stat && a.gout == b.gout && a.zero_x == b.zero_x;
+}
+
 hipGraphExec_t period_graph(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
     for (auto& g : b->graphs)
         if (g.precision == precision && g.parity == b->parity && g.steps == steps && g.momentum == momentum &&
-            g.switch_stat == cp.switch_stat)
+            same_check(g.cp, cp))
             return g.exec;
+    // A handle solved at many tolerances keeps a bounded set: past the cap the
+    // cached executables are dropped once the stream has drained them.
+    constexpr size_t kMaxGraphs = 48;
+    if (b->graphs.size() >= kMaxGraphs) {
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+        b->graphs.clear();
+    }
     const int p0 = b->parity;
-    hipGraph_t graph;
+    hipGraph_t graph = nullptr;
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_period(b, precision, steps, momentum, cp);
+    try {
+        enqueue_period(b, precision, steps, momentum, cp);
+    } catch (...) {
+        // leave the stream out of capture mode before the error goes up
+        if (hipStreamEndCapture(b->stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        b->parity = p0;
+        throw;
+    }
     HIPCHECK(hipStreamEndCapture(b->stream, &graph));
     hipGraphExec_t exec;
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIPCHECK(e);
     b->parity = p0;
-    b->graphs.push_back({precision, p0, steps, momentum, cp.switch_stat, exec});
+    b->graphs.push_back({precision, p0, steps, momentum, cp, exec});
     return exec;
 }
 

@@ -47,7 +47,9 @@ constexpr int kSlabRows = 4096;    // rows per row block: its FP32 sums take 16 
 // Columns per slab and rows per row block with G workgroups per block: three
 // slabs in the LDS ring plus the block's FP32 sums fit 160 KiB.
 constexpr int slab_width(int groups) { return groups >= 4 ? 8192 : groups == 2 ? 10240 : kSlabWidth; }
-constexpr int slab_rows_max(int groups) { return groups >= 4 ? 16384 : groups == 2 ? 8192 : kSlabRows; }
+// (16383, not 16384, for four groups: local row 16383 with both run marks set
+// would read as the padding code.)
+constexpr int slab_rows_max(int groups) { return groups >= 4 ? 16383 : groups == 2 ? 8192 : kSlabRows; }
 // First slab of range h when `slabs` slabs are split over G workgroups.
 __host__ __device__ inline int slab_range_start(int h, int slabs, int groups) {
     return (h * slabs + groups - 1) / groups;
@@ -57,6 +59,15 @@ constexpr unsigned kSlabPad = 0xFFFF;     // row value of a padding entry
 constexpr unsigned kSlabCont = 0x8000;    // row mark: an entry of a run whose first chunk is an earlier one
 constexpr unsigned kSlabFollow = 0x4000;  // row mark: the run goes on into the next chunk
 constexpr unsigned kSlabRowMask = 0x3FFF; // row within the block
+
+// No real entry of a block of any shape can take the padding code, whatever
+// marks it carries, and every local row fits the row field.
+constexpr bool slab_row_word_ok(int groups) {
+    return unsigned(slab_rows_max(groups) - 1) <= kSlabRowMask &&
+           (unsigned(slab_rows_max(groups) - 1) | kSlabCont | kSlabFollow) != kSlabPad;
+}
+static_assert(slab_row_word_ok(1) && slab_row_word_ok(2) && slab_row_word_ok(4),
+              "the last local row of a block, with both marks, must not equal the padding code");
 
 struct SlabMatrix {
     int blocks = 0;                 // row blocks; 0: no slabs

@@ -253,3 +253,90 @@ class Oracle:
 
 def available(kind):
     return os.path.exists(LIBS[kind][0])
+
+
+# -- numpy restatement of the FISTA step in a chosen precision -------------------
+
+def row_abs_sums(start, value):
+    """Per-row sum of |H_ik| in stored order, accumulated in FP64 one term at a
+    time (the Jacobi step denominators before their 1 + 2^-20 factor)."""
+    return _ordered_row_sums(start, np.abs(np.asarray(value, dtype=np.float64)), None, np.float64)
+
+
+def _ordered_row_sums(start, value, gathered, dtype):
+    """sum_k value[k] * gathered[k] over each row, added one rounded term at a
+    time in stored column order (sparse_matrix.hpp:26-33), in `dtype`
+    arithmetic; with gathered None the terms are value[k] themselves. Rows are
+    processed position by position, so the work is O(nnz)."""
+    start = np.asarray(start, dtype=np.int64)
+    n = len(start) - 1
+    lens = np.diff(start)
+    order = np.argsort(-lens, kind="stable")  # rows by decreasing length
+    sorted_lens = lens[order]
+    s = np.zeros(n, dtype=dtype)
+    maxlen = int(lens.max()) if n else 0
+    for j in range(maxlen):
+        count = int(np.searchsorted(-sorted_lens, -j, side="left"))  # rows with more than j entries
+        rows = order[:count]
+        k = start[rows] + j
+        term = value[k] if gathered is None else value[k] * gathered[k]
+        s[rows] = s[rows] + term
+    return s
+
+
+def _column_range(start, index, lo, hi):
+    """The CSR restricted to entries with lo <= column < hi, stored order kept:
+    (start, positions of the kept entries)."""
+    keep = (index >= lo) & (index < hi)
+    n = len(start) - 1
+    row_of = np.repeat(np.arange(n), np.diff(start))
+    counts = np.bincount(row_of[keep], minlength=n)
+    sub = np.zeros(len(start), dtype=np.int64)
+    np.cumsum(counts, out=sub[1:])
+    return sub, np.nonzero(keep)[0]
+
+
+def fista_steps_np(start, index, value, c, lower, upper, denominator, betas, x, w, dtype=np.float64, ranges=None):
+    """len(betas) projected gradient steps with extrapolation
+    (accelerated_qp.cpp:47-57, CPU branch) in `dtype` arithmetic:
+
+        g    = H w                         (each row in stored column order)
+        next = max(l, min(u, w - (g + c) / d))
+        w    = next + beta (next - x);  x = next
+
+    d is the scalar L (a float) or an array of per-row denominators. Matrix,
+    vectors, d and each beta are first rounded to `dtype`; every product,
+    sum, quotient and clip is a single IEEE operation in `dtype`, with no
+    fused multiply-add. In FP64 with the scalar L this is the oracle's
+    orc_fista_steps bit for bit (tests/test_oracle.py); in FP32 it is the
+    device's FP32 step.
+
+    `ranges` (column boundaries 0 = b_0 < b_1 < ... < b_G = n) restates a
+    row sum taken by column ranges: each range's entries are summed in stored
+    order from 0, and the G range sums are added in range order,
+    g = ((p_0 + p_1) + p_2) + ... Returns the new (x, w)."""
+    f = np.dtype(dtype).type
+    start = np.asarray(start, dtype=np.int64)
+    index = np.asarray(index, dtype=np.int64)
+    val = np.asarray(value, dtype=np.float64).astype(dtype)
+    c, lo, up = (np.asarray(a, dtype=np.float64).astype(dtype) for a in (c, lower, upper))
+    den = np.asarray(denominator, dtype=np.float64).astype(dtype)
+    x = np.asarray(x, dtype=np.float64).astype(dtype)
+    w = np.asarray(w, dtype=np.float64).astype(dtype)
+    parts = None
+    if ranges is not None:
+        parts = [_column_range(start, index, ranges[h], ranges[h + 1]) for h in range(len(ranges) - 1)]
+    for b in betas:
+        if parts is None:
+            g = _ordered_row_sums(start, val, w[index], dtype)
+        else:
+            g = None
+            for sub, pos in parts:
+                p = _ordered_row_sums(sub, val[pos], w[index[pos]], dtype)
+                g = p if g is None else g + p
+        v = w - (g + c) / den
+        m = np.where(v < up, v, up)
+        nxt = np.where(lo < m, m, lo)
+        w = nxt + f(b) * (nxt - x)
+        x = nxt
+    return x, w

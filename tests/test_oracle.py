@@ -253,3 +253,113 @@ def test_planted_optimum_is_stationary(oracle, hg):
     assert abs(c["objective"] - d["objective"]) <= 1e-13 * (1 + abs(d["objective"]))
     r = oracle.solve_box(q, tolerance=1e-9, iteration_limit=5000)
     assert r["status"] == "optimal" and np.abs(r["primal"] - d["optimum"]).max() < 1e-6
+
+
+def test_numpy_steps_equal_oracle_in_fp64(oracle, hg):
+    """fista_steps_np in FP64 with the scalar L is orc_fista_steps bit for bit
+    (long rows, empty rows, a planted instance), so the numpy restatement has
+    the oracle's order of operations before it is run in FP32."""
+    from helpers import planted_qp, random_csr_qp
+    from oracle import fista_steps_np
+    cases = [random_csr_qp(400, 3, long_rows=(5, 399), density=0.02, empty_rows=(7, 8)),
+             planted_qp(hg, 2000, 8, 4)[0], random_triplets(5)]
+    for q in cases:
+        s, i, v = oracle.build_csr(q)
+        L = oracle.lipschitz(s, v)
+        x0 = np.clip(np.zeros(q.n), q.lower, q.upper)
+        betas = oracle.betas(12)
+        xc, wc = oracle.fista_steps(s, i, v, q.cost, q.lower, q.upper, L, betas, x0, x0)
+        xn, wn = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, L, betas, x0, x0)
+        assert xn.dtype == np.float64
+        assert np.array_equal(xn, xc) and np.array_equal(wn, wc)
+
+
+def test_numpy_steps_fp32_brute_force(oracle):
+    """FP32 steps against scalar loops of np.float32 operations, one rounding per
+    product, sum, quotient and extrapolation, rows in stored column order; with
+    per-row denominators (the Jacobi metric) and with the scalar L."""
+    from helpers import random_csr_qp
+    from oracle import fista_steps_np, row_abs_sums
+    q = random_csr_qp(60, 11, long_rows=(3,), density=0.1, empty_rows=(20,))
+    s, i, v = oracle.build_csr(q)
+    L = oracle.lipschitz(s, v)
+    d = row_abs_sums(s, v) * (1 + 2.0 ** -20)
+    d[np.diff(s) == 0] = L
+    f = np.float32
+    betas = oracle.betas(6)
+    for den in (L, d):
+        x = np.clip(np.zeros(q.n), q.lower, q.upper).astype(f)
+        w = x.copy()
+        dd = np.broadcast_to(np.asarray(den, dtype=np.float64), (q.n,)).astype(f)
+        for b in betas:
+            g = np.zeros(q.n, dtype=f)
+            for r in range(q.n):
+                acc = f(0)
+                for k in range(s[r], s[r + 1]):
+                    acc = f(acc + f(f(v[k]) * w[i[k]]))
+                g[r] = acc
+            nx = np.empty(q.n, dtype=f)
+            nw = np.empty(q.n, dtype=f)
+            for r in range(q.n):
+                val = f(w[r] - f(f(g[r] + f(q.cost[r])) / dd[r]))
+                m = val if val < f(q.upper[r]) else f(q.upper[r])
+                nx[r] = m if f(q.lower[r]) < m else f(q.lower[r])
+                nw[r] = f(nx[r] + f(f(b) * f(nx[r] - x[r])))
+            x, w = nx, nw
+        x0 = np.clip(np.zeros(q.n), q.lower, q.upper)
+        xn, wn = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, den, betas, x0, x0, dtype=np.float32)
+        assert xn.dtype == np.float32
+        assert np.array_equal(xn, x) and np.array_equal(wn, w)
+        x64, _ = oracle.fista_steps(s, i, v, q.cost, q.lower, q.upper, L, betas, x0, x0)
+        if np.isscalar(den):  # FP32 rounding only: close to the FP64 steps
+            assert np.abs(xn - x64).max() < 1e-5
+
+
+def test_row_abs_sums_brute_force(oracle):
+    from helpers import random_csr_qp
+    from oracle import row_abs_sums
+    q = random_csr_qp(300, 2, long_rows=(1,), density=0.05, empty_rows=(4,))
+    s, i, v = oracle.build_csr(q)
+    expect = np.abs(dense_of(q)).sum(axis=1)
+    got = row_abs_sums(s, v)
+    assert got[4] == 0 and np.allclose(got, expect, rtol=1e-14, atol=0)
+    assert got.max() == oracle.lipschitz(s, v)
+
+
+def test_numpy_steps_by_column_range(oracle):
+    """A row sum taken by column ranges: with one range it is the plain ordered
+    sum; with ranges it equals, for every row, the range sums computed from
+    masked copies of the matrix and added in range order (checked in FP32,
+    where the grouping changes the rounding)."""
+    from helpers import random_csr_qp
+    from oracle import fista_steps_np
+    q = random_csr_qp(500, 8, long_rows=(2, 499), density=0.03, empty_rows=(9, 499 - 1))
+    s, i, v = oracle.build_csr(q)
+    L = oracle.lipschitz(s, v)
+    x0 = np.clip(np.zeros(q.n), q.lower, q.upper)
+    betas = oracle.betas(5)
+    for dt in (np.float64, np.float32):
+        a = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, L, betas, x0, x0, dtype=dt)
+        b = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, L, betas, x0, x0, dtype=dt, ranges=[0, q.n])
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    bounds = [0, 130, 260, 377, q.n]
+    got = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, L, betas[:1], x0, x0, dtype=np.float32, ranges=bounds)
+    f = np.float32
+    w = x0.astype(f)
+    g = np.zeros(q.n, dtype=f)
+    for h in range(4):
+        vh = np.where((i >= bounds[h]) & (i < bounds[h + 1]), v, 0.0)  # zeros add exactly
+        p = np.zeros(q.n, dtype=f)
+        for r in range(q.n):
+            acc = f(0)
+            for k in range(s[r], s[r + 1]):
+                if bounds[h] <= i[k] < bounds[h + 1]:
+                    acc = f(acc + f(f(vh[k]) * w[i[k]]))
+            p[r] = acc
+        g = p if h == 0 else f32_add(g, p)
+    expect = np.clip(w - (g + q.cost.astype(f)) / f(L), q.lower.astype(f), q.upper.astype(f))
+    assert np.array_equal(got[0], expect)
+
+
+def f32_add(a, b):
+    return (a.astype(np.float32) + b.astype(np.float32)).astype(np.float32)
