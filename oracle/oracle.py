@@ -55,6 +55,15 @@ class Options(C.Structure):
                 ("iteration_limit", C.c_uint64), ("threads", C.c_int32)]
 
 
+class QplibView(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("nterms", C.c_int64),
+                ("hrow", C.POINTER(C.c_int64)), ("hcol", C.POINTER(C.c_int64)), ("hval", C.POINTER(C.c_double)),
+                ("cost", C.POINTER(C.c_double)), ("lower", C.POINTER(C.c_double)), ("upper", C.POINTER(C.c_double)),
+                ("rstart", C.POINTER(C.c_int64)), ("rcol", C.POINTER(C.c_int64)), ("rval", C.POINTER(C.c_double)),
+                ("row_lower", C.POINTER(C.c_double)), ("row_upper", C.POINTER(C.c_double)),
+                ("offset", C.c_double), ("maximize", C.c_int32)]
+
+
 class OracleError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(message)
@@ -233,6 +242,31 @@ class Oracle:
         self._raise(self._fn("check_rows")(C.byref(s), C.byref(rs), _ptr(x, C.c_double), _ptr(y, C.c_double),
                                            tol, C.byref(r)))
         return r.as_dict()
+
+    def read_qplib(self, text):
+        """The reference's own QPLIB reader (read_qplib, qp.cpp:67-117; reference
+        library only): None when it rejects the text, else the parsed model as
+        the dict that hgqp.read_qplib returns."""
+        if self.kind != "reference":
+            raise OracleError(3, "the QPLIB reader is pinned against the compiled reference only")
+        if isinstance(text, str):
+            text = text.encode()
+        f = self.lib.ref_read_qplib
+        f.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(QplibView)]
+        v = QplibView()
+        if f(text, len(text), C.byref(v)) != 0:
+            return None
+
+        def arr(p, k, t):
+            return np.ctypeslib.as_array(p, shape=(k,)).astype(t, copy=True) if k else np.zeros(0, t)
+        k = int(v.rstart[v.m]) if v.m else 0
+        return dict(n=v.n, hrow=arr(v.hrow, v.nterms, np.int64), hcol=arr(v.hcol, v.nterms, np.int64),
+                    hval=arr(v.hval, v.nterms, np.float64), cost=arr(v.cost, v.n, np.float64),
+                    lower=arr(v.lower, v.n, np.float64), upper=arr(v.upper, v.n, np.float64),
+                    offset=v.offset, maximize=bool(v.maximize), rows=v.m,
+                    rstart=arr(v.rstart, v.m + 1, np.int64), rcol=arr(v.rcol, k, np.int64),
+                    rval=arr(v.rval, k, np.float64), row_lower=arr(v.row_lower, v.m, np.float64),
+                    row_upper=arr(v.row_upper, v.m, np.float64))
 
     def solve_admm(self, q, rows, tolerance=1e-7, iteration_limit=200000, time_limit=60.0, threads=1):
         """The reference's own solve_qp_admm (reference library only)."""

@@ -62,6 +62,26 @@ typedef struct orc_options {
     int32_t threads;          /* QpOptions::threads */
 } orc_options;
 
+/* A parsed QPLIB model (read_qplib, qp.cpp:67-117), flattened: Hessian
+ * triplets in model order, rows as CSR in model order. Arrays are owned by the
+ * library and stay valid until its next read on the same thread. */
+typedef struct orc_qplib_view {
+    int64_t n, m, nterms;
+    const int64_t* hrow;
+    const int64_t* hcol;
+    const double* hval;
+    const double* cost;
+    const double* lower;     /* [n] */
+    const double* upper;     /* [n] */
+    const int64_t* rstart;   /* [m + 1] */
+    const int64_t* rcol;
+    const double* rval;
+    const double* row_lower; /* [m] */
+    const double* row_upper; /* [m] */
+    double offset;
+    int32_t maximize;
+} orc_qplib_view;
+
 /* Return codes: 0 ok, 1 invalid argument (reference std::invalid_argument),
  * 2 runtime error, 3 path not restated (zero-Hessian LP fallback). */
 #define ORC_OK 0

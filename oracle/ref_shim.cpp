@@ -20,8 +20,10 @@
 #include "sparse_matrix.hpp"
 
 #include <cstring>
+#include <sstream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace {
 thread_local std::string g_error;
@@ -180,6 +182,54 @@ int ref_solve_admm(const orc_problem* p, const orc_rows* rows, const orc_options
             std::memcpy(primal_out, r.primal.data(), r.primal.size() * sizeof(double));
         if (row_dual_out && r.row_dual.size() == q.linear.rows.size())
             std::memcpy(row_dual_out, r.row_dual.data(), r.row_dual.size() * sizeof(double));
+    });
+}
+
+/* The reference's QPLIB reader (read_qplib, qp.cpp:67-117) on a text; any
+ * exception it raises is a rejection (nonzero return). The view's arrays live
+ * until the next call on this thread. */
+int ref_read_qplib(const char* text, size_t length, orc_qplib_view* out) {
+    struct Held {
+        sovereign::QuadraticProblem q;
+        std::vector<int64_t> hrow, hcol, rstart, rcol;
+        std::vector<double> hval, rval;
+    };
+    static thread_local Held held;
+    return guarded([&] {
+        std::istringstream in(std::string(text, length));
+        held = Held();
+        held.q = sovereign::read_qplib(in);
+        for (const auto& t : held.q.hessian) {
+            held.hrow.push_back(int64_t(t.row));
+            held.hcol.push_back(int64_t(t.col));
+            held.hval.push_back(t.value);
+        }
+        const auto& lp = held.q.linear;
+        held.rstart.push_back(0);
+        for (const auto& row : lp.rows) {
+            for (const auto& e : row) {
+                held.rcol.push_back(int64_t(e.first));
+                held.rval.push_back(e.second);
+            }
+            held.rstart.push_back(int64_t(held.rcol.size()));
+        }
+        std::memset(out, 0, sizeof(*out));
+        out->n = int64_t(lp.cost.size());
+        out->m = int64_t(lp.rows.size());
+        out->nterms = int64_t(held.hval.size());
+        out->hrow = held.hrow.data();
+        out->hcol = held.hcol.data();
+        out->hval = held.hval.data();
+        out->cost = lp.cost.data();
+        out->lower = lp.lower.data();
+        out->upper = lp.upper.data();
+        out->rstart = held.rstart.data();
+        out->rcol = held.rcol.data();
+        out->rval = held.rval.data();
+        out->row_lower = lp.row_lower.data();
+        out->row_upper = lp.row_upper.data();
+        out->offset = lp.offset;
+        out->maximize = lp.maximize ? 1 : 0;
     });
 }
 

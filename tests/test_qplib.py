@@ -143,3 +143,116 @@ def test_cli_sends_linear_rows_to_admm(tmp_path):
     assert abs(r["objective"] - 0.25) <= 1e-6
     assert np.abs(np.array(r["primal"]) - 0.5).max() <= 1e-6
     assert abs(r["row_dual"][0] + 0.5) <= 1e-6
+
+
+# ---- the reader against the reference's own reader (oracle/_ref) -------------
+
+TINY = """tiny QCB maximize
+2 # variables
+3 # Hessian entries, lower triangle of Q in 1/2 x'Qx
+1 1 -4
+2 1 3
+2 2 -6
+0 # default cost
+1 # one nondefault cost
+2 5.5
+0.25 # offset
+1e20 # infinity
+-1 # default lower
+0
+1e20 # default upper (infinite)
+1
+1 2
+0 0
+0 0
+0
+0
+"""
+
+POOL = ["0", "-1", "1", "2", "3", "1.5", "-0", "1e20", "-1e20", "1e400", "-1e400", "1e-5000", "4e-324", "nan",
+        "inf", "-inf", "0x1p1", "+2", "2x", "abc", "100000", "100001", "5000001", "1e30", "QCB", "QCL", "QIB",
+        "minimize", "maximize", "#", "1#2"]
+
+
+def parse_or_none(hg, text):
+    try:
+        return hg.read_qplib(text)
+    except hg.InvalidArgument:
+        return None
+
+
+def bits(a):
+    a = np.asarray(a)
+    return a.view(np.uint64) if a.dtype == np.float64 else a
+
+
+def assert_same_model(a, b, label):
+    assert (a is None) == (b is None), (label, "accepted by one reader only", a is None, b is None)
+    if a is None:
+        return
+    for k in ("n", "rows", "maximize"):
+        assert a[k] == b[k], (label, k)
+    assert bits(np.float64(a["offset"])) == bits(np.float64(b["offset"])), (label, "offset")
+    for k in ("hrow", "hcol", "hval", "cost", "lower", "upper", "rstart", "rcol", "rval", "row_lower", "row_upper"):
+        assert len(a[k]) == len(b[k]) and np.array_equal(bits(a[k]), bits(b[k])), (label, k)
+
+
+def corpus():
+    from helpers import QCL
+    bases = {"tiny": TINY, "qcl": QCL}
+    out = [("8792", model_text("8792").decode()), ("8991", model_text("8991").decode())]
+    for name, text in bases.items():
+        out.append((name, text))
+        words = text.split()
+        out.append((name + ":one_line", " ".join(words)))
+        out.append((name + ":crlf_tabs", text.replace("\n", "\r\n").replace(" ", "\t")))
+        for k in range(len(words)):
+            out.append((f"{name}:cut{k}", " ".join(words[:k])))
+            out.append((f"{name}:drop{k}", " ".join(words[:k] + words[k + 1:])))
+            for w in POOL:
+                out.append((f"{name}:w{k}={w}", " ".join(words[:k] + [w] + words[k + 1:])))
+        for tail in (" 7", "\n7", "\n# trailing comment\n\n", "\n\v\n", "\n \t\r\n", "\f", "\n\f\n", "\n#x 7\n",
+                     "\n\x00\n"):
+            out.append((f"{name}:tail{tail!r}", text.rstrip("\n") + tail))
+    return out
+
+
+def test_reader_equals_reference_reader(hg, reference):
+    """Accept/reject and every parsed array (bit for bit, signed zeros included)
+    equal the reference's own read_qplib on the two public models, two small
+    models, and every truncation, single-word deletion and single-word
+    replacement (from a pool of numbers, out-of-range values, non-finite and
+    malformed words, type codes and comment marks) of the small ones, plus
+    trailing content of every kind."""
+    accepted = rejected = 0
+    for label, text in corpus():
+        mine, ref = parse_or_none(hg, text), reference.read_qplib(text)
+        assert_same_model(mine, ref, label)
+        accepted += mine is not None
+        rejected += mine is None
+    print("accepted", accepted, "rejected", rejected, flush=True)
+    assert accepted > 100 and rejected > 500
+
+
+def test_reader_random_mutations_equal_reference(hg, reference):
+    """Seeded random edits of the small models (two to four word swaps,
+    duplications and replacements at once) are accepted or rejected alike, and
+    accepted ones parse to the same model."""
+    from helpers import QCL
+    rng = np.random.default_rng(2026)
+    for base in (TINY, QCL):
+        words = base.split()
+        for trial in range(1500):
+            w = list(words)
+            for _ in range(int(rng.integers(2, 5))):
+                op = int(rng.integers(0, 3))
+                k = int(rng.integers(0, len(w)))
+                if op == 0:
+                    j = int(rng.integers(0, len(w)))
+                    w[k], w[j] = w[j], w[k]
+                elif op == 1:
+                    w.insert(k, w[k])
+                else:
+                    w[k] = POOL[int(rng.integers(0, len(POOL)))]
+            text = "\n".join(w)
+            assert_same_model(parse_or_none(hg, text), reference.read_qplib(text), f"trial {trial}: {text!r}")
