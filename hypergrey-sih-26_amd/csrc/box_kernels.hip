@@ -758,7 +758,7 @@ void launch_set_control(Control* ctrl, const Control& value, hipStream_t stream)
 }
 
 void launch_check(const PanelMatrix<double>& pm, const StepBuffers<double>& b, Control* ctrl, double* partials,
-                  int parity, bool ordered, const CheckParams& params, hipStream_t stream) {
+                  int parity, bool ordered, const CheckParams& params, hipStream_t stream, int max_groups) {
     // at x = 0 the check reads no matrix entries, so the panel passes are skipped
     const double* carry = params.zero_x ? nullptr
         : launch_carry(pm, PassSource<double>{b.x[parity], b.x[parity], ctrl, parity, 0, params.set_done}, ordered,
@@ -769,6 +769,7 @@ void launch_check(const PanelMatrix<double>& pm, const StepBuffers<double>& b, C
     // finalize launch fewer partials to combine
     constexpr unsigned kZeroCheckGroups = 512;
     if (params.zero_x) grid.x = std::min(grid.x, kZeroCheckGroups);
+    if (max_groups > 0) grid.x = std::min(grid.x, unsigned(max_groups));
     if (ordered)
         hipLaunchKernelGGL(check_kernel<true>, grid, block, 0, stream, m, b, ctrl, partials, parity, params, carry);
     else

@@ -114,8 +114,10 @@ void launch_step(const PanelMatrix<T>& m, const StepBuffers<T>& b, Control* ctrl
 
 // Sets *ctrl to `value` in stream order (a one-thread launch).
 void launch_set_control(Control* ctrl, const Control& value, hipStream_t stream);
+// max_groups > 0 caps the check launch's workgroups (it grid-strides over the
+// row blocks), and so the partials it writes.
 void launch_check(const PanelMatrix<double>& m, const StepBuffers<double>& b, Control* ctrl, double* partials,
-                  int parity, bool ordered, const CheckParams& params, hipStream_t stream);
+                  int parity, bool ordered, const CheckParams& params, hipStream_t stream, int max_groups = 0);
 
 void launch_check32(const PanelMatrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
                     int parity, bool ordered, const CheckParams& params, hipStream_t stream);

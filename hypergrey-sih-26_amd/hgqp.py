@@ -19,6 +19,7 @@ PRECISION_FP64, PRECISION_FP32_POLISH, PRECISION_FP32_REFINE = 0, 1, 2
 MOMENTUM_REFERENCE, MOMENTUM_RESTART = 0, 1
 REDUCE_ORDERED, REDUCE_TREE = 0, 1
 STEP_SCALAR, STEP_JACOBI = 0, 1
+NOALLOC_MAX_ROWS = 2097152  # HGQP_NOALLOC_MAX_ROWS
 
 
 class HgqpError(RuntimeError):
@@ -216,6 +217,10 @@ SIGNATURES = {
     "hgqp_gen_tridiagonal": ([C.c_int64, P_I64, P_I64, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64], C.c_int),
     "hgqp_gen_planted": ([C.POINTER(Planted), P_I64, P_I32, P_I32, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64],
                          C.c_int),
+    # device pointers: start, index, value, cost, lower, upper, x; then tolerance, maximize, offset,
+    # diagonal_hessian, stream, device report
+    "hgqp_check_device_noalloc": ([C.c_int64, C.c_int64] + [C.c_void_p] * 7 +
+                                  [C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.c_void_p], C.c_int),
 }
 
 _lib = None
@@ -504,6 +509,15 @@ def gen_batch(count, seed, n_min=1000, n_max=100000, draws_min=4, draws_max=32, 
         d["draws"] = draws
         out.append(d)
     return out
+
+
+def check_device_noalloc(n, nnz, start, index, value, cost, lower, upper, x, tolerance, report, maximize=False,
+                         offset=0.0, diagonal_hessian=0, stream=None):
+    """hgqp_check_device_noalloc: every array argument and `report` are device
+    addresses (ints); the report is written there in stream order."""
+    _raise(lib().hgqp_check_device_noalloc(int(n), int(nnz), start, index, value, cost, lower, upper, x,
+                                           float(tolerance), int(bool(maximize)), float(offset),
+                                           int(diagonal_hessian), stream, report))
 
 
 def lower_triplets(start, index, value):

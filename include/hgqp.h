@@ -533,6 +533,33 @@ typedef struct hgqp_planted {
 int hgqp_gen_planted(const hgqp_planted* spec, int64_t* nnz, int32_t* start, int32_t* index, double* value,
                      double* cost, double* lower, double* upper, double* optimum, double* objective);
 
+/* ---- check without device allocation ------------------------------------ */
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc): the FP64 check of hgqp_box_check (check_qp_solution,
+ * qp_admm.cpp:63-117, rows = 0) of x, on a mirrored sign-normalized CSR, cost
+ * and bounds that the caller already holds in DEVICE memory. The report is
+ * written to DEVICE memory (one hgqp_check_report) in stream order on
+ * `stream` (NULL: the null stream); the call does not wait. Rows are summed in
+ * stored column order, as with HGQP_REDUCE_ORDERED; objective and bounds may
+ * differ from hgqp_box_check's in the last bits (another grouping of the
+ * double-double partial sums), stationarity and violation do not.
+ *
+ * The check's work buffers (control block, 2048 workgroup partials, row
+ * blocks) live in one fixed device array of the library, about 8.2 MB per GPU,
+ * held until the process ends; nothing is allocated per call. n may be at
+ * most HGQP_NOALLOC_MAX_ROWS; larger n is HGQP_INVALID_ARGUMENT. Every call on
+ * one GPU uses that GPU's array: calls that overlap on one GPU (other threads
+ * or streams) race and must be serialized by the caller. The CSR must be valid
+ * (start[0] = 0, start[n] = nnz, nondecreasing, columns in [0, n)); it is not
+ * checked on the device. diagonal_hessian: 1 when every stored entry is on a
+ * positive diagonal (the separable dual bound), else 0. */
+#define HGQP_NOALLOC_MAX_ROWS 2097152
+int hgqp_check_device_noalloc(int64_t n, int64_t nnz, const int32_t* start, const int32_t* index,
+                              const double* value, const double* cost, const double* lower, const double* upper,
+                              const double* x, double tolerance, int32_t maximize, double offset,
+                              int32_t diagonal_hessian, void* stream, hgqp_check_report* report);
+
 #ifdef __cplusplus
 }
 #endif
