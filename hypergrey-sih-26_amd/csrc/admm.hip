@@ -447,10 +447,15 @@ __global__ __launch_bounds__(kGroup) void norms_at(View v) {
 int grid_elem(int64_t n) { return int(std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096)); }
 
 // Captured launch sequences of one solve, destroyed with it.
+// Captured graphs of one solve. On every exit, the error paths included, the
+// stream drains before the executables are destroyed, so none is destroyed
+// while a launch of it may still be queued.
 struct Graphs {
+    hipStream_t stream = nullptr;
     std::vector<hipGraphExec_t> all;
     std::map<int, hipGraphExec_t> batch;
     ~Graphs() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (auto e : all) (void)hipGraphExecDestroy(e);
     }
 };
@@ -902,8 +907,12 @@ QpResult solve_qp_admm(const QuadraticProblem& original, const QpOptions& opt) {
         hipLaunchKernelGGL(relax_x, dim3(groups_e), blk, 0, st, v);
     };
     Graphs graphs;
+    graphs.stream = st;
+    // Captures begin on a drained stream only (a batch size first needed in the
+    // middle of a solve waits for the launches queued before it).
     auto capture = [&](auto&& launches) {
         hipGraph_t g;
+        HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         launches();
         HIPCHECK(hipStreamEndCapture(st, &g));

@@ -347,6 +347,7 @@ hipGraphExec_t period_graph(hgqp_box* b, int precision, int steps, int momentum,
     }
     const int p0 = b->parity;
     hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamSynchronize(b->stream));  // captures begin on a drained stream only
     HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     try {
         enqueue_period(b, precision, steps, momentum, cp);

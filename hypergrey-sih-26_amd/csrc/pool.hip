@@ -227,6 +227,9 @@ void launch_periods(hgqp_pool* p, int periods) {
         if (p->opt.use_graph) {
             if (!p->graph) {
                 hipGraph_t g;
+                // capture on a drained stream: the admissions' uploads and first
+                // checks queued before the first period complete first
+                HIPCHECK(hipStreamSynchronize(p->stream));
                 HIPCHECK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
                 launch_pool_period(p->v, p->loads, int(std::min<int64_t>(p->max_blocks, 1 << 30)), p->opt.check_every,
                                    p->stream);
