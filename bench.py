@@ -156,46 +156,82 @@ def whole_model(name, inst, check_every, polish_every, reps=2):
                           reduce=reduce, step_metric=metric)
         times.append(time.perf_counter() - a)
     return {"mode": name, "terms": int(len(hv)), "seconds": round(float(np.median(times)), 4),
+            "host_check_seconds": round(r["host_check_seconds"], 4),
+            "assembly_seconds": round(r["assembly_seconds"], 4),
             "status": r["status"], "iterations": int(r["iterations"]), "algorithm": r["algorithm"],
             "max_abs_err_vs_planted_x": float(np.abs(r["primal"] - inst["optimum"]).max())}
 
 
-def cpu_baseline(inst, iterations_needed, threads):
-    """The reference's own CPU path (oracle/_ref; the oracle port if it is absent)
-    on the same instance: one run that stops after setup and the iteration-0
-    check, one that runs 100 iterations (checks at 25, 50, 75, 100 included); a
-    full solve is extrapolated to the iteration count the reference schedule
-    needs on this instance."""
+def host_cpu():
+    """The host's CPU model, its logical CPUs, the CPUs this process may run on
+    and the physical cores among them (from /proc/cpuinfo)."""
+    usable = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    model, cores, cur = None, set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    continue
+                k, v = (s.strip() for s in line.split(":", 1))
+                if k == "processor":
+                    cur = {"cpu": int(v)}
+                elif k == "model name":
+                    model = model or v
+                elif k == "physical id":
+                    cur["package"] = v
+                elif k == "core id" and cur.get("cpu") in usable:
+                    cores.add((cur.get("package"), v))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": len(usable),
+            "usable_physical_cores": len(cores) or None}
+
+
+def reference_cpu():
+    """The reference's own CPU sources (oracle/_ref), else the oracle port."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import BoxQP, Oracle, available
+    from oracle import Oracle, available
     kind = "reference" if available("reference") else "oracle"
-    o = Oracle(kind)
+    source = "oracle/_ref (reference CPU sources)" if kind == "reference" else "oracle port"
+    return Oracle(kind), ("reference" if kind == "reference" else "port"), source
+
+
+def cpu_full_solve(o, q, tolerance, threads):
+    """One full CPU solve at the reference's timing boundary (scale_benchmark.cpp:20-21:
+    solve_accelerated_box from the triplet model, CSR assembly and the check every
+    25 iterations included, generation excluded)."""
+    a = time.perf_counter()
+    r = o.solve_box(q, tolerance=tolerance, iteration_limit=20000, threads=threads, time_limit=600)
+    return time.perf_counter() - a, r
+
+
+def cpu_baseline(inst, threads):
+    """The reference CPU path on the same instance, timed as full solves at the
+    reference's boundary: once on every CPU this process may use, once on one
+    thread; a setup-only run (iteration limit 0: assembly and the iteration-0
+    check) gives the loop time beside it."""
+    o, kind, source = reference_cpu()
+    from oracle import BoxQP
     hr, hc, hv = hgqp.lower_triplets(inst["start"], inst["index"], inst["value"])
     q = BoxQP(inst["n"], hr, hc, hv, inst["cost"], inst["lower"], inst["upper"])
-    source = 'oracle/_ref (reference CPU sources)' if kind == 'reference' else 'oracle port'
 
-    def sample(nthreads, sample_iterations):
+    def run(nthreads):
+        full, r = cpu_full_solve(o, q, KKT_TOL, nthreads)
         a = time.perf_counter()
         o.solve_box(q, tolerance=KKT_TOL, iteration_limit=0, threads=nthreads, time_limit=600)
-        t_setup = time.perf_counter() - a
-        a = time.perf_counter()
-        o.solve_box(q, tolerance=KKT_TOL, iteration_limit=sample_iterations, threads=nthreads, time_limit=600)
-        t_run = time.perf_counter() - a
-        per_iter = max(t_run - t_setup, 0.0) / sample_iterations
-        full = t_setup + per_iter * iterations_needed
-        text = (f"{source}, {nthreads} OpenMP thread{'s' if nthreads > 1 else ''}: setup+check {t_setup:.2f}s, "
-                f"setup+{sample_iterations} iterations {t_run:.2f}s; full solve = setup + {iterations_needed} "
-                f"iterations x {per_iter * 1e3:.1f} ms (checks every 25 included) = {full:.2f}s")
-        return full, t_setup, per_iter, text
+        setup = time.perf_counter() - a
+        text = (f"{source}, {nthreads} OpenMP thread{'s' if nthreads > 1 else ''}: one full solve of this "
+                f"instance from its triplet model, {r['iterations']} iterations ({r['status']}): {full:.2f}s, "
+                f"of which setup (CSR assembly + iteration-0 check) {setup:.2f}s")
+        return {"value": 1.0 / full, "unit": "solves/s", "cores": nthreads, "sample": text, "solve_seconds": full,
+                "setup_seconds": setup, "loop_seconds": full - setup, "iterations": int(r["iterations"]),
+                "status": r["status"], "objective": r["objective"]}
 
-    full, t_setup, per_iter, text = sample(threads, 100)
-    out = {"value": 1.0 / full, "unit": "solves/s", "cores": threads,
-           "kind": "reference" if kind == "reference" else "port", "sample": text,
-           "setup_seconds": t_setup, "ms_per_iteration": per_iter * 1e3}
+    out = run(threads)
+    out["kind"] = kind
+    out["host"] = host_cpu()
     if threads > 1:
-        full1, setup1, per1, text1 = sample(1, 25)
-        out["single_thread"] = {"value": 1.0 / full1, "unit": "solves/s", "cores": 1, "sample": text1,
-                                "setup_seconds": setup1, "ms_per_iteration": per1 * 1e3}
+        out["single_thread"] = run(1)
     return out
 
 
@@ -433,6 +469,103 @@ def dense_workload(args):
     }
 
 
+# The reference's published runs of its seeded tridiagonal family (median of
+# three, seconds): CPU 1 thread, CPU 4 threads, Tesla T4
+# (reports/kaggle-gpu-03/REPORT.md:19-23).
+PUBLISHED_TRIDIAGONAL = {10000: (0.1159, 0.0838, 0.3821), 50000: (0.5729, 0.3955, 0.5139),
+                         200000: (2.5509, 1.6158, 0.9799), 500000: (7.3144, 4.0860, 1.8996),
+                         1000000: (15.923, 8.342, 3.501)}
+TRIDIAGONAL_MODES = ("fp64_reference", "fp64_restart_jacobi", "fp32_refine_restart_jacobi")
+
+
+def tridiagonal_workload(args):
+    """The reference's own scale benchmark (scale_benchmark.cpp:13-25): the
+    seeded tridiagonal family H = tridiag(-1, 2.02, -1), x in [0, 3], tolerance
+    1e-7, timed from the triplet model to the verified result (assembly, upload,
+    iterations, checks and the read-back of x), generation excluded, median of
+    three. Each n runs the FP64 reference schedule (bit-identical to the CPU path:
+    1700 iterations) and the fast modes through hgqp_solve_qp, the reference's CPU
+    sources on every usable CPU and on one thread, and holds every result to the
+    reference's pass rule (optimal and verified, objective within 1e-8 of the
+    planted one, max |x - x*| < 1e-4)."""
+    o, kind, source = reference_cpu()
+    from oracle import BoxQP
+    threads = args.cpu_threads or host_cpu()["usable_cpus"]
+    sizes = [int(s) for s in args.tri_sizes.split(",")]
+    out = []
+    for n in sizes:
+        d = hgqp.gen_tridiagonal(n)
+        f_star = d["objective"]
+
+        def assess(r, seconds):
+            err = float(np.abs(r["primal"] - d["optimum"]).max())
+            obj_err = abs(r["objective"] - f_star) / (1 + abs(f_star))
+            return {"seconds": seconds, "iterations": int(r["iterations"]), "status": r["status"],
+                    "objective": r["objective"], "objective_rel_err": obj_err, "max_abs_err": err,
+                    "passed": bool(r["status"] == "optimal" and r["optimality_verified"] and obj_err <= 1e-8
+                                   and err < 1e-4)}
+        row = {"n": n, "gpu": {}}
+        for name in TRIDIAGONAL_MODES:
+            precision, momentum, reduce, metric = MODES[name]
+            kw = dict(tolerance=1e-7, iteration_limit=20000, time_limit=120, precision=precision, momentum=momentum,
+                      reduce=reduce, step_metric=metric)
+            hgqp.solve_qp(n, d["hrow"], d["hcol"], d["hval"], d["cost"], d["lower"], d["upper"], **kw)  # warm
+            times, r = [], None
+            for _ in range(3):
+                a = time.perf_counter()
+                r = hgqp.solve_qp(n, d["hrow"], d["hcol"], d["hval"], d["cost"], d["lower"], d["upper"], **kw)
+                times.append(time.perf_counter() - a)
+            row["gpu"][name] = assess(r, float(np.median(times)))
+            row["gpu"][name]["algorithm"] = r["algorithm"]
+            if name == "fp64_reference":
+                gpu_x = r["primal"]
+        q = BoxQP(n, d["hrow"], d["hcol"], d["hval"], d["cost"], d["lower"], d["upper"])
+        t_all, r_all = cpu_full_solve(o, q, 1e-7, threads)
+        t_one, r_one = cpu_full_solve(o, q, 1e-7, 1)
+        row["cpu"] = {"threads": threads, "all_threads": assess(r_all, t_all), "one_thread": assess(r_one, t_one)}
+        row["fp64_reference_bit_identical_to_cpu"] = bool(
+            np.array_equal(gpu_x, r_all["primal"]) and row["gpu"]["fp64_reference"]["iterations"] == r_all["iterations"])
+        pub = PUBLISHED_TRIDIAGONAL.get(n)
+        if pub:
+            row["published"] = {"cpu_1_thread_s": pub[0], "cpu_4_threads_s": pub[1], "t4_gpu_s": pub[2],
+                                "source": "reports/kaggle-gpu-03/REPORT.md:19-23 (median of three)"}
+            row["speedup_vs_published_t4"] = {k: round(pub[2] / v["seconds"], 2) for k, v in row["gpu"].items()}
+        print(f"# tridiagonal n={n}: " + ", ".join(f"{k} {v['seconds'] * 1e3:.1f} ms ({v['iterations']} it)"
+                                                 for k, v in row["gpu"].items())
+              + f"; cpu {threads} threads {t_all:.3f}s, 1 thread {t_one:.3f}s", file=sys.stderr, flush=True)
+        out.append(row)
+    # roofline of the FP64 reference step at the largest n (HIP events around the step launches)
+    n = sizes[-1]
+    d = hgqp.gen_tridiagonal(n)
+    start, index, value, L = hgqp.csr_assemble(n, d["hrow"], d["hcol"], d["hval"], d["cost"], d["lower"], d["upper"])
+    with hgqp.Box(start, index, value, d["cost"], d["lower"], d["upper"], L, timing=1, use_graph=0) as box:
+        rep = box.solve(tolerance=1e-7, iteration_limit=20000, time_limit=120)
+    us = rep["step_kernel_ms"] / rep["step_launches"] * 1e3
+    nbytes = step_bytes(n, len(value), 8, 8)
+    big = out[-1]
+    head = max((k for k, v in big["gpu"].items() if v["passed"]), key=lambda k: 1 / big["gpu"][k]["seconds"],
+               default="fp64_reference")
+    sec = big["gpu"][head]["seconds"]
+    return {
+        "metric": METRIC, "value": round(1.0 / sec, 3), "unit": "solves/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if MODES[head][0] == hgqp.PRECISION_FP64 else "f32+f64",
+        "data": "synthetic (the reference's seeded tridiagonal family, scale_benchmark.cpp:13-18)",
+        "config": {"workload": f"the reference's scale benchmark: tridiagonal planted box QP, n in {sizes}, "
+                               f"tolerance 1e-7, one solve from the triplet model per step (host arrays in and out); "
+                               f"value = fastest passing mode at n={n}", "mode": head},
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / us / 1e3, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "step_kernel<double> (FP64 reference step, one panel)", "algorithmic_bytes": nbytes,
+                     "avg_launch_us": round(us, 2)},
+        "cpu_baseline": {"value": round(1.0 / big["cpu"]["all_threads"]["seconds"], 4), "unit": "solves/s",
+                         "cores": threads, "kind": kind,
+                         "sample": f"{source}: one full solve per n at the reference's boundary, n={n} shown",
+                         "host": host_cpu()},
+        "sizes": out, "device": hgqp.device_name(),
+    }
+
+
 def gen_rows_qp(n, m_eq, m_ineq, seed, row_nnz=6, free=0.1):
     """Convex QP with linear rows, feasible by construction: a sparse strictly
     diagonally dominant H (lower triplets, ~1.5 off-diagonals per row), m_eq
@@ -546,7 +679,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--check-every", type=int, default=10, help="check cadence of the non-reference modes")
     ap.add_argument("--polish-every", type=int, default=5, help="FP64 check cadence after an FP32 phase")
-    ap.add_argument("--workload", choices=["single", "batch", "serve", "dense", "admm"], default="single")
+    ap.add_argument("--workload", choices=["single", "batch", "serve", "dense", "admm", "tridiagonal"],
+                    default="single")
+    ap.add_argument("--tri-sizes", default="10000,50000,200000,500000,1000000",
+                    help="tridiagonal: the n of the reference's scale benchmark")
     ap.add_argument("--admm-n", type=int, default=100_000, help="admm: variables (rows: n/20 equality, 3n/20 inequality)")
     ap.add_argument("--dense-n", type=int, default=4096)
     ap.add_argument("--dense-k", type=int, default=256)
@@ -584,6 +720,9 @@ def main():
         return
     if args.workload == "admm":
         print(json.dumps(admm_workload(args)))
+        return
+    if args.workload == "tridiagonal":
+        print(json.dumps(tridiagonal_workload(args)))
         return
 
     device = hgqp.device_name()
@@ -680,9 +819,10 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        iters = modes["fp64_reference"]["iterations"] if "fp64_reference" in modes else hm["iterations"]
-        cpu = cpu_baseline(inst, iters, threads)
+        threads = args.cpu_threads or host_cpu()["usable_cpus"]
+        cpu = cpu_baseline(inst, threads)
+        print(f"# cpu baseline: {cpu['sample']}; {cpu.get('single_thread', {}).get('sample')}", file=sys.stderr,
+              flush=True)
 
     line = {
         "metric": METRIC, "value": round(hm["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1,

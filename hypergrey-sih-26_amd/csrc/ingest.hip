@@ -96,6 +96,44 @@ __global__ void merge_runs(int64_t m, const uint64_t* ekey, const double* eval, 
     }
 }
 
+// The diagonal-dominance fast path of check_convex_hessian (qp_admm.cpp:30-39)
+// over the sorted entries: the thread at the head of each row's run walks the
+// row. Its diagonal sum adds the row's diagonal terms one by one in term order
+// (one contiguous run of equal keys), the reference's value bit for bit; its
+// radius adds |v| of the row's off-diagonal entries in column order, where the
+// reference adds them in term order. Both radius sums of k nonnegative terms
+// lie within gamma_{k-1} of the exact sum, so a row whose decision
+// diagonal < radius could depend on the order is marked ambiguous (its
+// verdict then comes from the host pass in term order).
+// Bits: kNotConvex (a negative diagonal, or a zero one with a nonzero
+// radius), kNotDominant (diagonal < radius for every order), kAmbiguous.
+__global__ void row_dominance(int64_t m, const uint64_t* ekey, const double* eval, unsigned* flags) {
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < m; i += int64_t(gridDim.x) * kThreads) {
+        const uint64_t row = ekey[i] >> 32;
+        if (i > 0 && (ekey[i - 1] >> 32) == row) continue;
+        double diag = 0, rad = 0;
+        int64_t k = 0;
+        for (int64_t q = i; q < m && (ekey[q] >> 32) == row; ++q) {
+            if ((ekey[q] & 0xffffffffull) == row) {
+                diag += eval[q];
+            } else {
+                rad += fabs(eval[q]);
+                ++k;
+            }
+        }
+        unsigned bits = 0;
+        if (diag < 0 || (diag == 0 && rad > 0)) {
+            bits = kNotConvex;
+        } else {
+            const double ku = double(k) * 0x1p-53;
+            const double tol = 2.5 * (ku / (1 - ku)) * rad;  // > twice gamma_{k-1} of the exact sum
+            if (diag < rad - tol) bits = kNotDominant;
+            else if (diag < rad + tol) bits = kAmbiguous;
+        }
+        if (bits) atomicOr(flags, bits);
+    }
+}
+
 __global__ void scatter_csr(int64_t m, const uint64_t* ekey, const int* keep, const int* pos, const double* sum,
                             int* col, double* val, int* count) {
     for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < m; i += int64_t(gridDim.x) * kThreads) {
@@ -200,6 +238,9 @@ IngestReport ingest_terms(int64_t n, int64_t nterms, const TermView& terms, doub
     uint64_t *ek = ekey.p, *ek2 = ekey2.p;
     double *ev = eval.p, *ev2 = eval2.p;
     if (m) sort_pairs(ek, ek2, ev, ev2, m, 32 + bits_for(uint64_t(n)), st);
+    if (m)
+        hipLaunchKernelGGL(row_dominance, dim3(grid_of(m)), dim3(kThreads), 0, st, m, ek, ev, flags.p + 1);
+    HIPCHECK(hipGetLastError());
     Buf<int> keep(size_t(m) + 1), pos(size_t(m) + 1);
     Buf<double> sum(size_t(m) + 1);
     hipLaunchKernelGGL(fill<int>, dim3(1), dim3(kThreads), 0, st, int64_t(1), keep.p + m, 0);
@@ -231,6 +272,7 @@ IngestReport ingest_terms(int64_t n, int64_t nterms, const TermView& terms, doub
     unsigned long long lb = 0;
     out.host_start.resize(size_t(n) + 1);
     HIPCHECK(hipMemcpyAsync(&lb, best.p, sizeof lb, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&rep.dominance, flags.p + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(out.host_start.data(), out.start, size_t(n + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     double L;

@@ -30,10 +30,19 @@ struct DeviceCsr {
     ~DeviceCsr();
 };
 
+// Row verdicts of the diagonal-dominance fast path of check_convex_hessian
+// (qp_admm.cpp:30-39), ORed over the rows. None set: every row is diagonally
+// dominant and the Hessian is convex. kNotConvex: some row has a negative
+// diagonal, or a zero diagonal and a nonzero radius (not convex). Otherwise
+// kNotDominant or kAmbiguous (a row whose dominance depends on the order of
+// its radius sum) leave the verdict to the host pass in term order.
+constexpr unsigned kNotConvex = 1u, kNotDominant = 2u, kAmbiguous = 4u;
+
 struct IngestReport {
     bool bad_term = false;     // a term with an index out of [0, n) or a non-finite value
     bool off_nonzero = false;  // some off-diagonal term is nonzero
     double lipschitz = 0;      // max over rows of sum |value|, in CSR order
+    unsigned dominance = 0;    // kNotConvex | kNotDominant | kAmbiguous
 };
 
 // Terms already on the device. Returns the report; `out` holds the CSR unless

@@ -231,6 +231,7 @@ struct PreparedBox {
     std::vector<double> cost;
     double lipschitz = 0;
     bool positive_diagonal = false;
+    double host_check_seconds = 0, assembly_seconds = 0;
 };
 
 struct StreamHandle {
@@ -267,39 +268,38 @@ bool prepare_model(const ModelView& m, const QpOptions& opt, QpResult& r, Prepar
     if (opt.step_metric == HGQP_STEP_JACOBI) r.algorithm += "_jacobi";
     const double sign = m.maximize ? -1 : 1;
     const auto term = [&](std::size_t k, int64_t* rr, int64_t* cc, double* vv) { m.term(k, rr, cc, vv); };
-    if (!convex_terms(m.n, sign, m.nterms, term)) {
-        r.status = "unsupported_curvature";
-        return false;
-    }
-    // positive diagonal: no nonzero off-diagonal term, every diagonal sum > 0
-    bool positive_diagonal = true, all_zero = true;
-    for (std::size_t k = 0; k < m.nterms; ++k) {
-        int64_t rr, cc;
-        double vv;
-        m.term(k, &rr, &cc, &vv);
-        if (vv != 0) all_zero = false;
-        if (rr != cc && vv != 0) positive_diagonal = false;
-        if (!all_zero && !positive_diagonal) break;
-    }
-    // every term zero: L = 0 for certain (accelerated_qp.cpp:28); cancelling
-    // duplicates are found after the assembly below
-    if (all_zero) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
-    if (positive_diagonal) {
-        std::vector<long double> diagonal(m.n, 0);
-        for (std::size_t k = 0; k < m.nterms; ++k) {
+    double host_check = 0;
+    // the exact host pass of check_convex_hessian, in term order
+    auto convex_on_host = [&] {
+        const auto a = std::chrono::steady_clock::now();
+        const bool convex = convex_terms(m.n, sign, m.nterms, term);
+        host_check += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        return convex;
+    };
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess) devices = 0;
+    if (devices == 0) {
+        // no device: the outcomes the host can decide, in the reference's order
+        // (curvature, then the zero Hessian of accelerated_qp.cpp:28), before
+        // the missing device is reported
+        if (!convex_on_host()) {
+            r.status = "unsupported_curvature";
+            return false;
+        }
+        bool all_zero = true;
+        for (std::size_t k = 0; k < m.nterms && all_zero; ++k) {
             int64_t rr, cc;
             double vv;
             m.term(k, &rr, &cc, &vv);
-            if (rr == cc) diagonal[rr] += sign * vv;
+            all_zero = vv == 0;
         }
-        for (long double d : diagonal)
-            if (!(d > 0)) positive_diagonal = false;
+        if (all_zero) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
     }
-    out.positive_diagonal = positive_diagonal;
     out.cost.assign(m.cost, m.cost + m.n);
     for (double& v : out.cost) v *= sign;
-    // CSR and L on the device
+    // CSR, L and the dominance fast path of the convexity test on the device
     require_device();
+    const auto t_assemble = std::chrono::steady_clock::now();
     StreamHandle st;
     TermView tv{};
     std::unique_ptr<DeviceBytes> buf;
@@ -320,9 +320,35 @@ bool prepare_model(const ModelView& m, const QpOptions& opt, QpResult& r, Prepar
         tv = TermView{base, base + m.nterms, reinterpret_cast<const double*>(base + 2 * m.nterms), 1};
     }
     const IngestReport rep = ingest_terms(int64_t(m.n), int64_t(m.nterms), tv, sign, out.csr, st.s);
+    out.assembly_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_assemble).count();
     if (rep.bad_term) throw std::invalid_argument("Invalid Hessian");
+    // check_convex_hessian (qp_admm.cpp:30-62): decided by the device when every
+    // row is dominant or some row rules convexity out; a matrix that is not
+    // dominant (the LDL^T test) or a row too close to call goes to the host pass
+    bool convex = !(rep.dominance & kNotConvex);
+    if (convex && (rep.dominance & (kNotDominant | kAmbiguous))) convex = convex_on_host();
+    out.host_check_seconds = host_check;
+    if (!convex) {
+        r.status = "unsupported_curvature";
+        return false;
+    }
     if (rep.lipschitz == 0) throw UnsupportedModel("Zero-Hessian models use the CPU LP path");
     out.lipschitz = rep.lipschitz;
+    // positive diagonal (the separable dual bound of qp_admm.cpp:106-112): no
+    // nonzero off-diagonal term and every diagonal sum > 0 (terms checked above)
+    bool positive_diagonal = !rep.off_nonzero;
+    if (positive_diagonal) {
+        std::vector<long double> diagonal(m.n, 0);
+        for (std::size_t k = 0; k < m.nterms; ++k) {
+            int64_t rr, cc;
+            double vv;
+            m.term(k, &rr, &cc, &vv);
+            if (rr == cc) diagonal[rr] += sign * vv;
+        }
+        for (long double d : diagonal)
+            if (!(d > 0)) positive_diagonal = false;
+    }
+    out.positive_diagonal = positive_diagonal;
     for (std::size_t j = 0; j < m.n; ++j)
         if (m.lower[j] > m.upper[j]) {
             r.status = "infeasible_bounds";
@@ -337,6 +363,8 @@ QpResult solve_model(const ModelView& m, const QpOptions& opt, std::chrono::stea
     QpResult r;
     PreparedBox pb;
     if (!prepare_model(m, opt, r, pb)) return r;
+    r.host_check_seconds = pb.host_check_seconds;
+    r.assembly_seconds = pb.assembly_seconds;
     hgqp_box_options bo;
     hgqp_box_options_default(&bo);
     bo.precision = opt.precision;
@@ -519,6 +547,8 @@ extern "C" int hgqp_solve_qp(const hgqp_problem* problem, const hgqp_qp_options*
         result->optimality_verified = r.optimality_verified ? 1 : 0;
         result->restarts = r.restarts;
         result->fp32_iterations = r.fp32_iterations;
+        result->host_check_seconds = r.host_check_seconds;
+        result->assembly_seconds = r.assembly_seconds;
         result->solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (primal_out && r.primal.size() == m.n) std::memcpy(primal_out, r.primal.data(), m.n * sizeof(double));
     });

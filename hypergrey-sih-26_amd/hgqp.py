@@ -104,7 +104,8 @@ class QpResult(C.Structure):
                 ("primal_residual", C.c_double), ("stationarity_residual", C.c_double),
                 ("complementarity_residual", C.c_double), ("dual_bound", C.c_double), ("relative_gap", C.c_double),
                 ("iterations", C.c_uint64), ("optimality_verified", C.c_int32), ("restarts", C.c_int32),
-                ("fp32_iterations", C.c_uint64), ("solve_seconds", C.c_double), ("inner_iterations", C.c_uint64)]
+                ("fp32_iterations", C.c_uint64), ("solve_seconds", C.c_double), ("inner_iterations", C.c_uint64),
+                ("host_check_seconds", C.c_double), ("assembly_seconds", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

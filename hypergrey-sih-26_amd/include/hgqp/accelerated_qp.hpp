@@ -62,6 +62,8 @@ struct QpResult {
     std::size_t fp32_iterations = 0;
     int restarts = 0;
     std::size_t inner_iterations = 0;  // ADMM: PCG iterations of the solve
+    double host_check_seconds = 0;     // host passes over the terms (convexity, diagonal scan)
+    double assembly_seconds = 0;       // term upload and device CSR assembly
 };
 
 // include/qp.hpp:24-29 plus the backend's iteration arithmetic.

@@ -266,6 +266,10 @@ typedef struct hgqp_qp_result {
     uint64_t fp32_iterations;
     double solve_seconds;
     uint64_t inner_iterations;  /* hgqp_solve_qp_admm: PCG iterations of the solve  */
+    double host_check_seconds;  /* hgqp_solve_qp: the host passes over the terms
+                                   (convexity gate, positive-diagonal scan)        */
+    double assembly_seconds;    /* hgqp_solve_qp: term upload and device assembly
+                                   of the mirrored CSR and L                       */
 } hgqp_qp_result;
 
 /* Replaces solve_accelerated_box(q, options, cuda=true)

@@ -182,3 +182,67 @@ def test_resolve_at_another_tolerance(hg, oracle, mode):
                 assert rep["iterations"] == expect[tol][0]["iterations"]
                 assert np.array_equal(x, expect[tol][1])
     assert expect[1e-9][0]["iterations"] > expect[1e-6][0]["iterations"]
+
+
+def convexity_corpus(seed):
+    """Small models for the convexity gate: dominant, dominant up to an exact tie
+    whose radius rounds differently by order, one ulp short of that tie, negative
+    and zero diagonals, PSD but not dominant (LDL^T), indefinite, maximization."""
+    rng = np.random.default_rng(seed)
+    out = []
+    vals = [0.1, 0.2, 0.3, 0.7, 1e-17, 3.3]
+    for case in range(40):
+        n = int(rng.integers(3, 9))
+        hr, hc, hv = [], [], []
+        for i in range(1, n):
+            for j in range(i):
+                if rng.random() < 0.6:
+                    hr.append(i); hc.append(j); hv.append(float(rng.choice(vals)) * float(rng.choice([-1, 1])))
+        radius = np.zeros(n)
+        for i, j, v in zip(hr, hc, hv):  # term order, as the reference adds them
+            radius[i] += abs(v)
+            radius[j] += abs(v)
+        kind = case % 8
+        diag = radius.copy()                                   # an exact tie in term order
+        if kind == 1:
+            diag = radius + rng.uniform(0.1, 1.0, n)           # dominant
+        elif kind == 2:
+            diag = np.nextafter(radius, -np.inf)               # one ulp short: LDL^T decides
+        elif kind == 3:
+            diag[0] = -0.5                                     # negative diagonal
+        elif kind == 4:
+            diag[int(rng.integers(0, n))] = 0.0                # zero diagonal
+        elif kind == 5:
+            diag = 0.6 * radius + 1e-3                         # not dominant, maybe PSD
+        elif kind == 6:
+            diag = 0.2 * radius                                # likely indefinite
+        maximize = kind == 7
+        sign = -1.0 if maximize else 1.0
+        d = list(range(n))
+        q = BoxQP(n, hr + d, hc + d, [sign * v for v in hv] + [sign * x for x in diag],
+                  rng.normal(0, 1, n), -np.ones(n), np.ones(n), maximize=maximize)
+        out.append((case, q))
+    return out
+
+
+def test_device_convexity_gate_equals_reference(hg, reference):
+    """hgqp_solve_qp decides curvature with the device's dominance pass (exact
+    diagonal sums, order-robust radius bounds) and the host LDL^T pass where the
+    device cannot decide; the outcome equals the compiled reference's
+    check_convex_hessian on every model of the corpus."""
+    seen = set()
+    for seed in (1, 2, 3):
+        for case, q in convexity_corpus(seed):
+            expect = reference.check_convex(q)
+            try:
+                got = hg.solve_qp(q.n, q.hrow, q.hcol, q.hval, q.cost, q.lower, q.upper, maximize=q.maximize,
+                                  iteration_limit=0, time_limit=10)["status"]
+            except hg.Unsupported:  # a zero Hessian: rejected after the gate passed it (accelerated_qp.cpp:28)
+                assert not np.any(q.hval), (seed, case)
+                got = "zero_hessian"
+            assert (got != "unsupported_curvature") == expect, (seed, case, got, expect)
+            seen.add((case % 8, expect))
+    # exact ties accepted, dominant accepted, negative diagonals rejected, and the
+    # one-ulp-short rows decided by the LDL^T pass one way or the other
+    assert {(0, True), (1, True), (3, False)} <= seen
+    assert (2, True) in seen or (2, False) in seen
