@@ -220,8 +220,14 @@ __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int
     StepEpilogue<float> epi = slab_epilogue(bufs, parity, use_x, beta);
     const int r0 = rb[blockIdx.x], rows = rb[blockIdx.x + 1] - r0;
     const int last = tc[size_t(blocks) * slabs] - 1;
-    slab_rows<kSlabWidth, LONG>(slabs, rows, last, GlobalBounds{tc, size_t(blockIdx.x) * slabs}, row, col, val,
-                                epi.gat, ws0, ws1, ws2, acc);
+    if (slabs + 1 <= LaneBounds<kBoundRegs>::kCapacity) {  // workgroup-uniform
+        LaneBounds<kBoundRegs> tb;
+        tb.load(tc, size_t(blockIdx.x) * slabs, slabs + 1);
+        slab_rows<kSlabWidth, LONG>(slabs, rows, last, tb, row, col, val, epi.gat, ws0, ws1, ws2, acc);
+    } else {
+        slab_rows<kSlabWidth, LONG>(slabs, rows, last, GlobalBounds{tc, size_t(blockIdx.x) * slabs}, row, col, val,
+                                    epi.gat, ws0, ws1, ws2, acc);
+    }
     for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(r0 + r, acc[r]);
     // ws0 is free: every thread is past slab_rows' last barrier
     slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, reinterpret_cast<double*>(ws0));
@@ -233,8 +239,12 @@ __global__ __launch_bounds__(kSlabThreads) void slab_step_kernel(int blocks, int
 // the XCDs in turn, so the G ranges run on different XCDs and each XCD's L2
 // serves a G-th of the vector. `follow` = 1: the step's source point and early
 // exit come from ctrl.
+// Shape of the split sum launch: threads, stager waves, tiles of chunk
+// registers per summing thread.
+constexpr int kSumLaunch = 1024, kSumStagers = 4, kSumAhead = 3;
+
 template <int G, bool LONG>
-__global__ __launch_bounds__(kSlabThreads) void slab_sum_kernel(int slabs, const int* __restrict__ rb,
+__global__ __launch_bounds__(kSumLaunch) void slab_sum_kernel(int slabs, const int* __restrict__ rb,
                                                                 const int* __restrict__ tc,
                                                                 const uint4* __restrict__ row,
                                                                 const uint4* __restrict__ col,
@@ -257,10 +267,19 @@ __global__ __launch_bounds__(kSlabThreads) void slab_sum_kernel(int slabs, const
     const int s0 = slab_range_start(h, slabs, G), P = slab_range_start(h + 1, slabs, G) - s0;
     const int r0 = rb[b], rows = rb[b + 1] - r0;
     const int last = tc[size_t(gridDim.x / G) * slabs] - 1;
-    slab_rows<W, LONG>(P, rows, last, GlobalBounds{tc, size_t(b) * slabs + s0}, row, col, val,
-                       gat + size_t(s0) * W, ws0, ws1, ws2, acc);
+    if (P + 1 <= LaneBounds<kBoundRegs>::kCapacity) {  // workgroup-uniform
+        LaneBounds<kBoundRegs> tb;
+        tb.load(tc, size_t(b) * slabs + s0, P + 1);
+        slab_rows_split<kSumLaunch, kSumStagers, kSumAhead, W, LONG>(P, rows, last, tb, row, col, val,
+                                                                     gat + size_t(s0) * W, ws0, ws1, ws2, acc);
+    } else {
+        slab_rows_split<kSumLaunch, kSumStagers, kSumAhead, W, LONG>(P, rows, last,
+                                                                     GlobalBounds{tc, size_t(b) * slabs + s0}, row,
+                                                                     col, val, gat + size_t(s0) * W, ws0, ws1, ws2,
+                                                                     acc);
+    }
     float* out = part + h * stride + r0;
-    for (int r = threadIdx.x; r < rows; r += kSlabThreads) out[r] = acc[r];
+    for (int r = threadIdx.x; r < rows; r += kSumLaunch) out[r] = acc[r];
 }
 
 constexpr int kFinal = 1024;  // threads of a finalize launch
@@ -705,7 +724,7 @@ void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control*
         const int follow = momentum != kMomentumImmediate;
         const dim3 sum(m.groups * m.blocks);
 #define HGQP_SLAB_SUM(G, LONG)                                                                                   \
-    hipLaunchKernelGGL((slab_sum_kernel<G, LONG>), sum, block, 0, stream, m.slabs, m.rb, m.tc, m.row, m.col, m.val, \
+    hipLaunchKernelGGL((slab_sum_kernel<G, LONG>), sum, dim3(kSumLaunch), 0, stream, m.slabs, m.rb, m.tc, m.row, m.col, m.val, \
                        b.x[parity], b.w[parity], ctrl, parity, follow, m.part, m.part_stride)
         if (m.groups == 4) {
             if (m.long_runs) HGQP_SLAB_SUM(4, true);

@@ -600,13 +600,18 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         }
     }
     const bool own32 = f32 && p32 != p64;
-    // Two workgroups per row block (a split layout) when the reduction order is
-    // free and each gets at least two slabs; four only when asked for (on the
-    // 1e6-row planted instance the step took 92 us with two, 95 us with four,
-    // 106 us with one).
+    // A split layout when the reduction order is free: four workgroups per row
+    // block when each gets at least two slabs, else two on the same terms (on
+    // the 1e6-row planted instance the sum and finish launches of a step took
+    // 73 us with four, 78 us with two, and the one-group step 102 us).
     int slab_groups = opt.slab_groups;
-    if (slab_groups == 0)
-        slab_groups = opt.reduce != HGQP_REDUCE_ORDERED && slab_padded(n, slab_width(2)) / slab_width(2) >= 4 ? 2 : 1;
+    if (slab_groups == 0) {
+        slab_groups = 1;
+        if (opt.reduce != HGQP_REDUCE_ORDERED) {
+            if (slab_padded(n, slab_width(4)) / slab_width(4) >= 8) slab_groups = 4;
+            else if (slab_padded(n, slab_width(2)) / slab_width(2) >= 4) slab_groups = 2;
+        }
+    }
     // the FP32 iterates hold whole slabs when the steps may read them slab by slab
     const size_t un32 = use_slabs ? static_cast<size_t>(slab_padded(n, slab_width(slab_groups))) : un;
 

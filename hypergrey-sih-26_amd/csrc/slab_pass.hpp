@@ -201,6 +201,40 @@ struct LdsBounds {
     const int* tcs;
     __device__ int operator()(int s) const { return tcs[s]; }
 };
+// Up to 64 R tile bounds held in registers, loaded once: lane l of v[i] holds
+// bound 64 i + l, read back with a wave-uniform lane index. A bound read from
+// memory inside the tile loop is a scalar load whose wait also drains the
+// wave's LDS operations; under the chunk stream that load has cost more than
+// the tile's sums.
+template <int R>
+struct LaneBounds {
+    static constexpr int kCapacity = R * kWave;
+    int v[R];
+    // count: bounds to load (at most kCapacity); lanes past it repeat the last.
+    __device__ void load(const int* __restrict__ tc, size_t base, int count) {
+        const int lane = threadIdx.x % kWave;
+#pragma unroll
+        for (int i = 0; i < R; ++i) v[i] = tc[base + min(i * kWave + lane, count - 1)];
+    }
+    __device__ int operator()(int s) const {
+        int r = __builtin_amdgcn_readlane(v[0], s % kWave);
+#pragma unroll
+        for (int i = 1; i < R; ++i)
+            if (s / kWave == i) r = __builtin_amdgcn_readlane(v[i], s % kWave);
+        return r;
+    }
+};
+constexpr int kBoundRegs = 2;  // tile bounds a box step keeps in registers: 64 kBoundRegs
+
+// Chunk thread i loads into its registers for tile t: chunk tb(t) + i when the
+// tile has that many, else the tile's last chunk (chunk 0 for an empty tile at
+// the start), so the lanes with no chunk of their own share one request rather
+// than fetch the next tile's chunks ahead of time.
+template <class Bounds>
+__device__ __forceinline__ int tile_chunk(const Bounds& tb, int t, int i, int last) {
+    const int k0 = tb(t), k1 = tb(t + 1);
+    return k0 + i < k1 ? k0 + i : min(max(k1 - 1, 0), last);
+}
 
 // Tile s: waits for slab s and tile s's chunk registers `q`, issues slab
 // s + 2's copies into `ahead_buf`, sums tile s from `cur_buf`, then issues
@@ -241,7 +275,7 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
         }
     }
     __asm__ volatile("" ::: "memory");
-    load_chunk(q, row, col, val, min(tb(min(s + 3, P - 1)) + tid, last));
+    load_chunk(q, row, col, val, tile_chunk(tb, min(s + 3, P - 1), tid, last));
     __asm__ volatile("" ::: "memory");
 }
 
@@ -260,14 +294,14 @@ __device__ __forceinline__ void slab_rows(int P, int R, int last, const Bounds& 
     SlabChunk q0, q1, q2;
     const int s1 = min(1, P - 1), s2 = min(2, P - 1);
     // the order of a tile's operations: copies of slab 0, chunks of tile 0, ...
-    load_chunk(q0, row, col, val, min(tb(0) + tid, last));
+    load_chunk(q0, row, col, val, tile_chunk(tb, 0, tid, last));
     stage_slab<W>(gat, ws0);
     __asm__ volatile("" ::: "memory");
-    load_chunk(q1, row, col, val, min(tb(s1) + tid, last));
+    load_chunk(q1, row, col, val, tile_chunk(tb, s1, tid, last));
     __asm__ volatile("" ::: "memory");
     stage_slab<W>(gat + size_t(s1) * W, ws1);
     __asm__ volatile("" ::: "memory");
-    load_chunk(q2, row, col, val, min(tb(s2) + tid, last));
+    load_chunk(q2, row, col, val, tile_chunk(tb, s2, tid, last));
     // three tiles per trip with fixed buffer roles; the one or two left follow
     int s = 0;
     for (; s + 3 <= P; s += 3) {
@@ -277,6 +311,142 @@ __device__ __forceinline__ void slab_rows(int P, int R, int last, const Bounds& 
     }
     if (s < P) slab_tile<W, LONG, true>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
     if (s + 1 < P) slab_tile<W, LONG, true>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
+    barrier_vm<0>();
+}
+
+// ---- the pipeline with split roles ---------------------------------------------
+// Stager waves only copy slabs into the LDS ring; the other waves only load
+// and sum chunks. Each role then waits on its own vector-memory count: a
+// stager for slab s (slab s + 1 may stay in flight), a summing wave for tile
+// s's chunks with the later tiles' chunk loads still in flight. With one
+// counter for both streams (slab_rows) the wait for slab s also waits for
+// every chunk load issued before its copies.
+//
+// Shape: NT threads per workgroup, S stager waves, K tiles of chunk registers
+// per summing thread (a multiple of the 3-buffer ring).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt holds 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | 0x70 | (((N >> 4) & 3) << 14));
+}
+
+// Copies of 1 KiB each of S stager waves issues per slab W columns wide.
+template <int W, int S>
+constexpr int stage_copies() {
+    static_assert(W * 4 % (1024 * S) == 0, "a slab splits into whole 1 KiB copies per stager wave");
+    return W * 4 / 1024 / S;
+}
+
+// Stager wave sw copies its stage_copies<W, S>() KiB of a slab, lane l's 16
+// bytes at +16 l of each KiB.
+template <int W, int S>
+__device__ __forceinline__ void stage_slab_wave(const float* src, float* dst, int sw, int lane) {
+    constexpr int kCopies = stage_copies<W, S>();
+#pragma unroll
+    for (int i = 0; i < kCopies; ++i) {
+        const int off = (sw * kCopies + i) * 1024;
+        __builtin_amdgcn_global_load_lds((GlobalPtr)(reinterpret_cast<const char*>(src) + off + lane * 16),
+                                         (LdsPtr)(reinterpret_cast<char*>(dst) + off), 16, 0, 0);
+    }
+}
+
+// The stager's side of tile s: wait for slab s's copies (slab s + 1's may stay
+// in flight), pass the barrier, then copy slab s + 2 into ahead_buf (clamped to
+// the last slab: a repeated copy lands in a buffer no later tile reads). ALL:
+// wait for everything (the tiles after the unrolled loop).
+template <int W, int S, bool ALL>
+__device__ __forceinline__ void stager_tile(int P, int s, const float* __restrict__ gat, float* ahead_buf, int sw,
+                                            int lane) {
+    if (ALL) wait_vm<0>();
+    else wait_vm<stage_copies<W, S>()>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    stage_slab_wave<W, S>(gat + size_t(min(s + 2, P - 1)) * W, ahead_buf, sw, lane);
+}
+
+// A summing wave's side of tile s: wait for tile s's chunk registers q (the
+// later tiles' chunk loads stay in flight), pass the barrier, add tile s's
+// chunks from cur_buf (chunk k0 + ctid, then every SUM-th after it), then load
+// tile s + K's chunks into q (clamped to the last tile). ALL: wait for
+// everything and load nothing more.
+template <int K, int SUM, bool LONG, bool ALL, class Bounds>
+__device__ __forceinline__ void summer_tile(int P, const Bounds& tb, const uint4* __restrict__ row,
+                                            const uint4* __restrict__ col, const float* __restrict__ val, int s,
+                                            int last, const float* cur_buf, float* acc, SlabChunk& q, int ctid) {
+    if (ALL) wait_vm<0>();
+    else wait_vm<kChunkLoads*(K - 1)>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    const int k0 = tb(s), k1 = tb(s + 1);
+    if (k0 + ctid < k1) sum_chunk<LONG>(q, cur_buf, acc, row, col, val, k0 + ctid, last);
+    int ch = k0 + ctid + SUM;
+    if (ch < k1) {
+        SlabChunk nx;
+        load_chunk(nx, row, col, val, ch);
+        for (;;) {
+            const SlabChunk here = nx;
+            const int nch = ch + SUM;
+            if (nch < k1) load_chunk(nx, row, col, val, nch);
+            sum_chunk<LONG>(here, cur_buf, acc, row, col, val, ch, last);
+            if (nch >= k1) break;
+            ch = nch;
+        }
+    }
+    if (!ALL) {
+        __asm__ volatile("" ::: "memory");
+        load_chunk(q, row, col, val, tile_chunk(tb, min(s + K, P - 1), ctid, last));
+        __asm__ volatile("" ::: "memory");
+    }
+}
+
+// slab_rows with split roles (same contract) for NT threads: waves 0 .. S - 1
+// stage, the others sum. The role is wave-uniform (a scalar branch), and each
+// role runs its own loop with one barrier per tile, so the barriers pair up
+// across the roles and each loop's waits count only its own stream.
+template <int NT, int S, int K, int W, bool LONG, class Bounds>
+__device__ __forceinline__ void slab_rows_split(int P, int R, int last, const Bounds& tb,
+                                                const uint4* __restrict__ row, const uint4* __restrict__ col,
+                                                const float* __restrict__ val, const float* __restrict__ gat,
+                                                float* ws0, float* ws1, float* ws2, float* acc) {
+    static_assert(K % 3 == 0, "chunk registers rotate with the three slab buffers");
+    constexpr int SUM = NT - S * kWave;
+    const int tid = threadIdx.x;
+    for (int r = tid; r < R; r += NT) acc[r] = 0.0f;
+    float* const bufs[3] = {ws0, ws1, ws2};
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    if (wave < S) {
+        const int lane = tid % kWave;
+        stage_slab_wave<W, S>(gat, ws0, wave, lane);
+        stage_slab_wave<W, S>(gat + size_t(min(1, P - 1)) * W, ws1, wave, lane);
+        int s = 0;
+        for (; s + 3 <= P; s += 3) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) stager_tile<W, S, false>(P, s + j, gat, bufs[(j + 2) % 3], wave, lane);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (s + j < P) stager_tile<W, S, true>(P, s + j, gat, bufs[(j + 2) % 3], wave, lane);
+    } else {
+        const int ctid = tid - S * kWave;
+        SlabChunk q[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            load_chunk(q[j], row, col, val, tile_chunk(tb, min(j, P - 1), ctid, last));
+            __asm__ volatile("" ::: "memory");
+        }
+        int s = 0;
+        for (; s + K <= P; s += K) {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                summer_tile<K, SUM, LONG, false>(P, tb, row, col, val, s + j, last, bufs[j % 3], acc, q[j], ctid);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (s + j < P)
+                summer_tile<K, SUM, LONG, true>(P, tb, row, col, val, s + j, last, bufs[j % 3], acc, q[j], ctid);
+    }
     barrier_vm<0>();
 }
 

@@ -138,7 +138,7 @@ def test_tiles_wider_than_a_launch(hg, oracle):
 def test_full_size_slabs(hg):
     """Planted n = 1e6, 16 draws: the handle picks slabs for its FP32 steps by
     itself (and keeps 4 / 2 column panels for the FP64 passes and FP32 checks):
-    under the TREE reduction the split layout (98 slabs in two groups), whose
+    under the TREE reduction the split layout (123 slabs in four groups), whose
     solve is verified and recovers x*; under ORDERED one group of 82 slabs,
     whose solve equals the one without slabs bit for bit."""
     n = 1000000
@@ -148,7 +148,7 @@ def test_full_size_slabs(hg):
     opt = modes(hg)["refine_restart_jacobi"]
     print("case: n=1e6 headline solve with slabs", flush=True)
     on = solve(hg, data, reduce=hg.REDUCE_TREE, **opt)
-    assert on[0]["slabs32"] == 98 and on[0]["slab_groups"] == 2
+    assert on[0]["slabs32"] == 123 and on[0]["slab_groups"] == 4
     assert on[0]["column_panels"] == 4 and on[0]["column_panels32"] == 2
     print(on[1], flush=True)
     assert on[1]["status"] == "optimal" and on[1]["optimality_verified"]
@@ -253,12 +253,12 @@ def test_split_equals_one_group_without_straddling_rows(hg, oracle, mode, groups
     assert_same_solve(split, one)
 
 
-@pytest.mark.parametrize("n,groups", [(200000, 0), (61440 + 777, 0), (200000, 4)])
-def test_split_solve_meets_contract(hg, oracle, n, groups):
-    """Planted instances whose rows reach every range: the split solve (two
-    groups by the automatic choice, or four) is verified on the device and by
-    the oracle, lands on the one-group optimum and x*, and repeats bit for bit
-    (a fixed summation order)."""
+@pytest.mark.parametrize("n,groups,expect", [(200000, 0, 4), (40960 + 777, 0, 2), (200000, 2, 2)])
+def test_split_solve_meets_contract(hg, oracle, n, groups, expect):
+    """Planted instances whose rows reach every range: the split solve (four
+    groups or two by the automatic choice, or two asked for) is verified on the
+    device and by the oracle, lands on the one-group optimum and x*, and repeats
+    bit for bit (a fixed summation order)."""
     q, d = planted_qp(hg, n, 8, 21)
     data = engine_inputs(oracle, q)
     opt = dict(modes(hg)["refine_restart_jacobi"], reduce=hg.REDUCE_TREE)
@@ -266,8 +266,8 @@ def test_split_solve_meets_contract(hg, oracle, n, groups):
     split = solve(hg, data, slabs=2, slab_groups=groups, **opt)
     again = solve(hg, data, slabs=2, slab_groups=groups, **opt)
     one = solve(hg, data, slabs=2, slab_groups=1, **opt)
-    width = 8192 if groups == 4 else SPLIT_WIDTH
-    assert split[0]["slab_groups"] == (groups or 2) and split[0]["slabs32"] == -(-n // width)
+    width = 8192 if expect == 4 else SPLIT_WIDTH
+    assert split[0]["slab_groups"] == expect and split[0]["slabs32"] == -(-n // width)
     assert one[0]["slab_groups"] == 1
     print(split[1], flush=True)
     for r in (split, one):
@@ -280,16 +280,19 @@ def test_split_solve_meets_contract(hg, oracle, n, groups):
 
 
 def test_split_choice(hg, oracle):
-    """The automatic choice: two groups under TREE once there are 4 slabs of the
-    split width, one under ORDERED or below that size; two groups asked for on a
-    matrix of one split slab leave the CSR passes in place."""
+    """The automatic choice under TREE: four groups once there are 8 slabs of
+    8192 columns, else two once there are 4 of the two-group width, else one;
+    one under ORDERED at any size. Two groups asked for on a matrix of one split
+    slab leave the CSR passes in place."""
     q = planted_qp(hg, 30000, 1, 3)[0]  # 3 split slabs
     data = engine_inputs(oracle, q)
     fast = modes(hg)["refine_restart_jacobi"]
     assert solve(hg, data, slabs=2, reduce=hg.REDUCE_TREE, **fast)[0]["slab_groups"] == 1
-    big = engine_inputs(oracle, planted_qp(hg, 4 * SPLIT_WIDTH, 1, 3)[0])
+    big = engine_inputs(oracle, planted_qp(hg, 4 * SPLIT_WIDTH, 1, 3)[0])  # 5 slabs of 8192
     assert solve(hg, big, slabs=2, reduce=hg.REDUCE_TREE, **fast)[0]["slab_groups"] == 2
     assert solve(hg, big, slabs=2, reduce=hg.REDUCE_ORDERED, **fast)[0]["slab_groups"] == 1
+    bigger = engine_inputs(oracle, planted_qp(hg, 7 * 8192 + 1, 1, 3)[0])  # 8 slabs of 8192
+    assert solve(hg, bigger, slabs=2, reduce=hg.REDUCE_TREE, **fast)[0]["slab_groups"] == 4
     small = engine_inputs(oracle, planted_qp(hg, 9000, 4, 3)[0])
     info, rep, _ = solve(hg, small, slabs=2, slab_groups=2, reduce=hg.REDUCE_TREE, **fast)
     assert info["slabs32"] == 0 and info["slab_groups"] == 0 and rep["optimality_verified"]
