@@ -27,6 +27,11 @@ struct DeviceCsr {
     DeviceCsr() = default;
     DeviceCsr(const DeviceCsr&) = delete;
     DeviceCsr& operator=(const DeviceCsr&) = delete;
+    DeviceCsr(DeviceCsr&& o) noexcept
+        : n(o.n), nnz(o.nnz), start(o.start), col(o.col), val(o.val), host_start(std::move(o.host_start)) {
+        o.start = o.col = nullptr;
+        o.val = nullptr;
+    }
     ~DeviceCsr();
 };
 

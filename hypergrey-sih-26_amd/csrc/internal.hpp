@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <exception>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -94,5 +95,15 @@ struct CsrSource {
 // derived from the matrix (used when options->diagonal_hessian is -1).
 hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double* cost, const double* lower,
                      const double* upper, double lipschitz, const hgqp_box_options* options, bool diag);
+
+struct DeviceCsr;
+// hgqp_server_submit for an instance whose CSR the device ingest assembled:
+// the request takes csr over (row starts from csr->host_start), and the
+// serving loop copies its columns and values device to device. diag is the
+// positive-diagonal flag of the model's terms. Throws Error.
+void server_submit_device(hgqp_server* server, std::unique_ptr<DeviceCsr> csr, const double* cost,
+                          const double* lower, const double* upper, double lipschitz, int32_t maximize,
+                          double offset, double tolerance, uint64_t iteration_limit, int64_t tag, double* x_out,
+                          bool diag);
 
 }  // namespace hgqp

@@ -2,6 +2,7 @@
 // advanced together by graph-replayed check periods (pool_kernels.hip).
 #include "device_util.hpp"
 #include "hgqp.h"
+#include "ingest.hpp"
 #include "internal.hpp"
 #include "pool_kernels.hpp"
 #include "slabs.hpp"
@@ -529,7 +530,9 @@ PoolInst slot_entry(const SlotHost& h, double lipschitz, double tol, uint64_t li
 
 // Device part of an admission, enqueued on the pool stream: the uploads, the
 // move to pool numbering, the FP32 values and the slot entry. All host memory
-// named here must stay valid until the stream passes this point.
+// named here must stay valid until the stream passes this point. index and
+// value may also be device memory (a server request assembled on the device):
+// their copies take the direction from the pointers.
 void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, const double* value, const double* cost,
             const double* lower, const double* upper, const PoolInst* entry) {
     const SlotHost& h = p->slots[s];
@@ -538,8 +541,8 @@ void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, con
     int* dcol = const_cast<int*>(p->v.col) + h.ent0;
     HIPCHECK(hipMemcpyAsync(dstart, start, (h.n + 1) * 4, hipMemcpyHostToDevice, st));
     if (h.nnz) {
-        HIPCHECK(hipMemcpyAsync(dcol, index, h.nnz * 4, hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.val) + h.ent0, value, h.nnz * 8, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(dcol, index, h.nnz * 4, hipMemcpyDefault, st));
+        HIPCHECK(hipMemcpyAsync(const_cast<double*>(p->v.val) + h.ent0, value, h.nnz * 8, hipMemcpyDefault, st));
         launch_convert<float, double>(h.nnz, p->v.val + h.ent0, const_cast<float*>(p->v.val32) + h.ent0, st);
     }
     if (p->opt.slabs) {
@@ -730,7 +733,9 @@ struct hgqp_server {
 
     // A submitted instance: its arrays in one host block (pinned staging, or
     // pageable when the staging region is full), laid out as start, index,
-    // block table, value, cost, lower, upper, slot entry.
+    // block table, value, cost, lower, upper, slot entry. A request assembled
+    // on the device keeps its columns and values there (dev) and its block
+    // leaves out index and value.
     struct Request {
         int64_t n = 0, nnz = 0;
         double lipschitz = 0, offset = 0, tol = 0;
@@ -748,20 +753,24 @@ struct hgqp_server {
         double* x_stage = nullptr;  // pinned x read-back while resident (nullptr: straight to x_out)
         int slot = -1;
         std::chrono::steady_clock::time_point submitted, admitted;
+        std::unique_ptr<DeviceCsr> dev;  // device-resident columns and values, or null
 
-        static size_t bytes_for(int64_t n, int64_t nnz) {
-            return align16((n + 1) * 4) + align16(nnz * 4) + align16(3 * (n + 1) * 4) + align16(nnz * 8) +
+        // entries: room for index and value in the block (host sources)
+        static size_t bytes_for(int64_t n, int64_t nnz, bool entries = true) {
+            const int64_t e = entries ? nnz : 0;
+            return align16((n + 1) * 4) + align16(e * 4) + align16(3 * (n + 1) * 4) + align16(e * 8) +
                    3 * align16(n * 8) + align16(sizeof(PoolInst));
         }
-        void carve(char* p) {
+        void carve(char* p, bool entries = true) {
+            const int64_t e = entries ? nnz : 0;
             start = reinterpret_cast<int32_t*>(p);
             p += align16((n + 1) * 4);
-            index = reinterpret_cast<int32_t*>(p);
-            p += align16(nnz * 4);
+            index = entries ? reinterpret_cast<int32_t*>(p) : nullptr;
+            p += align16(e * 4);
             blk = reinterpret_cast<int32_t*>(p);
             p += align16(3 * (n + 1) * 4);
-            value = reinterpret_cast<double*>(p);
-            p += align16(nnz * 8);
+            value = entries ? reinterpret_cast<double*>(p) : nullptr;
+            p += align16(e * 8);
             cost = reinterpret_cast<double*>(p);
             p += align16(n * 8);
             lower = reinterpret_cast<double*>(p);
@@ -775,6 +784,9 @@ struct hgqp_server {
     std::map<int, std::unique_ptr<Request>> resident;     // by slot (touched by the loop thread only)
     std::deque<hgqp_served> done;
     int64_t outstanding = 0;                              // submitted, not polled
+    // device CSRs of finished requests: freed by the next submission, on the
+    // caller's thread (hipFree waits for the device), never by the loop
+    std::vector<std::unique_ptr<DeviceCsr>> retired_dev;
 
     void give_block(Request& r) {
         staging.give(r.block, r.block_bytes);
@@ -837,9 +849,11 @@ struct hgqp_server {
         HIPCHECK(hipStreamWaitEvent(pool->stream, copies_done, 0));
         const auto now = std::chrono::steady_clock::now();
         std::vector<hgqp_served> out;
+        std::vector<std::unique_ptr<DeviceCsr>> dead;
         for (int s : fin) {
             std::unique_ptr<Request> r = std::move(resident[s]);
             resident.erase(s);
+            if (r->dev) dead.push_back(std::move(r->dev));
             if (r->x_stage) std::memcpy(r->x_out, r->x_stage, size_t(r->n) * 8);
             staging.give(reinterpret_cast<char*>(r->x_stage), size_t(r->n) * 8);
             hgqp_served sv;
@@ -851,6 +865,27 @@ struct hgqp_server {
         }
         std::lock_guard<std::mutex> lk(mu);
         for (const auto& sv : out) done.push_back(sv);
+        for (auto& d : dead) retired_dev.push_back(std::move(d));
+    }
+
+    // Queues a filled request for the loop (the caller's thread).
+    void enqueue(std::unique_ptr<Request> r) {
+        std::vector<std::unique_ptr<DeviceCsr>> dead;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            dead.swap(retired_dev);
+            if (!error.empty()) {
+                give_block(*r);
+                throw Error(HGQP_DEVICE_ERROR, "hgqp_server: " + error);
+            }
+            if (stopping) {
+                give_block(*r);
+                throw Error(HGQP_BUSY, "hgqp_server: stopping");
+            }
+            queue.push_back(std::move(r));
+            ++outstanding;
+        }
+        cv.notify_all();
     }
 
     // Each turn enqueues period i (after the admissions of this turn), then
@@ -1082,20 +1117,7 @@ int hgqp_server_submit(hgqp_server* server, int64_t n, int64_t nnz, const int32_
         r->tag = tag;
         r->x_out = x_out;
         r->submitted = std::chrono::steady_clock::now();
-        {
-            std::lock_guard<std::mutex> lk(server->mu);
-            if (!server->error.empty()) {
-                server->give_block(*r);
-                throw Error(HGQP_DEVICE_ERROR, "hgqp_server: " + server->error);
-            }
-            if (server->stopping) {
-                server->give_block(*r);
-                throw Error(HGQP_BUSY, "hgqp_server: stopping");
-            }
-            server->queue.push_back(std::move(r));
-            ++server->outstanding;
-        }
-        server->cv.notify_all();
+        server->enqueue(std::move(r));
     });
 }
 
@@ -1128,3 +1150,58 @@ int hgqp_server_stop(hgqp_server* server) {
 }
 
 }  // extern "C"
+
+namespace hgqp {
+
+void server_submit_device(hgqp_server* server, std::unique_ptr<DeviceCsr> csr, const double* cost,
+                          const double* lower, const double* upper, double lipschitz, int32_t maximize,
+                          double offset, double tolerance, uint64_t iteration_limit, int64_t tag, double* x_out,
+                          bool diag) {
+    if (!server || !csr) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_server handle");
+    const int64_t n = csr->n, nnz = csr->nnz;
+    const int32_t* start = csr->host_start.data();
+    // the ingest produced the CSR (columns in range, finite nonzero values);
+    // the rest is checked as hgqp_server_submit checks it
+    if (n <= 0 || int64_t(csr->host_start.size()) != n + 1 || !cost || !lower || !upper || !x_out)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit_problem: null array or bad size");
+    if (!(lipschitz > 0) || !std::isfinite(lipschitz))
+        throw Error(HGQP_UNSUPPORTED, "hgqp_server_submit_problem: needs a positive finite Lipschitz bound");
+    if (!(tolerance > 0) || !std::isfinite(tolerance))
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit_problem: bad tolerance");
+    if (start[0] != 0 || start[n] != nnz) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit_problem: bad CSR");
+    bool zero = true;
+    for (int64_t i = 0; i < n; ++i) {
+        if (std::isnan(lower[i]) || std::isnan(upper[i]) || lower[i] > upper[i])
+            throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit_problem: bounds must satisfy lower <= upper");
+        zero = zero && lower[i] <= 0 && 0 <= upper[i];
+    }
+    if (n + 1 > server->pool->opt.max_rows || nnz > server->pool->opt.max_nnz)
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_server_submit_problem: instance larger than the pool budget");
+    auto r = std::make_unique<hgqp_server::Request>();
+    r->n = n;
+    r->nnz = nnz;
+    r->block_bytes = hgqp_server::Request::bytes_for(n, nnz, false);
+    r->block = server->staging.take(r->block_bytes);
+    if (!r->block) r->heap.resize(r->block_bytes);
+    r->carve(r->block ? r->block : r->heap.data(), false);
+    std::memcpy(r->start, start, size_t(n + 1) * 4);
+    std::memcpy(r->cost, cost, size_t(n) * 8);
+    std::memcpy(r->lower, lower, size_t(n) * 8);
+    std::memcpy(r->upper, upper, size_t(n) * 8);
+    r->index = csr->col;
+    r->value = csr->val;
+    r->dev = std::move(csr);
+    r->lipschitz = lipschitz;
+    r->maximize = maximize;
+    r->offset = offset;
+    r->tol = tolerance;
+    r->diag = diag;
+    r->zero = zero;
+    r->limit = iteration_limit;
+    r->tag = tag;
+    r->x_out = x_out;
+    r->submitted = std::chrono::steady_clock::now();
+    server->enqueue(std::move(r));
+}
+
+}  // namespace hgqp

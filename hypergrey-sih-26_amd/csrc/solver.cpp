@@ -454,17 +454,10 @@ extern "C" int hgqp_server_submit_problem(hgqp_server* server, const hgqp_proble
                                                       : HGQP_STATUS_UNSUPPORTED_CURVATURE;
             return;
         }
-        // the serving loop stages host arrays: read the assembled CSR back
-        std::vector<int32_t> col(size_t(pb.csr.nnz));
-        std::vector<double> val(size_t(pb.csr.nnz));
-        if (pb.csr.nnz) {
-            HIPCHECK(hipMemcpy(col.data(), pb.csr.col, col.size() * 4, hipMemcpyDeviceToHost));
-            HIPCHECK(hipMemcpy(val.data(), pb.csr.val, val.size() * 8, hipMemcpyDeviceToHost));
-        }
-        const int code = hgqp_server_submit(server, problem->n, pb.csr.nnz, pb.csr.host_start.data(), col.data(),
-                                            val.data(), pb.cost.data(), m.lower, m.upper, pb.lipschitz,
-                                            m.maximize ? 1 : 0, m.offset, tolerance, iteration_limit, tag, x_out);
-        if (code != HGQP_OK) throw Error(code, hgqp_last_error());
+        // the assembled CSR stays on the device: the request takes it over
+        server_submit_device(server, std::make_unique<DeviceCsr>(std::move(pb.csr)), pb.cost.data(), m.lower,
+                             m.upper, pb.lipschitz, m.maximize ? 1 : 0, m.offset, tolerance, iteration_limit, tag,
+                             x_out, pb.positive_diagonal);
     });
     return unsupported ? HGQP_UNSUPPORTED : rc;
 }

@@ -496,8 +496,10 @@ int hgqp_server_stop(hgqp_server* server);
 /* Submits a model in the reference's own form (hgqp_problem: lower-triangle
  * triplets with full values, original sense) to the serving loop. Runs the host
  * side of solve_accelerated_box first (accelerated_qp.cpp:14-35): validation,
- * the convexity gate, sign normalization, the mirrored CSR and L. A model that
- * ends there is not submitted; *status then holds
+ * the convexity gate, sign normalization, the mirrored CSR and L (assembled on
+ * the device; the request keeps that CSR there and the serving loop copies it
+ * into the pool device to device). A model that ends there is not submitted;
+ * *status then holds
  * HGQP_STATUS_UNSUPPORTED_CURVATURE or HGQP_STATUS_INFEASIBLE_BOUNDS. Otherwise
  * *status is HGQP_STATUS_RUNNING and the result arrives through
  * hgqp_server_poll under `tag`. Invalid models return HGQP_INVALID_ARGUMENT, a

@@ -156,3 +156,50 @@ def test_server_results_equal_alone(hg):
     for k in range(4):
         assert np.array_equal(got[k]["primal"], alone[k]["primal"])
         assert got[k]["iterations"] == alone[k]["iterations"]
+
+
+def serve(hg, submit, count, **options):
+    """Runs `submit(server)` and polls until `count` results are back."""
+    s = hg.Server(**options)
+    try:
+        submit(s)
+        got = {}
+        t0 = time.time()
+        while len(got) < count and time.time() - t0 < 60:
+            for r in s.poll():
+                got[r["tag"]] = r
+            time.sleep(0.001)
+    finally:
+        s.stop()
+    assert sorted(got) == list(range(count))
+    return got
+
+
+def test_server_problem_submission_equals_csr_submission(hg, oracle):
+    """hgqp_server_submit_problem assembles the model on the device and hands
+    that CSR to the serving loop as it is (no copy back to the host): its
+    results equal hgqp_server_submit's on the host CSR of the same models (the
+    planted CSR is the mirrored assembly of its lower triangle) in status,
+    iterations and every primal bit, and each passes the oracle's check."""
+    batch = hg.gen_batch(8, seed=23, n_min=300, n_max=3000)
+    opts = dict(max_rows=3 * 3100, max_nnz=3 * 3100 * 70, max_instances=3)
+
+    def by_problem(s):
+        for k, d in enumerate(batch):
+            hr, hc, hv = hg.lower_triplets(d["start"], d["index"], d["value"])
+            assert s.submit_problem(d["n"], hr, hc, hv, d["cost"], d["lower"], d["upper"], tag=k,
+                                    tolerance=KKT_TOL) == "running"
+
+    def by_csr(s):
+        for k, d in enumerate(batch):
+            s.submit(d, tag=k, tolerance=KKT_TOL)
+
+    print("case: server submit_problem vs submit, 8 planted instances", flush=True)
+    a = serve(hg, by_problem, len(batch), **opts)
+    b = serve(hg, by_csr, len(batch), **opts)
+    for k, d in enumerate(batch):
+        assert a[k]["status"] == b[k]["status"] == "optimal"
+        assert a[k]["iterations"] == b[k]["iterations"]
+        assert np.array_equal(a[k]["primal"], b[k]["primal"])
+        chk = oracle.check(as_qp(hg, d), a[k]["primal"], KKT_TOL)
+        assert chk["optimality_verified"] and chk["stationarity_residual"] <= KKT_TOL
