@@ -478,6 +478,42 @@ PUBLISHED_TRIDIAGONAL = {10000: (0.1159, 0.0838, 0.3821), 50000: (0.5729, 0.3955
 TRIDIAGONAL_MODES = ("fp64_reference", "fp64_restart_jacobi", "fp32_refine_restart_jacobi")
 
 
+def slab_groups_workload(args):
+    """FP32 step time of the slab layouts on the planted instance: a fixed run
+    of FP32 steps (no phase switch, one check at the end) on handles with one
+    (ORDERED), two and four (TREE) workgroups per row block, timed with the
+    handle's HIP events around each period's step launches."""
+    inst = hgqp.gen_planted(args.n, args.draws, args.seed)
+    L = hgqp.lipschitz(inst["start"], inst["value"])
+    steps = args.group_steps
+    out = {}
+    for groups in (1, 2, 4):
+        reduce = hgqp.REDUCE_ORDERED if groups == 1 else hgqp.REDUCE_TREE
+        box = hgqp.Box(inst["start"], inst["index"], inst["value"], inst["cost"], inst["lower"], inst["upper"], L,
+                       precision=hgqp.PRECISION_FP32_POLISH, momentum=hgqp.MOMENTUM_REFERENCE, reduce=reduce,
+                       step_metric=hgqp.STEP_JACOBI, slabs=2, slab_groups=groups, timing=1, use_graph=0)
+        try:
+            info = box.info()
+            per = []
+            for rep in range(3):
+                box.reset()
+                r = box.solve(tolerance=KKT_TOL, iteration_limit=steps, check_every=steps, fp32_switch=1e-300,
+                              time_limit=60)
+                per.append(r["step32_kernel_ms"] / r["step32_launches"] * 1e3)
+        finally:
+            box.close()
+        out[str(groups)] = {"us_per_step": [round(u, 2) for u in per], "slabs32": info["slabs32"],
+                            "slab_groups": info["slab_groups"], "row_blocks": info["row_blocks"]}
+        print(f"# {groups} group(s): {out[str(groups)]}", file=sys.stderr, flush=True)
+    nbytes = step_bytes(args.n, len(inst["value"]), 4, 4, jacobi=True)
+    best = min(min(v["us_per_step"]) for v in out.values())
+    return {"metric": "fp32_step_us", "value": round(best, 2), "unit": "us", "n_gpus": 1, "steps": steps,
+            "warmup": 0, "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded planted instance)",
+            "config": {"workload": f"planted n={args.n}, {args.draws} draws/row, nnz={len(inst['value'])}: "
+                                   f"{steps} FP32 Jacobi steps per run, three runs per layout"},
+            "algorithmic_bytes_per_step": nbytes, "groups": out, "device": hgqp.device_name()}
+
+
 def tridiagonal_workload(args):
     """The reference's own scale benchmark (scale_benchmark.cpp:13-25): the
     seeded tridiagonal family H = tridiag(-1, 2.02, -1), x in [0, 3], tolerance
@@ -679,8 +715,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--check-every", type=int, default=10, help="check cadence of the non-reference modes")
     ap.add_argument("--polish-every", type=int, default=5, help="FP64 check cadence after an FP32 phase")
-    ap.add_argument("--workload", choices=["single", "batch", "serve", "dense", "admm", "tridiagonal"],
+    ap.add_argument("--workload", choices=["single", "batch", "serve", "dense", "admm", "tridiagonal", "slab-groups"],
                     default="single")
+    ap.add_argument("--group-steps", type=int, default=200, help="slab-groups: FP32 steps per timed run")
     ap.add_argument("--tri-sizes", default="10000,50000,200000,500000,1000000",
                     help="tridiagonal: the n of the reference's scale benchmark")
     ap.add_argument("--admm-n", type=int, default=100_000, help="admm: variables (rows: n/20 equality, 3n/20 inequality)")
@@ -723,6 +760,9 @@ def main():
         return
     if args.workload == "tridiagonal":
         print(json.dumps(tridiagonal_workload(args)))
+        return
+    if args.workload == "slab-groups":
+        print(json.dumps(slab_groups_workload(args)))
         return
 
     device = hgqp.device_name()
@@ -794,7 +834,7 @@ def main():
     tname = "float" if use32 else "double"
     if slabs and groups > 1:
         kernel = (f"slab_sum_kernel + slab_finish_kernel (one FP32 step over {slabs} column slabs staged in LDS, "
-                  f"two workgroups per row block)")
+                  f"{groups} workgroups per row block)")
     elif slabs:
         kernel = f"slab_step_kernel (one FP32 step over {slabs} column slabs staged in LDS)"
     elif panels == 1:
