@@ -235,6 +235,9 @@ def cpu_baseline(inst, threads):
     return out
 
 
+SLAB_BLOCK_CANDIDATES = (16384, 32768, 65536, 131072)  # stored entries per slab block
+
+
 def batch_workload(args):
     """512 uneven planted instances solved together in one pool (FP32 correction
     cycles, FP64 verification), against the plain FP64 single-instance loop."""
@@ -243,9 +246,24 @@ def batch_workload(args):
     nnz = sum(len(d["value"]) for d in batch)
     Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in batch]
     print(f"# batch: {len(batch)} instances, rows {rows}, nnz {nnz}", file=sys.stderr, flush=True)
-    pool = hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
-                     slabs=args.pool_slabs,
-                     check_every=args.check_every, step_metric=STEP[args.pool_step])
+
+    def make_pool(entries=0):
+        return hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
+                         slabs=args.pool_slabs, slab_block_entries=entries,
+                         check_every=args.check_every, step_metric=STEP[args.pool_step])
+
+    # slab pools: the slab-block size, timed (all instances active, six FP32 steps) for each candidate
+    slab_tuning, entries = None, 0
+    if args.pool_slabs:
+        slab_tuning = {}
+        for cand in SLAB_BLOCK_CANDIDATES:
+            with make_pool(cand) as probe:
+                for k, d in enumerate(batch):
+                    assert probe.admit(d, tag=k, tolerance=KKT_TOL, lipschitz_value=Ls[k]) is not None
+                slab_tuning[cand] = round(probe.time_step(6)[0], 1)
+            print(f"# slab blocks of {cand} entries: {slab_tuning[cand]} us per step", file=sys.stderr, flush=True)
+        entries = min(slab_tuning, key=slab_tuning.get)
+    pool = make_pool(entries)
     times, results, admits = [], None, []
     tuned = None
     try:
@@ -294,9 +312,13 @@ def batch_workload(args):
         "config": {"workload": f"batched: {len(batch)} planted box QPs, n log-uniform in [1e3, 1e5], "
                                f"draws/row uniform in 4..32, bounds l~U(-2,-0.5), u~U(0.5,2), KKT<=1e-6, "
                                f"one pool, graph-replayed check periods; one step = the whole batch",
-                   "rows": rows, "nnz": nnz, "autotuned_block_loads": tuned[0], "autotuned_block_rows": tuned[1],
-                   "check_every": args.check_every, "step_metric": args.pool_step,
-                   "slabs": args.pool_slabs},
+                   "rows": rows, "nnz": nnz, "check_every": args.check_every, "step_metric": args.pool_step,
+                   "slabs": args.pool_slabs,
+                   # what was timed: the slab-block size of a slab pool, the row-block shape of the others
+                   # (a slab pool's row blocks serve its FP64 checks only and keep a fixed shape)
+                   **({"autotuned_slab_block_entries": entries, "slab_block_step_us": slab_tuning,
+                       "check_block_loads": tuned[0], "check_block_rows": tuned[1]} if args.pool_slabs else
+                      {"autotuned_block_loads": tuned[0], "autotuned_block_rows": tuned[1]})},
         "roofline": {"bound": "hbm", "achieved": round(step_bytes_all / step_us / 1e3, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(step_bytes_all / step_us / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": (f"pool_slab_step (FP32 step over per-instance column slabs, all {len(batch)} "

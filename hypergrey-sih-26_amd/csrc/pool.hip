@@ -173,6 +173,8 @@ void validate_options(const hgqp_pool_options& o) {
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: fp32_switch must be positive");
     if (o.step_metric != HGQP_STEP_SCALAR && o.step_metric != HGQP_STEP_JACOBI)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: unknown step metric");
+    if (o.slab_block_entries != 0 && (o.slab_block_entries < 4096 || o.slab_block_entries > (1 << 22)))
+        throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: slab_block_entries must be 0 or in [4096, 4194304]");
     if (o.slabs != 0 && o.slabs != 1) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_pool: slabs must be 0 or 1");
 }
 
@@ -474,8 +476,9 @@ int reserve(hgqp_pool* p, int64_t n, int64_t nnz, const int32_t* start, int32_t*
     h.rows_len = rows_len;
     h.ent0 = e0;
     if (p->opt.slabs) {
-        // slab blocks of at most kPoolSlabRows rows and about 64K entries
-        const int target = int(std::max<int64_t>(1, (nnz + 65535) / 65536));
+        // slab blocks of at most kPoolSlabRows rows and about slab_block_entries entries
+        const int64_t per = p->opt.slab_block_entries ? p->opt.slab_block_entries : 65536;
+        const int target = int(std::max<int64_t>(1, (nnz + per - 1) / per));
         h.srb = slab_blocks(n, start, target, kPoolSlabRows);
         h.nsblk = int64_t(h.srb.size()) - 1;
         h.sblk0 = p->sblks.alloc(h.nsblk);

@@ -124,7 +124,7 @@ class PoolOptions(C.Structure):
     _fields_ = [("max_rows", C.c_int64), ("max_nnz", C.c_int64), ("max_instances", C.c_int32),
                 ("check_every", C.c_int32), ("block_loads", C.c_int32), ("block_rows", C.c_int32),
                 ("use_graph", C.c_int32), ("momentum", C.c_int32), ("fp32_switch", C.c_double),
-                ("step_metric", C.c_int32), ("slabs", C.c_int32)]
+                ("step_metric", C.c_int32), ("slabs", C.c_int32), ("slab_block_entries", C.c_int32)]
 
 
 class PoolResult(C.Structure):

@@ -413,6 +413,9 @@ typedef struct hgqp_pool_options {
                                 it; instances need ascending columns, stored
                                 entries and n <= 786432); 0 (default): row
                                 blocks. The server needs 0.                      */
+    int32_t slab_block_entries; /* pools with slabs: stored entries per slab block
+                                (about; at most 4000 rows each), 4096 .. 4194304;
+                                0 = 65536                                         */
 } hgqp_pool_options;
 
 void hgqp_pool_options_default(hgqp_pool_options* options);
