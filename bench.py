@@ -337,22 +337,55 @@ def batch_workload(args):
 
 def serve_workload(args):
     """Poisson arrival trace served by the continuous-batching loop, against the
-    plain one-at-a-time FP64 loop on the same trace (both in real time)."""
+    plain one-at-a-time FP64 loop on the same trace (both in real time). Every
+    request is its own planted instance (seeded by its index), generated ahead
+    of its arrival by a pool of host threads and dropped once submitted; with
+    --distinct K > 0 the requests draw from K instances instead."""
     import threading
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(args.seed)
-    distinct = [hgqp.gen_planted(int(round(np.exp(rng.uniform(np.log(1000), np.log(200000))))), 8,
-                                 (args.seed << 20) + k, -1.0, 1.0) for k in range(args.distinct)]
-    Ls = [hgqp.lipschitz(d["start"], d["value"]) for d in distinct]
+    total = max(args.requests, args.saturate)
+    sizes = np.round(np.exp(rng.uniform(np.log(1000), np.log(200000), size=total))).astype(np.int64)
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=args.requests))
-    pick = rng.integers(0, len(distinct), size=max(args.requests, args.saturate))
-    print(f"# serve: {args.requests} requests at {args.rate}/s over {arrivals[-1]:.1f}s; "
-          f"{len(distinct)} distinct instances", file=sys.stderr, flush=True)
+    pick = rng.integers(0, args.distinct, size=total) if args.distinct > 0 else np.arange(total)
 
-    def pool_server(arrivals, count, producers=1):
+    def make(j):
+        """Request j's instance and Lipschitz bound (the same for every replay of the trace)."""
+        k = int(pick[j])
+        d = hgqp.gen_planted(int(sizes[k]), 8, (args.seed << 20) + k, -1.0, 1.0)
+        return d, hgqp.lipschitz(d["start"], d["value"])
+
+    makers = ThreadPoolExecutor(max_workers=8)
+    lookahead = 256
+
+    class Ahead:
+        """Instances of requests first .. count-1 in order, generated `lookahead` ahead."""
+
+        def __init__(self, count):
+            self.count, self.next, self.q = count, 0, {}
+            for j in range(min(lookahead, count)):
+                self.q[j] = makers.submit(make, j)
+            self.next = min(lookahead, count)
+
+        def get(self, j):
+            inst = self.q.pop(j).result()
+            if self.next < self.count:
+                self.q[self.next] = makers.submit(make, self.next)
+                self.next += 1
+            return inst
+
+    sample = set(range(0, args.requests, max(1, args.requests // 200)))
+    print(f"# serve: {args.requests} requests at {args.rate}/s over {arrivals[-1]:.1f}s; "
+          f"{'a distinct instance per request' if args.distinct <= 0 else f'{args.distinct} distinct instances'}",
+          file=sys.stderr, flush=True)
+
+    def pool_server(arrivals, count, producers=1, ready=None):
+        """ready: instances generated beforehand (the capacity run); else generated ahead of arrival."""
         srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
                           check_every=args.check_every, use_graph=0 if args.no_graph else 1,
                           step_metric=STEP[args.pool_step])
-        done = {}
+        done, optimum = {}, {}
+        ahead = None if ready is not None else Ahead(count)
         t0 = time.perf_counter()
         try:
             failed = []
@@ -360,11 +393,13 @@ def serve_workload(args):
             def producer(first):
                 try:
                     for j in range(first, count, producers):
+                        d, L = ready[j] if ready is not None else ahead.get(j)
                         delay = t0 + arrivals[j] - time.perf_counter()
                         if delay > 0:
                             time.sleep(delay)
-                        k = int(pick[j])
-                        srv.submit(distinct[k], tag=j, tolerance=KKT_TOL, lipschitz_value=Ls[k])
+                        srv.submit(d, tag=j, tolerance=KKT_TOL, lipschitz_value=L)
+                        if j in sample:
+                            optimum[j] = d["optimum"]
                 except BaseException as e:  # surfaced by the poll loop below
                     failed.append(e)
             threads = [threading.Thread(target=producer, args=(i,)) for i in range(producers)]
@@ -383,24 +418,23 @@ def serve_workload(args):
             srv.stop()
         lat = np.array([done[j]["latency_seconds"] for j in range(count)]) * 1e3
         ok = all(done[j]["status"] == "optimal" for j in range(count))
-        err = max(float(np.abs(done[j]["primal"] - distinct[int(pick[j])]["optimum"]).max())
-                  for j in range(0, count, max(1, count // 200)))
+        err = max(float(np.abs(done[j]["primal"] - optimum[j]).max()) for j in optimum) if optimum else None
         return {"requests": count, "solves_per_s": count / wall, "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "all_verified": ok, "wall_s": wall,
-                "sampled_max_abs_err_vs_planted_x": err}
+                "sampled_max_abs_err_vs_planted_x": err, "sampled": len(optimum)}
 
-    def plain_loop(arrivals, count):
+    def plain_loop(arrivals, count, ready=None):
         """One request at a time in arrival order: upload, FP64 reference-schedule solve, read x."""
         lat = []
         ok = True
+        ahead = None if ready is not None else Ahead(count)
         t0 = time.perf_counter()
         for j in range(count):
+            d, L = ready[j] if ready is not None else ahead.get(j)
             now = time.perf_counter() - t0
             if now < arrivals[j]:
                 time.sleep(arrivals[j] - now)
-            d = distinct[int(pick[j])]
-            with hgqp.Box(d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"],
-                          Ls[int(pick[j])]) as b:
+            with hgqp.Box(d["start"], d["index"], d["value"], d["cost"], d["lower"], d["upper"], L) as b:
                 r = b.solve(tolerance=KKT_TOL, iteration_limit=20000, time_limit=60)
                 b.primal()
             ok &= r["status"] == "optimal"
@@ -418,13 +452,16 @@ def serve_workload(args):
     saturated = None
     if args.saturate:
         at_once = np.zeros(args.saturate)
-        sat_served = pool_server(at_once, args.saturate, producers=4)  # as the HTTP route submits
+        ready = list(makers.map(make, range(args.saturate)))  # generated before the clock starts
+        sat_served = pool_server(at_once, args.saturate, producers=4, ready=ready)  # as the HTTP route submits
         print(f"# served (saturated): {sat_served}", file=sys.stderr, flush=True)
-        sat_plain = plain_loop(at_once, args.saturate) if not args.no_plain else None
+        sat_plain = plain_loop(at_once, args.saturate, ready=ready) if not args.no_plain else None
+        del ready
         print(f"# plain (saturated): {sat_plain}", file=sys.stderr, flush=True)
         saturated = {"requests": args.saturate, "served": sat_served, "plain_fp64_one_at_a_time": sat_plain,
                      "speedup_solves_per_s": (sat_served["solves_per_s"] / sat_plain["solves_per_s"]
                                               if sat_plain else None)}
+    makers.shutdown(wait=False)
     return {
         "metric": METRIC, "value": round(served["solves_per_s"], 2), "unit": "solves/s", "n_gpus": 1,
         "steps": args.requests, "warmup": 0, "ms_per_step": round(served["wall_s"] / args.requests * 1e3, 3),
@@ -433,7 +470,8 @@ def serve_workload(args):
         "config": {"workload": f"serving: {args.requests} requests, Poisson lambda={args.rate}/s, n log-uniform in "
                                f"[1e3, 2e5], 8 draws/row, bounds [-1,1], KKT<=1e-6, continuous batching in one pool",
                    "slots": args.slots, "budget_rows": args.budget_rows, "budget_nnz": args.budget_nnz,
-                   "distinct_instances": args.distinct, "check_every": args.check_every,
+                   "distinct_instances": args.distinct if args.distinct > 0 else "one per request",
+                   "check_every": args.check_every,
                    "step_metric": args.pool_step,
                    "graph_replay": not args.no_graph},
         "p50_ms": round(served["p50_ms"], 3), "p99_ms": round(served["p99_ms"], 3),
@@ -750,7 +788,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=512, help="batch: instances timed one at a time in FP64")
     ap.add_argument("--requests", type=int, default=20000, help="serve: requests in the trace")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate per second")
-    ap.add_argument("--distinct", type=int, default=64, help="serve: distinct instances the trace draws from")
+    ap.add_argument("--distinct", type=int, default=0,
+                    help="serve: distinct instances the trace draws from; 0 = a distinct instance per request")
     ap.add_argument("--slots", type=int, default=256, help="serve: pool slots")
     ap.add_argument("--budget-rows", type=int, default=8_000_000, help="serve: pool row budget")
     ap.add_argument("--budget-nnz", type=int, default=160_000_000, help="serve: pool entry budget")
@@ -761,7 +800,7 @@ def main():
                     help="batch/serve: step metric of the pool (per-row Jacobi or the scalar 1/L)")
     ap.add_argument("--no-graph", action="store_true",
                     help="serve, admm: launch each kernel directly instead of replaying captured graphs")
-    ap.add_argument("--saturate", type=int, default=4000,
+    ap.add_argument("--saturate", type=int, default=2000,
                     help="serve: also replay this many requests queued at once (capacity); 0 = skip")
     args = ap.parse_args()
     if args.gpus != 1:
