@@ -905,15 +905,22 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         set_control(b, fresh_control());
         uint64_t it = 0;
         bool time_up = false;
-        // check at iteration 0 (accelerated_qp.cpp:40); at x0 = 0 the gradient is c, no matrix pass
-        {
+        // Check at iteration 0 (accelerated_qp.cpp:40); at x0 = 0 the gradient is c, no
+        // matrix pass. The FP32 modes skip it at x0 = 0: their first phase starts from
+        // x0 whatever it says (the FP64 check after the phase decides, also where x0 is
+        // already optimal), and the refine mode's first anchor gradient is g(0) = c,
+        // exactly what the check would write. That saves the check, its read-back and
+        // the host's wait for it.
+        const bool skip_first = f32 && b->at_zero && so.iteration_limit > 0;
+        Control c = fresh_control();  // the block set above
+        if (!skip_first) {
             CheckParams first = cp64;
             first.zero_x = b->at_zero ? 1 : 0;
             launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), first, b->stream);
             HIPCHECK(hipGetLastError());
-            b->at_zero = false;
+            c = get_control(b);
         }
-        Control c = get_control(b);
+        b->at_zero = false;
         // Resets the momentum state and the done flag at iteration `at` for a new phase.
         auto new_phase = [&](uint64_t at) {
             Control cc = c;
@@ -951,7 +958,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             // (switch_stat); later ones start from an FP64 anchor and aim below tol.
             for (int cycle = 0; cycle < 8 && c.done == kRunning && !time_up && it < so.iteration_limit; ++cycle) {
                 const int pa = b->parity;
-                launch_anchor(b->n, b->g, b->x[pa], b->lo, b->up, b->c32, b->lo32, b->up32, b->x32[pa], b->w32[pa],
+                const double* g = cycle == 0 && skip_first ? b->c : b->g;  // g(x0) = H 0 + c = c
+                launch_anchor(b->n, g, b->x[pa], b->lo, b->up, b->c32, b->lo32, b->up32, b->x32[pa], b->w32[pa],
                               b->stream);
                 HIPCHECK(hipGetLastError());
                 new_phase(it);
