@@ -351,9 +351,10 @@ __device__ void restart_decision(Control* ctrl, const double* partials, int grou
 // and stores (every vector starts 256-byte aligned; the partial stride is a
 // multiple of 64); workgroup 0 also takes the rows past the last whole group
 // of four, one by one. The restart partials are one per workgroup of this
-// launch.
-template <int MOM>
-__global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, int groups, const float* __restrict__ part,
+// launch. G (the layout's groups) is a template argument so that a thread's
+// G partial loads and its vector loads go out together.
+template <int MOM, int G>
+__global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, const float* __restrict__ part,
                                                                    int64_t stride, StepBuffers<float> bufs,
                                                                    Control* ctrl, double* partials, int parity,
                                                                    double beta_imm) {
@@ -370,7 +371,8 @@ __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, int gr
     const int quads = n / 4;
     for (int q = blockIdx.x * kSlabThreads + threadIdx.x; q < quads; q += gridDim.x * kSlabThreads) {
         float4 g = reinterpret_cast<const float4*>(part)[q];
-        for (int h = 1; h < groups; ++h) {
+#pragma unroll
+        for (int h = 1; h < G; ++h) {
             const float4 p = reinterpret_cast<const float4*>(part + h * stride)[q];
             g.x = g.x + p.x;
             g.y = g.y + p.y;
@@ -395,7 +397,8 @@ __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, int gr
     if (blockIdx.x == 0)
         for (int r = 4 * quads + int(threadIdx.x); r < n; r += kSlabThreads) {
             float g = part[r];
-            for (int h = 1; h < groups; ++h) g = g + part[h * stride + r];
+#pragma unroll
+            for (int h = 1; h < G; ++h) g = g + part[h * stride + r];
             epi(r, g);
         }
     slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, red);
@@ -717,6 +720,17 @@ template void launch_step<double>(const PanelMatrix<double>&, const StepBuffers<
 template void launch_step<float>(const PanelMatrix<float>&, const StepBuffers<float>&, Control*, double*, int, int,
                                  bool, double, hipStream_t, double);
 
+template <int MOM>
+void launch_slab_finish(const SlabMatrix& m, dim3 fin, dim3 block, const StepBuffers<float>& b, Control* ctrl,
+                        double* partials, int parity, double beta, hipStream_t stream) {
+    if (m.groups == 4)
+        hipLaunchKernelGGL((slab_finish_kernel<MOM, 4>), fin, block, 0, stream, int(m.n), m.part, m.part_stride, b,
+                           ctrl, partials, parity, beta);
+    else
+        hipLaunchKernelGGL((slab_finish_kernel<MOM, 2>), fin, block, 0, stream, int(m.n), m.part, m.part_stride, b,
+                           ctrl, partials, parity, beta);
+}
+
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
                       int momentum, double beta, hipStream_t stream, double switch_stat) {
     dim3 block(kSlabThreads);
@@ -735,13 +749,12 @@ void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control*
         }
 #undef HGQP_SLAB_SUM
         const dim3 fin(slab_finish_groups(m.n));
-#define HGQP_SLAB_FINISH(MOM)                                                                                      \
-    hipLaunchKernelGGL((slab_finish_kernel<MOM>), fin, block, 0, stream, m.n, m.groups, m.part, m.part_stride, b, \
-                       ctrl, partials, parity, beta)
-        if (momentum == kMomentumImmediate) HGQP_SLAB_FINISH(kMomentumImmediate);
-        else if (momentum == kMomentumReference) HGQP_SLAB_FINISH(kMomentumReference);
-        else HGQP_SLAB_FINISH(kMomentumRestart);
-#undef HGQP_SLAB_FINISH
+        if (momentum == kMomentumImmediate)
+            launch_slab_finish<kMomentumImmediate>(m, fin, block, b, ctrl, partials, parity, beta, stream);
+        else if (momentum == kMomentumReference)
+            launch_slab_finish<kMomentumReference>(m, fin, block, b, ctrl, partials, parity, beta, stream);
+        else
+            launch_slab_finish<kMomentumRestart>(m, fin, block, b, ctrl, partials, parity, beta, stream);
         if (momentum == kMomentumRestart)
             hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(fin.x), parity,
                                switch_stat);
