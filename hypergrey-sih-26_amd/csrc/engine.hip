@@ -370,6 +370,10 @@ hipGraphExec_t period_graph(hgqp_box* b, int precision, int steps, int momentum,
 struct LoopOutcome {
     bool time_up = false;
     uint64_t iterations = 0;  // iterations enqueued
+    // the control block of the read-back that saw the phase end (launches
+    // queued after it return early and leave the block as it was)
+    bool ended = false;
+    Control last{};
 };
 
 // Runs periods of one precision until a check sets `done`, the iteration limit
@@ -404,12 +408,41 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
         if (shrink || !(last_stat > 0) || !std::isfinite(prev_stat) || !(prev_stat > last_stat)) return false;
         return last_stat * (last_stat / prev_stat) <= cp.tol;
     };
+    // Residual phases: the iteration at which the steps' best residual reaches
+    // the phase's threshold, extrapolated geometrically from the last two
+    // read-backs (best residual and the iteration it was read at); 0 when there
+    // is no estimate. Periods are cut to end there, and none is queued past it
+    // before its read-back says whether the phase ended, so the launches after
+    // the deciding step are not a whole queued period. A wrong estimate costs a
+    // host round trip; the device still ends the phase at its own step.
+    uint64_t slot_iter[hgqp_box::kSlots] = {};
+    double best_last = std::numeric_limits<double>::quiet_NaN(), best_prev = best_last;
+    uint64_t iter_last = 0, iter_prev = 0;
+    auto predicted_end = [&]() -> uint64_t {
+        if (!shrink || !(cp.switch_stat > 0) || !std::isfinite(best_prev) || !std::isfinite(best_last) ||
+            !(best_last > cp.switch_stat) || !(best_prev > best_last) || iter_last <= iter_prev)
+            return 0;
+        const double rate = std::log(best_last / best_prev) / double(iter_last - iter_prev);  // < 0
+        const double steps = std::ceil(std::log(cp.switch_stat / best_last) / rate);
+        if (!(steps >= 1) || steps > 1e6) return 0;
+        // one step of margin: a late estimate costs one early-returning step, an
+        // early one a host round trip with the GPU idle
+        return iter_last + uint64_t(steps) + 1;
+    };
     // waits for read-back slot s; true when the phase is over
     auto read = [&](int s) {
         HIPCHECK(hipEventSynchronize(b->slot_event[s]));
-        if (b->host_slots[s].done != kRunning) return true;
+        if (b->host_slots[s].done != kRunning) {
+            out.ended = true;
+            out.last = b->host_slots[s];
+            return true;
+        }
         prev_stat = last_stat;
         last_stat = b->host_slots[s].stationarity;
+        best_prev = best_last;
+        iter_prev = iter_last;
+        best_last = b->host_slots[s].best_stat;
+        iter_last = slot_iter[s];
         if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= time_limit) {
             out.time_up = true;
             return true;
@@ -418,16 +451,19 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
     };
     int period = check_every;
     while (it < limit && !stop) {
-        if (!pending.empty() && likely_last()) {
+        uint64_t end = predicted_end();
+        if (!pending.empty() && (likely_last() || (end && end <= it))) {
             while (!pending.empty() && !stop) {
                 const int s = pending.front();
                 pending.erase(pending.begin());
                 stop = read(s);
             }
             if (stop) break;
+            end = predicted_end();
         }
-        const int steps = static_cast<int>(std::min<uint64_t>(period, limit - it));
+        int steps = static_cast<int>(std::min<uint64_t>(period, limit - it));
         if (shrink) period = std::max(2, (period + 1) / 2);
+        if (end > it && end - it < uint64_t(steps)) steps = int(end - it);
         if (timed) enqueue_period_timed(b, precision, steps, momentum, cp, it, timed);
         else if (b->opt.use_graph && (steps == check_every || (shrink && steps >= 2))) {
             hipGraphExec_t g = period_graph(b, precision, steps, momentum, cp);
@@ -439,6 +475,7 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
         it += steps;
         const int slot = next_slot;
         next_slot = next_slot % (hgqp_box::kSlots - 1) + 1;
+        slot_iter[slot] = it;
         HIPCHECK(hipMemcpyAsync(&b->host_slots[slot], b->ctrl, sizeof(Control), hipMemcpyDeviceToHost, b->stream));
         HIPCHECK(hipEventRecord(b->slot_event[slot], b->stream));
         pending.push_back(slot);
@@ -940,7 +977,9 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             auto o = run_phase(b, HGQP_PRECISION_FP32_POLISH, momentum, it, so.iteration_limit, so.check_every, cp32,
                                t0, so.time_limit, timed_ptr, c.stationarity);
             time_up = o.time_up;
-            c = get_control(b);
+            // the read-back that saw the phase end holds the final block; timed runs
+            // read it again, which also completes the periods' events
+            c = o.ended && !timed_ptr ? o.last : get_control(b);
             if (c.done != kRunning) {  // steps after the deciding check returned early
                 it = uint64_t(c.check_iter);
                 b->parity = c.check_parity;
