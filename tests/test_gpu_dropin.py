@@ -12,7 +12,8 @@ stationarity bits, and the same primal bit for bit.
 
 Cases: the n = 300 PSD block problem of tests/test_v2.cpp:18-24 (accepted by the
 LDL^T convexity test, not diagonally dominant), the QPLIB models 8792 and 8991
-read by the reference's read_qplib, and the reference's scale_benchmark.cpp
+read by the reference's read_qplib, the planted family of the benchmarks (C1:
+n = 1e4, 8 draws per row; and n = 1e5), and the reference's scale_benchmark.cpp
 driver itself (the seeded tridiagonal family, 1700 iterations) at n = 1e4 and
 1e5 with backend "cuda" against backend "cpu".
 """
@@ -77,6 +78,17 @@ def test_qplib_through_the_reference_loop(tmp_path, model):
     assert rec["device"]["iterations"] == recorded["iterations"]
     assert float.fromhex(rec["device"]["objective"]) == recorded["objective"]
     assert rec["device"]["status"] == recorded["status"]
+
+
+@pytest.mark.parametrize("n,draws,seed", [(10000, 8, 1), (10000, 8, 2026), (100000, 8, 7)])
+def test_planted_through_the_reference_loop(n, draws, seed):
+    """The BASELINE planted family (C1 is n = 1e4 with 8 draws) through the
+    reference's loop at KKT 1e-6: hgqp and the reference CPU path agree bit
+    for bit, and the solve ends optimal and verified."""
+    rc, rec = run([DROPIN, "planted", str(n), str(draws), str(seed)])
+    assert rc == 0
+    assert_same(rec)
+    assert rec["device"]["status"] == "optimal" and rec["device"]["verified"]
 
 
 @pytest.mark.parametrize("n", [10000, 100000])

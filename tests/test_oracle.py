@@ -363,3 +363,21 @@ def test_numpy_steps_by_column_range(oracle):
 
 def f32_add(a, b):
     return (a.astype(np.float32) + b.astype(np.float32)).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,draws,seed", [(10000, 8, 1), (10000, 8, 2026), (3000, 32, 5)])
+def test_planted_family_matches_reference(oracle, reference, hg, n, draws, seed):
+    """The benchmarks' planted family (C1: n = 1e4, 8 draws per row), which the
+    reference never ran: the oracle's solve equals the compiled reference's bit
+    for bit at KKT 1e-6 (status, iterations, objective, residuals, x), and both
+    recover the planted optimum."""
+    from helpers import planted_qp
+    q, d = planted_qp(hg, n, draws, seed)
+    a = oracle.solve_box(q, tolerance=1e-6, iteration_limit=20000, time_limit=120)
+    b = reference.solve_box(q, tolerance=1e-6, iteration_limit=20000, time_limit=120)
+    for k in ("status", "iterations", "objective", "stationarity_residual", "dual_bound", "relative_gap",
+              "optimality_verified", "primal_residual"):
+        assert a[k] == b[k], (k, a[k], b[k])
+    assert np.array_equal(a["primal"], b["primal"])
+    assert a["status"] == "optimal" and a["optimality_verified"]
+    assert np.abs(a["primal"] - d["optimum"]).max() < 1e-4
