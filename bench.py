@@ -518,7 +518,8 @@ def dense_workload(args):
         "config": {"workload": f"dense multi-RHS: n={args.dense_n}, {args.dense_k} cost vectors, bounds [-1,1], "
                                f"KKT<=1e-6; one step = all RHS; BF16 MFMA iterations, FP64 MFMA verification"},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": round(tflops / 2500.0, 4), "traffic": None, "kernel": "gemm_bf16",
+                     "frac": round(tflops / 2500.0, 4), "traffic": None,
+                     "kernel": "BF16 GEMM Q W' (gemm_bf16_wide or gemm_bf16, whichever the handle timed faster)",
                      "avg_launch_us": round(gemm_us, 2),
                      "flops_per_launch": 2 * args.dense_n ** 2 * args.dense_k},
         "all_verified": bool(rep["verified"] == args.dense_k), "max_iterations": int(rep["max_iterations"]),

@@ -153,6 +153,107 @@ __global__ __launch_bounds__(256) void gemm_bf16(int n, int K, const bf16* __res
         }
 }
 
+// The same product with 128-row x 128-RHS tiles for shapes that fit them: 512
+// threads, each wave a 32 x 64 part of the tile (2 x 4 MFMA blocks), k stages
+// of 64 through a ring of kWideRing LDS buffers that direct global-to-LDS
+// copies (16 bytes per lane, 1 KiB per instruction) fill kWideRing - 1 stages
+// ahead. A tile row's eight 16-byte k groups sit in LDS slot g ^ (row & 7), so
+// the 16 rows an MFMA operand read touches spread over the banks. Each wave
+// issues 4 copies per stage (two 1 KiB blocks of Q rows, two of RHS rows), so
+// the wait before stage s leaves the newest (kWideRing - 2) stages in flight;
+// stages past the last are clamped and land in a buffer no later stage reads.
+// Launch: grid (n/128, K/128, splits), 512 threads; n/splits a multiple of 64.
+constexpr int kWideM = 128, kWideN = 128, kWideRing = 4, kWideThreads = 512;
+typedef __attribute__((address_space(3))) void* DenseLds;
+typedef __attribute__((address_space(1))) void* DenseGlobal;
+
+template <int N>
+__device__ __forceinline__ void dense_wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt holds 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | 0x70 | (((N >> 4) & 3) << 14));
+}
+
+__global__ __launch_bounds__(kWideThreads) void gemm_bf16_wide(int n, int K, const bf16* __restrict__ Q,
+                                                              const bf16* __restrict__ Wt,
+                                                              const bf16* __restrict__ Dt,
+                                                              const Rhs* __restrict__ rhs, int parity,
+                                                              float* __restrict__ G) {
+    constexpr int kStageBytes = (kWideM + kWideN) * kBK * 2;  // 32 KiB: Q rows, then RHS rows
+    __shared__ __attribute__((aligned(1024))) char ring[kWideRing * kStageBytes];
+    const int t = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(t / 64), l = t % 64;
+    const int r0 = blockIdx.x * kWideM, c0 = blockIdx.y * kWideN;
+    const int span = n / gridDim.z, kb = blockIdx.z * span, stages = span / kBK;
+    // copy sources of this lane: row (block b, lane l) = 8 b + l / 8, k group (l % 8) ^ (row & 7)
+    const int sub = l / 8, grp = (l % 8) ^ (sub & 7);  // 8 b + sub has the same low three bits as sub
+    const bf16* src[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int row = 8 * (2 * wave + i) + sub;  // Q rows 0..127 of the tile
+        src[i] = Q + size_t(r0 + row) * n + kb + 8 * grp;
+        const int j = c0 + row;  // RHS rows 0..127 of the tile
+        src[2 + i] = (rhs[j].use_x[parity] ? Dt : Wt) + size_t(j) * n + kb + 8 * grp;
+    }
+    auto issue = [&](int st) {
+        const int sc = st < stages ? st : stages - 1;  // past the last: repeat it into a free buffer
+        char* buf = ring + (st % kWideRing) * kStageBytes;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int block = i < 2 ? 2 * wave + i : 16 + 2 * wave + (i - 2);  // 1 KiB blocks of the stage
+            __builtin_amdgcn_global_load_lds((DenseGlobal)(src[i] + size_t(sc) * kBK),
+                                             (DenseLds)(buf + block * 1024), 16, 0, 0);
+        }
+    };
+#pragma unroll
+    for (int st = 0; st < kWideRing - 1; ++st) issue(st);
+    const int wr = (wave / 2) * 32, wc = (wave % 2) * 64;  // this wave's part of the tile
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0, 0, 0, 0};
+    for (int st = 0; st < stages; ++st) {
+        dense_wait_vm<4 * (kWideRing - 2)>();  // this wave's copies of stage st have landed
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();  // and every wave's: the stage is whole in LDS
+        __asm__ volatile("" ::: "memory");
+        issue(st + kWideRing - 1);  // into the buffer stage st - 1 used
+        __asm__ volatile("" ::: "memory");
+        const char* a = ring + (st % kWideRing) * kStageBytes;
+        const char* b = a + kWideM * kBK * 2;
+#pragma unroll
+        for (int ks = 0; ks < kBK; ks += 32) {
+            const int g = ks / 8 + l / 16;  // this lane's k group of the 32-k MFMA step
+            bf16x8 fa[2], fb[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wr + i * 16 + l % 16;
+                fa[i] = *reinterpret_cast<const bf16x8*>(a + row * 128 + 16 * (g ^ (row & 7)));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = wc + j * 16 + l % 16;
+                fb[j] = *reinterpret_cast<const bf16x8*>(b + row * 128 + 16 * (g ^ (row & 7)));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        __asm__ volatile("" ::: "memory");
+    }
+    dense_wait_vm<0>();  // no copy may still target this workgroup's LDS when it ends
+    // lane l holds rows 4(l/16) .. +4 of RHS l % 16 in each 16 x 16 block
+    float* out = G + size_t(blockIdx.z) * K * n;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + wr + i * 16 + 4 * (l / 16), jj = c0 + wc + j * 16 + l % 16;
+            *reinterpret_cast<f32x4*>(&out[size_t(jj) * n + r]) = acc[i][j];
+        }
+}
+
 // Gt[j][r] = sum_k Q[r][k] X[j][k] in FP64 on the matrix cores. A lane takes
 // four consecutive k of its row and column (one 32-byte load each) and issues
 // four MFMAs; k quarters per wave as above.
@@ -448,6 +549,7 @@ using namespace hgqp;
 struct hgqp_dense {
     int64_t n = 0, k = 0;  // caller's sizes
     int np = 0, kp = 0;    // padded
+    bool gemm_wide = false;  // the BF16 GEMM runs with 128 x 128 tiles (gemm_bf16_wide)
     hgqp_dense_options opt{};
     double lipschitz = 0;
     hipStream_t stream = nullptr;
@@ -477,7 +579,18 @@ void check_dense(const hgqp_dense* d) {
     if (!d) throw Error(HGQP_INVALID_ARGUMENT, "null hgqp_dense handle");
 }
 
-void launch_gemm_bf16(hgqp_dense* d, int parity, int splits) {
+bool wide_fits(const hgqp_dense* d, int splits) {
+    return d->np % kWideM == 0 && d->kp % kWideN == 0 && splits >= 1 && splits <= kMaxSplits &&
+           (d->np / splits) % kBK == 0;
+}
+
+void launch_gemm_bf16(hgqp_dense* d, int parity, int splits, bool wide) {
+    if (wide) {
+        if (!wide_fits(d, splits)) throw Error(HGQP_DEVICE_ERROR, "hgqp_dense: GEMM shape does not fit the tiling");
+        hipLaunchKernelGGL(gemm_bf16_wide, dim3(d->np / kWideM, d->kp / kWideN, splits), dim3(kWideThreads), 0,
+                           d->stream, d->np, d->kp, d->q16, d->v.wb, d->v.db, d->v.rhs, parity, d->v.g32);
+        return;
+    }
     // shapes the kernel assumes: 64-row and 64-RHS tiles, k splits of whole 64-stages
     if (d->np % kBM || d->kp % kBN || splits < 1 || splits > kMaxSplits || (d->np / splits) % kBK)
         throw Error(HGQP_DEVICE_ERROR, "hgqp_dense: GEMM shape does not fit the tiling");
@@ -486,13 +599,13 @@ void launch_gemm_bf16(hgqp_dense* d, int parity, int splits) {
 }
 
 // Average time of one BF16 GEMM launch with `splits` k splits.
-double time_gemm_bf16(hgqp_dense* d, int splits, int reps) {
+double time_gemm_bf16(hgqp_dense* d, int splits, bool wide, int reps) {
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
-    launch_gemm_bf16(d, 0, splits);
+    launch_gemm_bf16(d, 0, splits, wide);
     HIPCHECK(hipEventRecord(a, d->stream));
-    for (int r = 0; r < reps; ++r) launch_gemm_bf16(d, 0, splits);
+    for (int r = 0; r < reps; ++r) launch_gemm_bf16(d, 0, splits, wide);
     HIPCHECK(hipEventRecord(b, d->stream));
     HIPCHECK(hipEventSynchronize(b));
     float ms = 0;
@@ -636,19 +749,21 @@ int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, co
         }
         v.lipschitz = d->lipschitz;
         d->host_rhs.resize(KP);
-        // k splits of the BF16 GEMM: the fastest of 1, 2, 4 on this shape
+        // BF16 GEMM: the fastest of the two tilings with 1, 2 or 4 k splits on this shape
         HIPCHECK(hipMemsetAsync(v.wb, 0, KP * N * 2, d->stream));
         HIPCHECK(hipMemsetAsync(v.db, 0, KP * N * 2, d->stream));
         HIPCHECK(hipMemsetAsync(v.rhs, 0, KP * sizeof(Rhs), d->stream));
         double best = std::numeric_limits<double>::infinity();
-        for (int sp = 1; sp <= kMaxSplits; sp *= 2) {
-            if ((d->np / sp) % kBK) continue;
-            const double us = time_gemm_bf16(d.get(), sp, 10);
-            if (us < best) {
-                best = us;
-                v.splits = sp;
+        for (int wide = 0; wide < 2; ++wide)
+            for (int sp = 1; sp <= kMaxSplits; sp *= 2) {
+                if (wide ? !wide_fits(d.get(), sp) : (d->np / sp) % kBK != 0) continue;
+                const double us = time_gemm_bf16(d.get(), sp, wide != 0, 10);
+                if (us < best) {
+                    best = us;
+                    v.splits = sp;
+                    d->gemm_wide = wide != 0;
+                }
             }
-        }
         HIPCHECK(hipStreamSynchronize(d->stream));
         *out = d.release();
     });
@@ -710,7 +825,7 @@ int hgqp_dense_solve(hgqp_dense* d, double tolerance, uint64_t iteration_limit, 
                     HIPCHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
                     int p = parity;
                     for (int s = 0; s < steps; ++s) {
-                        launch_gemm_bf16(d, p, v.splits);
+                        launch_gemm_bf16(d, p, v.splits, d->gemm_wide);
                         hipLaunchKernelGGL(dense_step, segs, dim3(256), 0, d->stream, v, p);
                         hipLaunchKernelGGL(dense_finalize_step, dim3(d->kp), dim3(64), 0, d->stream, v, p, limit);
                         p ^= 1;
@@ -787,7 +902,7 @@ int hgqp_dense_time_gemm(hgqp_dense* d, int32_t reps, double* avg_us, double* tf
     return guarded([&] {
         check_dense(d);
         if (reps < 1 || !avg_us) throw Error(HGQP_INVALID_ARGUMENT, "hgqp_dense_time_gemm: bad argument");
-        *avg_us = time_gemm_bf16(d, d->v.splits, reps);
+        *avg_us = time_gemm_bf16(d, d->v.splits, d->gemm_wide, reps);
         if (tflops) *tflops = 2.0 * double(d->np) * double(d->np) * double(d->kp) / (*avg_us * 1e-6) / 1e12;
     });
 }
