@@ -552,11 +552,12 @@ void upload(hgqp_pool* p, int s, const int32_t* start, const int32_t* index, con
         // the slab layout from the instance's own (local) CSR, before the move to pool numbering
         SlotHost& hs = p->slots[s];
         hs.slab = std::make_shared<SlabStore>();
-        if (!build_slabs(h.n, h.nnz, h.srb, 1, dstart, dcol, p->v.val + h.ent0, *hs.slab, st))
+        // the pool's row sums need not follow stored order: long runs are chained
+        if (!build_slabs(h.n, h.nnz, h.srb, 1, dstart, dcol, p->v.val + h.ent0, *hs.slab, st, true))
             throw Error(HGQP_DEVICE_ERROR, "internal: slab layout of an admitted instance");
         const SlabMatrix& m = hs.slab->m;
         hs.slab_desc = PoolSlab{m.rb, m.tc, m.row, m.col, m.val, m.blocks, m.slabs, int(hs.slab->chunks - 1),
-                                m.long_runs};
+                                m.long_runs, m.chained};
         HIPCHECK(hipMemcpyAsync(p->slab_dev + s, &hs.slab_desc, sizeof(PoolSlab), hipMemcpyHostToDevice, st));
         std::vector<int32_t>& stage = hs.srb_stage;
         stage.resize(size_t(2 * h.nsblk));

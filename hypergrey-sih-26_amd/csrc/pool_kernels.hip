@@ -124,8 +124,12 @@ __global__ __launch_bounds__(kSlabThreads) void pool_slab_step(PoolView v, int p
         const int lr0 = D.rb[lb], rows = D.rb[lb + 1] - lr0;
         for (int t = threadIdx.x; t <= D.slabs; t += kSlabThreads) tcs[t] = D.tc[size_t(lb) * D.slabs + t];
         __syncthreads();  // tile bounds in place; the previous block's reads of ws0 are done
-        slab_rows<kSlabWidth, true>(D.slabs, rows, D.last, LdsBounds{tcs}, D.row, D.col, D.val, epi.gat + s->r0, ws0, ws1, ws2,
-                        acc);
+        if (D.chained)  // workgroup-uniform
+            slab_rows<kSlabWidth, true, LdsBounds, true>(D.slabs, rows, D.last, LdsBounds{tcs}, D.row, D.col, D.val,
+                                                         epi.gat + s->r0, ws0, ws1, ws2, acc);
+        else
+            slab_rows<kSlabWidth, true>(D.slabs, rows, D.last, LdsBounds{tcs}, D.row, D.col, D.val, epi.gat + s->r0,
+                                        ws0, ws1, ws2, acc);
         const int g0 = s->r0 + lr0;
         for (int r = threadIdx.x; r < rows; r += kSlabThreads) epi(g0 + r, acc[r]);
         const double d = wave_sum(epi.dot), res = wave_max(epi.resid);

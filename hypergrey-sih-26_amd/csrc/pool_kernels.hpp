@@ -54,6 +54,7 @@ struct PoolSlab {
     const uint4* col;
     const float* val;
     int blocks, slabs, last, long_runs;  // last: the instance's last chunk
+    int chained;                         // 1: long runs kept within one wave's chunks (SlabMatrix::chained)
 };
 
 constexpr int kPoolSlabRows = 4000;  // rows per slab block in a pool (LDS also holds the block's tile bounds)

@@ -189,6 +189,74 @@ __device__ __forceinline__ void sum_chunk(const SlabChunk& q, const float* ws, f
     if (cur >= 0) acc[cur] = s;
 }
 
+// One chunk of a chained layout (SlabMatrix::chained), summed by all 64 lanes of
+// the wave together; `valid` is false for a lane with no chunk in this round
+// (its registers then count as padding). For layouts whose row sums need not
+// follow stored order (pools). A lane adds its chunk's own runs (entries not
+// marked kSlabCont) to their rows as sum_chunk does. A long run's chunks sit in
+// consecutive lanes of one wave (the layout keeps a run within one group of
+// kChainLanes chunks): the lane of its first chunk adds its entries to the
+// row's sum, every later lane sums the run's entries at the head of its chunk
+// from zero, and the first lane then adds those partial sums in lane order
+// (lane shuffles) and alone writes the row. Every row still gets one write per
+// tile, and the order of the additions is fixed, so results repeat bit for
+// bit; they differ from the stored-order sum only in how a long run's products
+// are grouped.
+__device__ __forceinline__ void sum_chunk_chain(const SlabChunk& q, bool valid, const float* ws, float* acc) {
+#pragma clang fp contract(off)
+    int r[kSlabChunk], c[kSlabChunk];
+    float v[kSlabChunk], g[kSlabChunk], a[kSlabChunk];
+    unpack_chunk(q, r, c, v);
+    bool own[kSlabChunk], cont[kSlabChunk];
+#pragma unroll
+    for (int j = 0; j < kSlabChunk; ++j) {
+        if (!valid) r[j] = int(kSlabPad);
+        own[j] = r[j] != int(kSlabPad) && !(r[j] & kSlabCont);
+        cont[j] = r[j] != int(kSlabPad) && (r[j] & kSlabCont);
+        g[j] = ws[c[j]];                                  // padding: column 0, read and unused
+        a[j] = acc[own[j] ? (r[j] & kSlabRowMask) : 0];   // the sums of the runs that start here
+    }
+    // the head: entries of a run that started in an earlier chunk
+    float head = 0.0f;
+    bool in_head = true;
+#pragma unroll
+    for (int j = 0; j < kSlabChunk; ++j) {
+        in_head = in_head && cont[j];
+        const float prod = v[j] * g[j];
+        if (in_head) head = head + prod;
+    }
+    // the chunk's own runs; a run that goes on into the next chunk is kept in s
+    const bool starts = own[kSlabChunk - 1] && (r[kSlabChunk - 1] & kSlabFollow);
+    const bool passes = cont[0] && in_head && (r[kSlabChunk - 1] & kSlabFollow);  // the whole chunk is one run's middle
+    float s = 0.0f;
+    int cur = -1;
+#pragma unroll
+    for (int j = 0; j < kSlabChunk; ++j) {
+        const int rr = r[j] & kSlabRowMask;
+        if (own[j] && rr != cur) {
+            if (cur >= 0) acc[cur] = s;
+            cur = rr;
+            s = a[j];
+        }
+        const float prod = v[j] * g[j];
+        if (own[j]) s = s + prod;
+    }
+    if (cur >= 0 && !starts) acc[cur] = s;
+    // a started run takes the heads of the following lanes in order
+    const int lane = threadIdx.x % kWave;
+    bool go = starts;
+    for (int d = 1; d < kChainLanes; ++d) {
+        const float h = __shfl(head, lane + d);
+        const int p = __shfl(passes ? 1 : 0, lane + d);
+        if (go) {
+            s = s + h;
+            go = p != 0;
+        }
+        if (!__any(go)) break;
+    }
+    if (starts) acc[cur] = s;
+}
+
 // Tile bounds of one row block: first chunk of tile s at operator()(s), s in
 // [0, slabs]. The box step reads them from its restrict-qualified argument (a
 // scalar load), the pool step from a copy in LDS.
@@ -248,7 +316,7 @@ __device__ __forceinline__ int tile_chunk(const Bounds& tb, int t, int i, int la
 // in a buffer no later tile reads). The tiles after the three-tile loop wait
 // for everything instead (ALL): loads into registers no later tile reads may
 // be dropped by the compiler, which would shift the count.
-template <int W, bool LONG, bool ALL, class Bounds>
+template <int W, bool LONG, bool ALL, bool CHAIN, class Bounds>
 __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* __restrict__ row,
                                           const uint4* __restrict__ col, const float* __restrict__ val, int s,
                                           int last, const float* __restrict__ gat, const float* cur_buf,
@@ -258,6 +326,28 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
     stage_slab<W>(gat + size_t(min(s + 2, P - 1)) * W, ahead_buf);
     __asm__ volatile("" ::: "memory");
     const int k0 = tb(s), k1 = tb(s + 1);
+    if (CHAIN) {
+        // every lane takes part in each round, with or without a chunk of its own;
+        // the next round's chunk is loaded before this one is summed
+        sum_chunk_chain(q, k0 + tid < k1, cur_buf, acc);
+        int base = k0 + kSlabThreads;
+        if (base < k1) {
+            SlabChunk nx;
+            load_chunk(nx, row, col, val, min(base + tid, last));
+            for (;;) {
+                const SlabChunk here = nx;
+                const int hb = base;
+                base += kSlabThreads;
+                if (base < k1) load_chunk(nx, row, col, val, min(base + tid, last));
+                sum_chunk_chain(here, hb + tid < k1, cur_buf, acc);
+                if (base >= k1) break;
+            }
+        }
+        __asm__ volatile("" ::: "memory");
+        load_chunk(q, row, col, val, tile_chunk(tb, min(s + 3, P - 1), tid, last));
+        __asm__ volatile("" ::: "memory");
+        return;
+    }
     if (k0 + tid < k1) sum_chunk<LONG>(q, cur_buf, acc, row, col, val, k0 + tid, last);
     // tiles with more chunks than threads: each further chunk's loads are issued
     // before the previous one is summed
@@ -284,7 +374,7 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
 // matrix's last chunk, tb the block's tile bounds; LONG when some run of the
 // matrix is longer than a chunk. On return acc[r] holds row r's sum for r < R,
 // every copy and load has landed and all threads have passed a barrier.
-template <int W, bool LONG, class Bounds>
+template <int W, bool LONG, class Bounds, bool CHAIN = false>
 __device__ __forceinline__ void slab_rows(int P, int R, int last, const Bounds& tb, const uint4* __restrict__ row,
                                           const uint4* __restrict__ col, const float* __restrict__ val,
                                           const float* __restrict__ gat, float* ws0, float* ws1, float* ws2,
@@ -305,12 +395,12 @@ __device__ __forceinline__ void slab_rows(int P, int R, int last, const Bounds& 
     // three tiles per trip with fixed buffer roles; the one or two left follow
     int s = 0;
     for (; s + 3 <= P; s += 3) {
-        slab_tile<W, LONG, false>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
-        slab_tile<W, LONG, false>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
-        slab_tile<W, LONG, false>(P, tb, row, col, val, s + 2, last, gat, ws2, ws1, acc, q2);
+        slab_tile<W, LONG, false, CHAIN>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
+        slab_tile<W, LONG, false, CHAIN>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
+        slab_tile<W, LONG, false, CHAIN>(P, tb, row, col, val, s + 2, last, gat, ws2, ws1, acc, q2);
     }
-    if (s < P) slab_tile<W, LONG, true>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
-    if (s + 1 < P) slab_tile<W, LONG, true>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
+    if (s < P) slab_tile<W, LONG, true, CHAIN>(P, tb, row, col, val, s, last, gat, ws0, ws2, acc, q0);
+    if (s + 1 < P) slab_tile<W, LONG, true, CHAIN>(P, tb, row, col, val, s + 1, last, gat, ws1, ws0, acc, q1);
     barrier_vm<0>();
 }
 

@@ -35,8 +35,8 @@ int grid_for(int64_t items) {
 __device__ inline int64_t thread_id() { return int64_t(blockIdx.x) * kThreads + threadIdx.x; }
 __device__ inline int64_t thread_count() { return int64_t(gridDim.x) * kThreads; }
 
-// Runs of row i; flags a descending column (bit 0) and a run longer than a
-// chunk (bit 1).
+// Runs of row i; flags a descending column (bit 0), a run longer than a chunk
+// (bit 1) and a run longer than kChainLanes chunks (bit 2).
 __global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* start, const int* col, int width,
                                                        int* runs, int* flags) {
     for (int64_t i = thread_id(); i < n; i += thread_count()) {
@@ -53,6 +53,7 @@ __global__ __launch_bounds__(kThreads) void count_runs(int64_t n, const int* sta
                 prev_slab = s;
             }
             if (++len > kSlabChunk) f |= 2;
+            if (len > kChainLanes * kSlabChunk) f |= 4;
         }
         runs[i] = count;
         if (f) atomicOr(flags, f);
@@ -108,9 +109,10 @@ __global__ __launch_bounds__(kThreads) void tile_first(const unsigned* key, int 
 }
 
 // Chunks of tile t under the packing rule, and each run's slot in its tile
-// (local chunk * kSlabChunk + first entry within the chunk).
+// (local chunk * kSlabChunk + first entry within the chunk). align: a long run
+// whose chunks would cross a group of kChainLanes chunks starts the next group.
 __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const uint64_t* value, int tiles, int* chunks,
-                                                       int* slot) {
+                                                       int* slot, int align) {
     constexpr int kBatch = 16;  // run lengths read per round, so the walk does not wait on each load
     for (int64_t t = thread_id(); t < tiles; t += thread_count()) {
         int count = 0, fill = 0;
@@ -125,8 +127,11 @@ __global__ __launch_bounds__(kThreads) void place_runs(const int* first, const u
                 const int len = lens[u];
                 int at;
                 if (len > kSlabChunk) {  // whole chunks, then a partial last one left open
+                    const int m = (len + kSlabChunk - 1) / kSlabChunk;
+                    if (align && count % kChainLanes + m > kChainLanes)
+                        count = (count / kChainLanes + 1) * kChainLanes;  // padding chunks up to the next group
                     at = count * kSlabChunk;
-                    count += (len + kSlabChunk - 1) / kSlabChunk;
+                    count += m;
                     fill = len - (len - 1) / kSlabChunk * kSlabChunk;
                 } else {
                     if (fill == 0 || fill + len > kSlabChunk) {
@@ -191,7 +196,7 @@ std::vector<int> slab_blocks(int64_t n, const int32_t* start, int target, int ma
 }
 
 bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int groups, const int* start,
-                 const int* col, const double* val, SlabStore& out, hipStream_t stream) {
+                 const int* col, const double* val, SlabStore& out, hipStream_t stream, bool align_chains) {
     if (groups != 1 && groups != 2 && groups != 4) return false;
     const int width = slab_width(groups);
     const int max_rows = slab_rows_max(groups);
@@ -222,6 +227,7 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int gro
     HIPCHECK(hipStreamSynchronize(stream));
     if (host[0] & 1) return false;
     const int long_runs = (host[0] & 2) ? 1 : 0;
+    const int chained = align_chains && !(host[0] & 4) ? 1 : 0;  // runs of more than a group stay unaligned
     const int records = host[1];
 
     DeviceTemp<unsigned> key{size_t(records)}, key2{size_t(records)};
@@ -243,7 +249,7 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int gro
                        tiles, first.p);
     HIPCHECK(hipMemsetAsync(chunks.p + tiles, 0, 4, stream));
     hipLaunchKernelGGL(place_runs, dim3(grid_for(tiles)), dim3(kThreads), 0, stream, first.p, value2.p, tiles,
-                       chunks.p, slot.p);
+                       chunks.p, slot.p, chained);
     tmp_bytes = 0;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, chunks.p, tc.p, tiles + 1, stream));
     {
@@ -284,6 +290,7 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int gro
     out.m.blocks = nb;
     out.m.slabs = int(P);
     out.m.long_runs = long_runs;
+    out.m.chained = chained;
     out.m.groups = groups;
     out.m.width = width;
     out.m.n = int(n);

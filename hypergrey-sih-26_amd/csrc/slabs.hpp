@@ -59,6 +59,10 @@ constexpr unsigned kSlabPad = 0xFFFF;     // row value of a padding entry
 constexpr unsigned kSlabCont = 0x8000;    // row mark: an entry of a run whose first chunk is an earlier one
 constexpr unsigned kSlabFollow = 0x4000;  // row mark: the run goes on into the next chunk
 constexpr unsigned kSlabRowMask = 0x3FFF; // row within the block
+// Chained layouts: every long run's chunks lie within one group of kChainLanes
+// consecutive chunks of its tile (group boundaries at multiples of kChainLanes
+// from the tile's first chunk), so one wave holds the whole run in a round.
+constexpr int kChainLanes = 64;
 
 // No real entry of a block of any shape can take the padding code, whatever
 // marks it carries, and every local row fits the row field.
@@ -73,6 +77,7 @@ struct SlabMatrix {
     int blocks = 0;                 // row blocks; 0: no slabs
     int slabs = 0;                  // column slabs
     int long_runs = 0;              // 1: some run is longer than a chunk (marked entries present)
+    int chained = 0;                // 1: the layout keeps every long run within one group of kChainLanes chunks
     int groups = 1;                 // workgroups per row block: 1, or 2 / 4 for a split layout
     int width = kSlabWidth;         // columns per slab: slab_width(groups)
     int n = 0;                      // rows of the matrix
@@ -126,9 +131,12 @@ struct SlabStore {
 // `groups` workgroups per block (1, 2 or 4; split layouts also get their partial
 // sums), slabs slab_width(groups) wide. Returns false and allocates nothing when
 // a row's columns are not ascending or a split layout has fewer slabs than
-// groups. Synchronizes `stream`.
+// groups. align_chains: a long run that would cross a group of kChainLanes
+// chunks starts the next group instead (padding chunks before it), and the
+// layout is marked chained unless some run is longer than kChainLanes chunks.
+// Synchronizes `stream`.
 bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int groups, const int* start,
-                 const int* col, const double* val, SlabStore& out, hipStream_t stream);
+                 const int* col, const double* val, SlabStore& out, hipStream_t stream, bool align_chains = false);
 
 // One FP32 step over slabs: the step_kernel contract of launch_step<float>
 // (same buffers, momentum sources and restart partials, slab_step_groups(m) of
