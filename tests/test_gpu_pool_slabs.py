@@ -110,3 +110,65 @@ def test_slab_pool_refusals(hg):
         assert p.admit(d, tag=1, tolerance=KKT_TOL) is not None
         (r,) = p.solve_all()
         assert r["optimality_verified"]
+
+
+def long_run_instance(hg, oracle, lengths, seed, n=4000, band=(200, 262, 20)):
+    """One-slab instance (n <= 12,288, so each row is one run) whose rows
+    `lengths` (row -> stored entries) have exactly that many, from disjoint
+    neighbour sets of plain rows, plus a band of consecutive rows of `band[2]`
+    entries each (three-chunk runs one after another, so the packing must move
+    runs to the next group of 64 chunks). Diagonally dominant, bounds [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    long_rows = dict(lengths)
+    for i in range(*band[:2]):
+        long_rows[i] = band[2]
+    plain = [j for j in range(n) if j not in long_rows]
+    rng.shuffle(plain)
+    at = 0
+    for i, length in long_rows.items():
+        nb = plain[at:at + length - 1]  # with the diagonal: `length` entries in row i
+        at += length - 1
+        assert len(nb) == length - 1, "not enough plain rows for disjoint neighbours"
+        for j in nb:
+            rows.append(max(i, j)); cols.append(min(i, j)); vals.append(float(rng.normal(0, 0.3)))
+    rows = np.array(rows, dtype=np.int64); cols = np.array(cols, dtype=np.int64); vals = np.array(vals)
+    radius = np.zeros(n)
+    np.add.at(radius, rows, np.abs(vals)); np.add.at(radius, cols, np.abs(vals))
+    diag = np.arange(n, dtype=np.int64)
+    hr = np.concatenate([rows, diag]); hc = np.concatenate([cols, diag])
+    hv = np.concatenate([vals, radius + rng.uniform(0.1, 1.0, size=n)])
+    q = BoxQP(n, hr, hc, hv, rng.normal(0, 1, size=n), -np.ones(n), np.ones(n))
+    start, index, value = oracle.build_csr(q)
+    lens = np.diff(start)
+    for i, length in long_rows.items():
+        assert lens[i] == length
+    d = dict(n=n, start=start, index=index, value=value, cost=q.cost, lower=q.lower, upper=q.upper)
+    return q, d
+
+
+@pytest.mark.parametrize("case", ["chained", "unchained"])
+def test_slab_pool_long_runs(hg, oracle, case):
+    """Runs of 16, 24, 64, 65, 200 and 512 entries (whole multiples of a chunk
+    among them) and 62 consecutive three-chunk runs, summed by the chained
+    pool step (a run's chunks added across lanes); with a run of 600 entries
+    (more than 64 chunks) the layout stays unchained and the one-thread follow
+    sums it. Either way: verified by the oracle's check, the FP64 oracle's
+    objective, the row-block pool's objective, and alone equals mix."""
+    lengths = {100: 16, 101: 24, 102: 64, 103: 65, 104: 200, 105: 512}
+    if case == "unchained":
+        lengths[106] = 600
+    q, d = long_run_instance(hg, oracle, lengths, seed=31)
+    print(f"case={case} n={d['n']} nnz={len(d['value'])} longest={int(np.diff(d['start']).max())}", flush=True)
+    others = hg.gen_batch(3, seed=41, n_min=800, n_max=6000)
+    mix = solve_batch(hg, [d] + others, slabs=1)
+    alone = solve_batch(hg, [d], slabs=1)[0]
+    blocks = solve_batch(hg, [d], slabs=0)[0]
+    r = mix[0]
+    assert r["status"] == "optimal" and r["optimality_verified"], r
+    chk = oracle.check(q, r["primal"], KKT_TOL)
+    assert chk["optimality_verified"] and chk["stationarity_residual"] <= KKT_TOL
+    ref = oracle.solve_box(q, tolerance=1e-9, iteration_limit=20000, time_limit=120)
+    assert abs(chk["objective"] - ref["objective"]) <= OBJ_REL * (1 + abs(ref["objective"]))
+    assert abs(blocks["objective"] - r["objective"]) <= OBJ_REL * (1 + abs(r["objective"]))
+    assert alone["iterations"] == r["iterations"] and np.array_equal(alone["primal"], r["primal"])
