@@ -346,7 +346,8 @@ __device__ void restart_decision(Control* ctrl, const double* partials, int grou
 }
 
 // Split layout, second launch of a step: each row's sum is its ranges' partials
-// added in range order, then the step epilogue. Thread t of workgroup j takes
+// added in range order, plus the product of its FP32 diagonal entry (kept out of
+// the tiles) and its gathered value, then the step epilogue. Thread t of workgroup j takes
 // rows 4 (kSlabThreads j + t) to 4 (kSlabThreads j + t) + 3 with 16-byte loads
 // and stores (every vector starts 256-byte aligned; the partial stride is a
 // multiple of 64); workgroup 0 also takes the rows past the last whole group
@@ -355,7 +356,8 @@ __device__ void restart_decision(Control* ctrl, const double* partials, int grou
 // G partial loads and its vector loads go out together.
 template <int MOM, int G>
 __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, const float* __restrict__ part,
-                                                                   int64_t stride, StepBuffers<float> bufs,
+                                                                   int64_t stride, const float* __restrict__ diag,
+                                                                   StepBuffers<float> bufs,
                                                                    Control* ctrl, double* partials, int parity,
                                                                    double beta_imm) {
 #pragma clang fp contract(off)
@@ -380,6 +382,11 @@ __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, const 
             g.w = g.w + p.w;
         }
         const float4 wv = reinterpret_cast<const float4*>(epi.gat)[q];
+        const float4 dv = reinterpret_cast<const float4*>(diag)[q];
+        g.x = g.x + dv.x * wv.x;
+        g.y = g.y + dv.y * wv.y;
+        g.z = g.z + dv.z * wv.z;
+        g.w = g.w + dv.w * wv.w;
         const float4 xv = reinterpret_cast<const float4*>(epi.xin)[q];
         const float4 den = epi.dstep ? reinterpret_cast<const float4*>(epi.dstep)[q]
                                      : make_float4(epi.lipschitz, epi.lipschitz, epi.lipschitz, epi.lipschitz);
@@ -399,6 +406,7 @@ __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, const 
             float g = part[r];
 #pragma unroll
             for (int h = 1; h < G; ++h) g = g + part[h * stride + r];
+            g = g + diag[r] * epi.gat[r];
             epi(r, g);
         }
     slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, red);
@@ -724,11 +732,11 @@ template <int MOM>
 void launch_slab_finish(const SlabMatrix& m, dim3 fin, dim3 block, const StepBuffers<float>& b, Control* ctrl,
                         double* partials, int parity, double beta, hipStream_t stream) {
     if (m.groups == 4)
-        hipLaunchKernelGGL((slab_finish_kernel<MOM, 4>), fin, block, 0, stream, int(m.n), m.part, m.part_stride, b,
-                           ctrl, partials, parity, beta);
+        hipLaunchKernelGGL((slab_finish_kernel<MOM, 4>), fin, block, 0, stream, int(m.n), m.part, m.part_stride,
+                           m.diag, b, ctrl, partials, parity, beta);
     else
-        hipLaunchKernelGGL((slab_finish_kernel<MOM, 2>), fin, block, 0, stream, int(m.n), m.part, m.part_stride, b,
-                           ctrl, partials, parity, beta);
+        hipLaunchKernelGGL((slab_finish_kernel<MOM, 2>), fin, block, 0, stream, int(m.n), m.part, m.part_stride,
+                           m.diag, b, ctrl, partials, parity, beta);
 }
 
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,

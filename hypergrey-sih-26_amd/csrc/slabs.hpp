@@ -29,7 +29,12 @@
 // ranges of the slabs, each writing its rows' partial sums; a finishing launch
 // adds the G partials in range order and applies the step epilogue. A row's
 // sum is then a sum of per-range ordered sums, not the ORDERED sum, so the
-// split serves handles whose reduction order is free.
+// split serves handles whose reduction order is free. Split layouts also keep
+// the diagonal out of the tiles: the tile holding a row block's diagonal would
+// carry one more entry per row than the others (about three times the chunks
+// of a tile of the rest on a sparse matrix, so its workgroup would take the
+// longest); the finishing launch adds the FP32 diagonal's product d_i w_i after
+// the G partials instead, from a vector of n floats.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -83,6 +88,7 @@ struct SlabMatrix {
     int n = 0;                      // rows of the matrix
     int64_t part_stride = 0;        // split: elements between two ranges' partial sums
     float* part = nullptr;          // split: [groups][part_stride] partial row sums
+    const float* diag = nullptr;    // split: [n] the diagonal of H rounded to FP32 (0 where absent), not in the tiles
     const int* rb = nullptr;        // [blocks + 1] first row of each block
     const int* tc = nullptr;        // [blocks * slabs + 1] first chunk of tile (b, s) at b * slabs + s
     const uint4* row = nullptr;     // [chunks] 8 x uint16: row within the block and marks, kSlabPad for padding
@@ -129,9 +135,10 @@ struct SlabStore {
 // Builds the slab layout of the device CSR (start, col, val) over the row blocks
 // `blocks` (from slab_blocks, at most slab_rows_max(groups) rows each) with
 // `groups` workgroups per block (1, 2 or 4; split layouts also get their partial
-// sums), slabs slab_width(groups) wide. Returns false and allocates nothing when
-// a row's columns are not ascending or a split layout has fewer slabs than
-// groups. align_chains: a long run that would cross a group of kChainLanes
+// sums and diagonal, whose entries stay out of the tiles), slabs
+// slab_width(groups) wide. Returns false and allocates nothing when a row's
+// columns are not ascending, a split layout's row holds two diagonal entries or
+// a split layout has fewer slabs than groups. align_chains: a long run that would cross a group of kChainLanes
 // chunks starts the next group instead (padding chunks before it), and the
 // layout is marked chained unless some run is longer than kChainLanes chunks.
 // Synchronizes `stream`.

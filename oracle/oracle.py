@@ -330,7 +330,8 @@ def _column_range(start, index, lo, hi):
     return sub, np.nonzero(keep)[0]
 
 
-def fista_steps_np(start, index, value, c, lower, upper, denominator, betas, x, w, dtype=np.float64, ranges=None):
+def fista_steps_np(start, index, value, c, lower, upper, denominator, betas, x, w, dtype=np.float64, ranges=None,
+                   diagonal_last=False):
     """len(betas) projected gradient steps with extrapolation
     (accelerated_qp.cpp:47-57, CPU branch) in `dtype` arithmetic:
 
@@ -348,7 +349,9 @@ def fista_steps_np(start, index, value, c, lower, upper, denominator, betas, x, 
     `ranges` (column boundaries 0 = b_0 < b_1 < ... < b_G = n) restates a
     row sum taken by column ranges: each range's entries are summed in stored
     order from 0, and the G range sums are added in range order,
-    g = ((p_0 + p_1) + p_2) + ... Returns the new (x, w)."""
+    g = ((p_0 + p_1) + p_2) + ...; with `diagonal_last` the diagonal entries
+    are left out of the ranges and added after them, g = (...) + h_ii w_i.
+    Returns the new (x, w)."""
     f = np.dtype(dtype).type
     start = np.asarray(start, dtype=np.int64)
     index = np.asarray(index, dtype=np.int64)
@@ -358,7 +361,14 @@ def fista_steps_np(start, index, value, c, lower, upper, denominator, betas, x, 
     x = np.asarray(x, dtype=np.float64).astype(dtype)
     w = np.asarray(w, dtype=np.float64).astype(dtype)
     parts = None
+    n = len(start) - 1
+    dvals = np.zeros(n, dtype=dtype)
     if ranges is not None:
+        if diagonal_last:
+            row_of = np.repeat(np.arange(n), np.diff(start))
+            on = index == row_of
+            np.add.at(dvals, row_of[on], val[on])  # one entry per row (assembled CSR)
+            index = np.where(on, -1, index)  # in no range
         parts = [_column_range(start, index, ranges[h], ranges[h + 1]) for h in range(len(ranges) - 1)]
     for b in betas:
         if parts is None:
@@ -368,6 +378,8 @@ def fista_steps_np(start, index, value, c, lower, upper, denominator, betas, x, 
             for sub, pos in parts:
                 p = _ordered_row_sums(sub, val[pos], w[index[pos]], dtype)
                 g = p if g is None else g + p
+            if diagonal_last:
+                g = g + dvals * w
         v = w - (g + c) / den
         m = np.where(v < up, v, up)
         nxt = np.where(lo < m, m, lo)

@@ -8,8 +8,9 @@ The device's FP32 steps must equal it bit for bit:
   - the CSR pass (step_kernel<float>, ORDERED) and the one-group slab step,
     with the scalar 1/L and the per-row Jacobi denominators;
   - the split slab step (two or four workgroups per row block), whose row sum
-    is its column ranges' ordered sums added in range order: the oracle takes
-    the same ranges (slab_range_start of the layout).
+    is its column ranges' ordered sums added in range order, the diagonal left
+    out of the ranges and its product added last: the oracle takes the same
+    ranges (slab_range_start of the layout) with diagonal_last.
 
 The steps run through hgqp_box_solve in the FP32-polish precision with the
 reference momentum (no restart), a switch threshold no residual reaches and
@@ -56,10 +57,12 @@ def device_steps(hg, data, steps, metric, **opt):
 
 
 def oracle_steps(oracle, data, steps, metric, ranges=None):
+    """K FP32 steps of the oracle; with `ranges`, the split layouts' row sum
+    (ranges without the diagonal, then the diagonal's product)."""
     s, i, v, c, lo, up, L = data
     x0 = np.clip(np.zeros(len(c)), lo, up)
     x32, _ = fista_steps_np(s, i, v, c, lo, up, denominators(s, v, L, metric), oracle.betas(steps), x0, x0,
-                            dtype=np.float32, ranges=ranges)
+                            dtype=np.float32, ranges=ranges, diagonal_last=ranges is not None)
     return x32
 
 
@@ -124,8 +127,9 @@ def test_fp32_steps_automatic_slabs(hg, oracle):
 @pytest.mark.parametrize("metric", ["scalar", "jacobi"])
 def test_split_fp32_steps_bit_exact(hg, oracle, groups, metric):
     """Planted n = 2e5 (rows reach every range) over a split layout: each row is
-    its ranges' ordered sums added in range order; 12 steps equal the oracle
-    with the same ranges bit for bit, and differ from the one-range sum."""
+    its ranges' ordered sums added in range order, then its diagonal product;
+    12 steps equal the oracle with the same ranges bit for bit, and differ from
+    the one-range sum."""
     q = planted_qp(hg, 200000, 8, 21)[0]
     data = inputs(oracle, q)
     bounds = range_bounds(q.n, groups)
@@ -136,6 +140,24 @@ def test_split_fp32_steps_bit_exact(hg, oracle, groups, metric):
     assert np.array_equal(x, expect), np.abs(x - expect).max()
     plain = widen(oracle_steps(oracle, data, 12, metric), q.lower, q.upper)
     assert not np.array_equal(plain, expect)  # the grouping is visible in FP32
+
+
+@pytest.mark.parametrize("groups", [2, 4])
+def test_split_fp32_steps_long_runs(hg, oracle, groups):
+    """A band of 11 entries per row (runs longer than a chunk once the diagonal
+    is out: 10 entries), no row coupled across a range boundary: 10 split
+    steps equal the oracle over the layout's ranges bit for bit."""
+    from test_gpu_slabs import CUT_LAYOUTS, cut_band_qp
+    width, count, cuts = CUT_LAYOUTS[groups]
+    q = cut_band_qp(45000, 5, cuts, 12)
+    data = inputs(oracle, q)
+    bounds = range_bounds(q.n, groups)
+    assert bounds[1:-1] == list(cuts)
+    print(f"case=cut_band groups={groups} ranges={bounds}", flush=True)
+    expect = widen(oracle_steps(oracle, data, 10, "jacobi", ranges=bounds), q.lower, q.upper)
+    info, x = device_steps(hg, data, 10, "jacobi", reduce=hg.REDUCE_TREE, slabs=2, slab_groups=groups)
+    assert info["slab_groups"] == groups and info["slabs32"] == count
+    assert np.array_equal(x, expect), np.abs(x - expect).max()
 
 
 def tall_block_qp(n, run, seed):

@@ -8,10 +8,11 @@ three-buffer ring, a ragged last slab, rows split over several slabs, empty
 rows, runs longer than a chunk (followed by the thread of their first chunk)
 and tiles with more chunks than a launch has threads.
 
-The split layout (two workgroups per row block, each over half of the slabs)
-sums each row as its first half's ordered sum plus its second half's. On a
-matrix where no row has entries in both halves that equals the one-group sum
-bit for bit; elsewhere the split solve is checked against the contract.
+The split layout (two or four workgroups per row block, each over a range of
+the slabs) sums each row as its ranges' ordered sums in range order, with the
+diagonal out of the ranges and its product added last; its steps are pinned
+against the FP32 oracle bit for bit in test_gpu_fp32_steps.py, and its solves
+here against the contract.
 Every call goes through the C ABI of libhgqp.so.
 """
 import numpy as np
@@ -232,25 +233,29 @@ CUT_LAYOUTS = {2: (SPLIT_WIDTH, 5, (3 * 10240,)), 4: (8192, 6, (2 * 8192, 3 * 81
 
 @pytest.mark.parametrize("groups", [2, 4])
 @pytest.mark.parametrize("mode", ["refine_restart_jacobi", "polish_reference"])
-def test_split_equals_one_group_without_straddling_rows(hg, oracle, mode, groups):
+def test_split_band_meets_contract(hg, oracle, mode, groups):
     """n = 45000 split over 2 groups (5 slabs of 10240 columns, the last 4040
-    wide: ranges 0-2 and 3-4) or 4 (6 slabs of 8192: ranges 0-1, 2, 3-4, 5).
-    With no row coupled across a range boundary every row's sum is one range's
-    ordered sum plus +0s, so under the same (TREE) reduction the split solve
-    equals the one-group solve bit for bit."""
+    wide: ranges 0-2 and 3-4) or 4 (6 slabs of 8192: ranges 0-1, 2, 3-4, 5), a
+    band whose rows hold runs longer than a chunk: the split solve is verified
+    on the device and by the oracle, lands on the one-group solve's objective,
+    and repeats bit for bit."""
     width, count, cuts = CUT_LAYOUTS[groups]
     q = cut_band_qp(45000, 5, cuts, 12)
     data = engine_inputs(oracle, q)
     opt = dict(modes(hg)[mode], reduce=hg.REDUCE_TREE)
     print(f"case=cut_band groups={groups} mode={mode} n={q.n} nnz={len(data[2])}", flush=True)
     split = solve(hg, data, slabs=2, slab_groups=groups, **opt)
+    again = solve(hg, data, slabs=2, slab_groups=groups, **opt)
     one = solve(hg, data, slabs=2, slab_groups=1, **opt)
     assert split[0]["slabs32"] == count and split[0]["slab_groups"] == groups
     assert one[0]["slabs32"] == 4 and one[0]["slab_groups"] == 1
     print(split[1], flush=True)
     assert split[1]["status"] == "optimal" and split[1]["optimality_verified"]
     assert split[1]["fp32_iterations"] > 0
-    assert_same_solve(split, one)
+    chk = oracle.check(q, split[2], KKT_TOL)
+    assert chk["optimality_verified"] and chk["stationarity_residual"] <= KKT_TOL
+    assert abs(split[1]["objective"] - one[1]["objective"]) <= OBJ_REL * (1 + abs(one[1]["objective"]))
+    assert_same_solve(split, again)
 
 
 @pytest.mark.parametrize("n,groups,expect", [(200000, 0, 4), (40960 + 777, 0, 2), (200000, 2, 2)])

@@ -361,6 +361,63 @@ def test_numpy_steps_by_column_range(oracle):
     assert np.array_equal(got[0], expect)
 
 
+def test_numpy_steps_diagonal_last(oracle):
+    """The split layouts' row sum, diagonal entries out of the ranges and added
+    after them: (1) with one range, on the lower triangle (each row's diagonal
+    is its last stored entry) it is the plain ordered sum bit for bit, in FP32
+    and FP64; (2) on the diagonal alone it is one rounded product d_i w_i;
+    (3) over four ranges it equals a loop over the entries in FP32 that skips
+    the diagonal and adds its product last (and differs from the sum with the
+    diagonal in its range)."""
+    from helpers import random_csr_qp
+    from oracle import fista_steps_np
+    q = random_csr_qp(400, 12, long_rows=(3, 399), density=0.04, empty_rows=(7,))
+    s, i, v = oracle.build_csr(q)
+    L = oracle.lipschitz(s, v)
+    rng = np.random.default_rng(4)
+    x0 = np.clip(rng.normal(0, 1, size=q.n), q.lower, q.upper)
+    betas = oracle.betas(4)
+    row_of = np.repeat(np.arange(q.n), np.diff(s))
+    low = i <= row_of  # the lower triangle keeps stored order; its diagonal ends each row
+    ls = np.zeros(q.n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(row_of[low], minlength=q.n), out=ls[1:])
+    li, lv = i[low], v[low]
+    assert np.all(li[ls[1:][np.diff(ls) > 0] - 1] == np.arange(q.n)[np.diff(ls) > 0])  # diagonal last
+    for dt in (np.float64, np.float32):
+        a = fista_steps_np(ls, li, lv, q.cost, q.lower, q.upper, L, betas, x0, x0, dtype=dt)
+        b = fista_steps_np(ls, li, lv, q.cost, q.lower, q.upper, L, betas, x0, x0, dtype=dt, ranges=[0, q.n],
+                           diagonal_last=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    f = np.float32
+    w = x0.astype(f)
+    on = i == row_of
+    dv = np.zeros(q.n)
+    dv[row_of[on]] = v[on]  # 0 on the empty row
+    ds, di = np.arange(q.n + 1), np.arange(q.n)
+    got = fista_steps_np(ds, di, dv, q.cost, q.lower, q.upper, L, betas[:1], x0, x0, dtype=f, ranges=[0, 200, q.n],
+                         diagonal_last=True)
+    expect = np.clip(w - (dv.astype(f) * w + q.cost.astype(f)) / f(L), q.lower.astype(f), q.upper.astype(f))
+    assert np.array_equal(got[0], expect)
+    bounds = [0, 100, 210, 333, q.n]
+    got = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, L, betas[:1], x0, x0, dtype=f, ranges=bounds,
+                         diagonal_last=True)
+    inside = fista_steps_np(s, i, v, q.cost, q.lower, q.upper, L, betas[:1], x0, x0, dtype=f, ranges=bounds)
+    g = np.zeros(q.n, dtype=f)
+    for r in range(q.n):
+        total = f(0)
+        for h in range(4):
+            acc = f(0)
+            for k in range(s[r], s[r + 1]):
+                if bounds[h] <= i[k] < bounds[h + 1] and i[k] != r:
+                    acc = f(acc + f(f(v[k]) * w[i[k]]))
+            total = acc if h == 0 else f(total + acc)
+        diag = [k for k in range(s[r], s[r + 1]) if i[k] == r]
+        g[r] = f(total + f(f(v[diag[0]]) * w[r])) if diag else total
+    expect = np.clip(w - (g + q.cost.astype(f)) / f(L), q.lower.astype(f), q.upper.astype(f))
+    assert np.array_equal(got[0], expect)
+    assert not np.array_equal(got[0], inside[0])
+
+
 def f32_add(a, b):
     return (a.astype(np.float32) + b.astype(np.float32)).astype(np.float32)
 
