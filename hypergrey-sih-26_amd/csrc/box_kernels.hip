@@ -86,17 +86,161 @@ const T* launch_carry(const PanelMatrix<T>& m, const PassSource<T>& src, bool or
     return m.count > 1 ? m.carry : nullptr;
 }
 
+// ---- restart decisions --------------------------------------------------------
+// A step's restart partials are summed by the first kDecideThreads threads of a
+// workgroup: thread t adds partials t, t + kDecideThreads, ... in that order,
+// then each wave's lanes combine by shuffles and the four wave sums are added in
+// wave order. The finalize launch and every workgroup of a step launch that
+// decides for the previous step sum in this one order, so all of them reach the
+// same decision bit for bit.
+constexpr int kDecideThreads = 256;
+constexpr int kDecideWaves = kDecideThreads / kWave;
+constexpr int kDecideScratch = 2 * kDecideWaves + 1;  // doubles of LDS a decision uses
+
+struct RestartOutcome {
+    int restart;  // the next step gathers x: the momentum restarts
+    int end;      // the phase ended with this step or before it: the next step does not run
+    double t;     // t of the next step
+};
+
+// The partial sums of a restart decision, by threads [0, kDecideThreads) of a
+// workgroup: the step's `groups` partial pairs in `bank` (dot products at
+// [0, groups), residual maxima at [groups, 2 groups)) into red[0, 2
+// kDecideWaves), and the phase state (ctrl->done) into red[2 kDecideWaves].
+// `red` is LDS for kDecideScratch doubles; the caller passes a barrier before
+// restart_outcome reads it.
+__device__ void restart_partials(const Control* ctrl, const double* bank, int groups, double* red) {
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    double s = 0, res = 0;
+    for (int g0 = tid; g0 < groups; g0 += 8 * kDecideThreads) {
+        double v[8], r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int g = g0 + u * kDecideThreads;
+            v[u] = g < groups ? bank[g] : 0.0;
+            r[u] = g < groups ? bank[groups + g] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            s += v[u];
+            res = fmax(res, r[u]);
+        }
+    }
+    s = wave_sum(s);
+    res = wave_max(res);
+    if (lane == 0) {
+        red[wave] = s;
+        red[kDecideWaves + wave] = res;
+    }
+    if (tid == 0) red[2 * kDecideWaves] = double(ctrl->done);
+}
+
+// Restart decision of the step that read parity p, from restart_partials' sums
+// in `red`; any thread may call it and every caller gets the same outcome.
+// With `write`, thread 0 stores the decision in ctrl unless the phase had
+// already ended: the next step's t, extrapolation source and iteration, the
+// restart count, the stall state and, when the step's residual ends an FP32
+// phase (switch_stat > 0: at the threshold, on stagnation at the FP32 floor,
+// or when it is not finite), the phase end. The stall state is double
+// buffered by parity, so the workgroups of a step launch that decide together
+// all read the state before this decision while one of them writes the state
+// after it.
+__device__ RestartOutcome restart_outcome(Control* ctrl, const double* red, int p, double switch_stat, bool write) {
+    double tot = 0, rmax = 0;
+#pragma unroll
+    for (int w = 0; w < kDecideWaves; ++w) {
+        tot += red[w];
+        rmax = fmax(rmax, red[kDecideWaves + w]);
+    }
+    const bool running = red[2 * kDecideWaves] == double(kRunning);
+    double next_t, beta;
+    momentum_of(ctrl->t[p], &next_t, &beta);
+    RestartOutcome o;
+    o.restart = tot > 0;
+    o.t = o.restart ? 1.0 : next_t;
+    int stall = ctrl->stall_p[p];
+    double best = ctrl->best_p[p];
+    bool end = false;
+    if (switch_stat > 0) {
+        if (!(rmax <= best * 0.5)) stall += 1;
+        else stall = 0;
+        if (rmax < best) best = rmax;
+        end = !(rmax > switch_stat) || stall >= 20;
+    }
+    o.end = !running || end;
+    if (write && threadIdx.x == 0 && running) {
+        ctrl->t[p ^ 1] = o.t;
+        ctrl->use_x[p ^ 1] = o.restart;
+        ctrl->restarts += o.restart;
+        ctrl->iter[p ^ 1] = ctrl->iter[p] + 1;
+        ctrl->stall_p[p ^ 1] = stall;
+        ctrl->best_p[p ^ 1] = best;
+        if (switch_stat > 0) {
+            ctrl->stall = stall;
+            ctrl->best_stat = best;
+            if (end) {
+                ctrl->stationarity = rmax;
+                ctrl->check_iter = ctrl->iter[p] + 1;
+                ctrl->check_parity = p ^ 1;
+                ctrl->done = kPhaseEnd;
+            }
+        }
+    }
+    return o;
+}
+
+// Both halves with the barrier between them: every thread of the workgroup
+// calls it and gets the outcome.
+__device__ RestartOutcome restart_decide(Control* ctrl, const double* bank, int groups, int p, double switch_stat,
+                                         double* red, bool write) {
+    if (threadIdx.x < kDecideThreads) restart_partials(ctrl, bank, groups, red);
+    __syncthreads();
+    return restart_outcome(ctrl, red, p, switch_stat, write);
+}
+
+// The previous step's restart decision as the prologue of a split sum launch
+// (slab_rows_split): its stager threads sum the partials while the summing
+// threads' first chunk loads are in flight; every thread reads the outcome and
+// workgroup 0 stores it. Inactive: the launch's source point is already known.
+struct SumDecision {
+    bool active;
+    Control* ctrl;
+    const double* bank;
+    int groups;
+    int p;
+    double switch_stat;
+    static constexpr int kThreads = kDecideThreads;
+    __device__ void partials(double* red) const { restart_partials(ctrl, bank, groups, red); }
+    // -1: the phase has ended (no step), 0: gather the extrapolated point, 1: gather x
+    __device__ int outcome(const double* red) const {
+        const RestartOutcome o = restart_outcome(ctrl, red, p, switch_stat, blockIdx.x == 0);
+        return o.end ? -1 : o.restart;
+    }
+};
+
+// A restart-schedule step launch with `prev` set first makes the previous
+// step's restart decision (restart_decide over prev, that step's partial bank,
+// `prev_groups` pairs, under switch_stat) in every workgroup; workgroup 0
+// stores it. The launch's own partials go to `partials`.
 template <class T, bool ORDERED, int MOM>
 __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T> bufs, Control* ctrl,
                                                       double* partials, int parity, double beta_imm,
-                                                      const T* carry) {
+                                                      const T* carry, const double* prev, int prev_groups,
+                                                      double switch_stat) {
     __shared__ T stage[kWavesPerGroup][kCap];
     __shared__ double wave_dot[kWavesPerGroup];
     __shared__ double wave_res[kWavesPerGroup];
+    __shared__ double red[kDecideScratch];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     double beta = beta_imm, next_t = 0;
     int use_x = 0;
-    if (MOM != kMomentumImmediate) {
+    if (MOM == kMomentumRestart && prev) {
+        const RestartOutcome o = restart_decide(ctrl, prev, prev_groups, parity ^ 1, switch_stat, red,
+                                                blockIdx.x == 0);
+        if (o.end) return;  // workgroup-uniform: the outcome comes from LDS after a barrier
+        momentum_of(o.t, &next_t, &beta);
+        use_x = o.restart;
+    } else if (MOM != kMomentumImmediate) {
         if (ctrl->done) return;  // grid-uniform
         momentum_of(ctrl->t[parity], &next_t, &beta);
         use_x = ctrl->use_x[parity];
@@ -249,16 +393,20 @@ __global__ __launch_bounds__(kSumLaunch) void slab_sum_kernel(int slabs, const i
                                                                 const uint4* __restrict__ row,
                                                                 const uint4* __restrict__ col,
                                                                 const float* __restrict__ val, const float* x,
-                                                                const float* w, const Control* ctrl, int parity,
+                                                                const float* w, Control* ctrl, int parity,
                                                                 int follow, float* __restrict__ part,
-                                                                int64_t stride) {
+                                                                int64_t stride, const double* prev,
+                                                                int prev_groups, double switch_stat) {
     constexpr int W = slab_width(G);
     __shared__ __attribute__((aligned(16))) float ws0[W];
     __shared__ __attribute__((aligned(16))) float ws1[W];
     __shared__ __attribute__((aligned(16))) float ws2[W];
-    __shared__ float acc[slab_rows_max(G)];
+    __shared__ __attribute__((aligned(16))) float acc[slab_rows_max(G)];
+    // with a decision to make, the first slabs are staged from w ahead of it
+    // and staged again from x after a restart
+    const SumDecision dec{follow && prev, ctrl, prev, prev_groups, parity ^ 1, switch_stat};
     int use_x = 0;
-    if (follow) {
+    if (follow && !prev) {
         if (ctrl->done) return;  // grid-uniform
         use_x = ctrl->use_x[parity];
     }
@@ -267,83 +415,24 @@ __global__ __launch_bounds__(kSumLaunch) void slab_sum_kernel(int slabs, const i
     const int s0 = slab_range_start(h, slabs, G), P = slab_range_start(h + 1, slabs, G) - s0;
     const int r0 = rb[b], rows = rb[b + 1] - r0;
     const int last = tc[size_t(gridDim.x / G) * slabs] - 1;
+    bool stepped;
     if (P + 1 <= LaneBounds<kBoundRegs>::kCapacity) {  // workgroup-uniform
         LaneBounds<kBoundRegs> tb;
         tb.load(tc, size_t(b) * slabs + s0, P + 1);
-        slab_rows_split<kSumLaunch, kSumStagers, kSumAhead, W, LONG>(P, rows, last, tb, row, col, val,
-                                                                     gat + size_t(s0) * W, ws0, ws1, ws2, acc);
+        stepped = slab_rows_split<kSumLaunch, kSumStagers, kSumAhead, W, LONG>(
+            P, rows, last, tb, row, col, val, gat + size_t(s0) * W, x + size_t(s0) * W, ws0, ws1, ws2, acc, dec);
     } else {
-        slab_rows_split<kSumLaunch, kSumStagers, kSumAhead, W, LONG>(P, rows, last,
-                                                                     GlobalBounds{tc, size_t(b) * slabs + s0}, row,
-                                                                     col, val, gat + size_t(s0) * W, ws0, ws1, ws2,
-                                                                     acc);
+        stepped = slab_rows_split<kSumLaunch, kSumStagers, kSumAhead, W, LONG>(
+            P, rows, last, GlobalBounds{tc, size_t(b) * slabs + s0}, row, col, val, gat + size_t(s0) * W,
+            x + size_t(s0) * W, ws0, ws1, ws2, acc, dec);
     }
+    if (!stepped) return;  // the decision ended the phase
     float* out = part + h * stride + r0;
     for (int r = threadIdx.x; r < rows; r += kSumLaunch) out[r] = acc[r];
 }
 
 constexpr int kFinal = 1024;  // threads of a finalize launch
 
-// Restart decision of a step: the sum of the step's `groups` workgroup
-// partials (dot products at [0, groups), residual maxima at [groups, 2 groups))
-// in a fixed order, then the t / extrapolation state of the next step. Run by
-// one workgroup of NT threads.
-template <int NT>
-__device__ void restart_decision(Control* ctrl, const double* partials, int groups, int parity, double switch_stat) {
-    __shared__ double wave_dot[NT / kWave];
-    __shared__ double wave_res[NT / kWave];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // thread t owns partials t, t + NT, ...; 8 loads in flight per batch
-    double s = 0, res = 0;
-    for (int g0 = threadIdx.x; g0 < groups; g0 += 8 * NT) {
-        double v[8], r[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int g = g0 + u * NT;
-            v[u] = g < groups ? partials[g] : 0.0;
-            r[u] = g < groups ? partials[groups + g] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            s += v[u];
-            res = fmax(res, r[u]);
-        }
-    }
-    s = wave_sum(s);
-    res = wave_max(res);
-    if (lane == 0) {
-        wave_dot[wave] = s;
-        wave_res[wave] = res;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double tot = 0, rmax = 0;
-        for (int w = 0; w < NT / kWave; ++w) {
-            tot += wave_dot[w];
-            rmax = fmax(rmax, wave_res[w]);
-        }
-        double next_t, beta;
-        momentum_of(ctrl->t[parity], &next_t, &beta);
-        const int restart = tot > 0;
-        ctrl->t[parity ^ 1] = restart ? 1.0 : next_t;
-        ctrl->use_x[parity ^ 1] = restart;
-        ctrl->restarts += restart;
-        ctrl->iter[parity ^ 1] = ctrl->iter[parity] + 1;
-        if (switch_stat > 0) {
-            // FP32 phase: the step's residual ends the phase at the switch
-            // threshold, on stagnation (FP32 floor) or when it is not finite
-            if (!(rmax <= ctrl->best_stat * 0.5)) ctrl->stall += 1;
-            else ctrl->stall = 0;
-            if (rmax < ctrl->best_stat) ctrl->best_stat = rmax;
-            if (!(rmax > switch_stat) || ctrl->stall >= 20) {
-                ctrl->stationarity = rmax;
-                ctrl->check_iter = ctrl->iter[parity] + 1;
-                ctrl->check_parity = parity ^ 1;
-                ctrl->done = kPhaseEnd;
-            }
-        }
-    }
-}
 
 // Split layout, second launch of a step: each row's sum is its ranges' partials
 // added in range order, plus the product of its FP32 diagonal entry (kept out of
@@ -412,12 +501,14 @@ __global__ __launch_bounds__(kSlabThreads) void slab_finish_kernel(int n, const 
     slab_step_end<MOM>(epi, ctrl, partials, parity, next_t, red);
 }
 
-// Restart decision of a step launch (restart_decision), as a launch of its own
-// after the step launch on the same stream.
-__global__ __launch_bounds__(kFinal) void finalize_restart(Control* ctrl, const double* partials, int groups,
-                                                           int parity, double switch_stat) {
+// Restart decision of a step launch (restart_decide), as a launch of its own
+// after the step launch on the same stream: kDecideThreads threads, `bank` the
+// step's partial bank.
+__global__ __launch_bounds__(kDecideThreads) void finalize_restart(Control* ctrl, const double* bank, int groups,
+                                                                   int parity, double switch_stat) {
+    __shared__ double red[kDecideScratch];
     if (ctrl->done) return;  // the step returned early
-    restart_decision<kFinal>(ctrl, partials, groups, parity, switch_stat);
+    (void)restart_decide(ctrl, bank, groups, parity, switch_stat, red, true);
 }
 
 template <bool ORDERED>
@@ -699,15 +790,40 @@ int groups_for(int nblocks) {
     return int(g < kMaxBoxGroups ? (g < 1 ? 1 : g) : kMaxBoxGroups);
 }
 
+// Restart partials of a step launch of parity p: bank p of `partials` (null for
+// immediate steps).
+inline double* partial_bank(double* partials, const StepLink& link, int p) {
+    return partials ? partials + (link.bank > 0 ? size_t(p) * size_t(link.bank) : 0) : nullptr;
+}
+
+// The previous step's restart decision for a launch of parity `parity` with
+// link.decide set: its partial bank when this launch makes the decision itself
+// (`fuse`), else null after a finalize_restart launch for it (a launch that
+// cannot decide in its own workgroups, or partials in one bank).
+inline const double* previous_decision(Control* ctrl, double* partials, const StepLink& link, int parity,
+                                       int momentum, int groups, double switch_stat, bool fuse, hipStream_t stream) {
+    if (momentum != kMomentumRestart || !link.decide) return nullptr;
+    const double* prev = partial_bank(partials, link, parity ^ 1);
+    if (fuse && link.bank > 0) return prev;
+    hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kDecideThreads), 0, stream, ctrl, prev, groups, parity ^ 1,
+                       switch_stat);
+    return nullptr;
+}
+
 template <class T>
 void launch_step(const PanelMatrix<T>& pm, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
-                 int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat) {
+                 int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat, StepLink link) {
     const bool dev = momentum != kMomentumImmediate;
-    const T* carry = launch_carry(pm, PassSource<T>{b.w[parity], b.x[parity], ctrl, parity, dev, dev}, ordered,
-                                  stream);
     const Matrix<T>& m = pm.pass[pm.count - 1];
     dim3 grid(groups_for(m.nblocks)), block(kGroup);
-#define HGQP_STEP(ORD, MOM) hipLaunchKernelGGL((step_kernel<T, ORD, MOM>), grid, block, 0, stream, m, b, ctrl, partials, parity, beta, carry)
+    // with column panels the carry launches pick the source point first, so the
+    // decision comes before them
+    const double* prev = previous_decision(ctrl, partials, link, parity, momentum, int(grid.x), switch_stat,
+                                           pm.count == 1, stream);
+    double* own = partial_bank(partials, link, parity);
+    const T* carry = launch_carry(pm, PassSource<T>{b.w[parity], b.x[parity], ctrl, parity, dev, dev}, ordered,
+                                  stream);
+#define HGQP_STEP(ORD, MOM) hipLaunchKernelGGL((step_kernel<T, ORD, MOM>), grid, block, 0, stream, m, b, ctrl, own, parity, beta, carry, prev, int(grid.x), switch_stat)
     if (ordered) {
         if (momentum == kMomentumImmediate) HGQP_STEP(true, kMomentumImmediate);
         else if (momentum == kMomentumReference) HGQP_STEP(true, kMomentumReference);
@@ -718,15 +834,15 @@ void launch_step(const PanelMatrix<T>& pm, const StepBuffers<T>& b, Control* ctr
         else HGQP_STEP(false, kMomentumRestart);
     }
 #undef HGQP_STEP
-    if (momentum == kMomentumRestart)
-        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity,
+    if (momentum == kMomentumRestart && link.finalize)
+        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kDecideThreads), 0, stream, ctrl, own, int(grid.x), parity,
                            switch_stat);
 }
 
 template void launch_step<double>(const PanelMatrix<double>&, const StepBuffers<double>&, Control*, double*, int,
-                                  int, bool, double, hipStream_t, double);
+                                  int, bool, double, hipStream_t, double, StepLink);
 template void launch_step<float>(const PanelMatrix<float>&, const StepBuffers<float>&, Control*, double*, int, int,
-                                 bool, double, hipStream_t, double);
+                                 bool, double, hipStream_t, double, StepLink);
 
 template <int MOM>
 void launch_slab_finish(const SlabMatrix& m, dim3 fin, dim3 block, const StepBuffers<float>& b, Control* ctrl,
@@ -740,14 +856,18 @@ void launch_slab_finish(const SlabMatrix& m, dim3 fin, dim3 block, const StepBuf
 }
 
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
-                      int momentum, double beta, hipStream_t stream, double switch_stat) {
+                      int momentum, double beta, hipStream_t stream, double switch_stat, StepLink link) {
     dim3 block(kSlabThreads);
+    double* own = partial_bank(partials, link, parity);
     if (m.groups > 1) {
         const int follow = momentum != kMomentumImmediate;
         const dim3 sum(m.groups * m.blocks);
+        const dim3 fin(slab_finish_groups(m.n));
+        const double* prev = previous_decision(ctrl, partials, link, parity, momentum, int(fin.x), switch_stat, true,
+                                               stream);
 #define HGQP_SLAB_SUM(G, LONG)                                                                                   \
     hipLaunchKernelGGL((slab_sum_kernel<G, LONG>), sum, dim3(kSumLaunch), 0, stream, m.slabs, m.rb, m.tc, m.row, m.col, m.val, \
-                       b.x[parity], b.w[parity], ctrl, parity, follow, m.part, m.part_stride)
+                       b.x[parity], b.w[parity], ctrl, parity, follow, m.part, m.part_stride, prev, int(fin.x), switch_stat)
         if (m.groups == 4) {
             if (m.long_runs) HGQP_SLAB_SUM(4, true);
             else HGQP_SLAB_SUM(4, false);
@@ -756,22 +876,22 @@ void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control*
             else HGQP_SLAB_SUM(2, false);
         }
 #undef HGQP_SLAB_SUM
-        const dim3 fin(slab_finish_groups(m.n));
         if (momentum == kMomentumImmediate)
-            launch_slab_finish<kMomentumImmediate>(m, fin, block, b, ctrl, partials, parity, beta, stream);
+            launch_slab_finish<kMomentumImmediate>(m, fin, block, b, ctrl, own, parity, beta, stream);
         else if (momentum == kMomentumReference)
-            launch_slab_finish<kMomentumReference>(m, fin, block, b, ctrl, partials, parity, beta, stream);
+            launch_slab_finish<kMomentumReference>(m, fin, block, b, ctrl, own, parity, beta, stream);
         else
-            launch_slab_finish<kMomentumRestart>(m, fin, block, b, ctrl, partials, parity, beta, stream);
-        if (momentum == kMomentumRestart)
-            hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(fin.x), parity,
-                               switch_stat);
+            launch_slab_finish<kMomentumRestart>(m, fin, block, b, ctrl, own, parity, beta, stream);
+        if (momentum == kMomentumRestart && link.finalize)
+            hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kDecideThreads), 0, stream, ctrl, own, int(fin.x),
+                               parity, switch_stat);
         return;
     }
     dim3 grid(m.blocks);
+    (void)previous_decision(ctrl, partials, link, parity, momentum, int(grid.x), switch_stat, false, stream);
 #define HGQP_SLAB_STEP(MOM, LONG)                                                                              \
     hipLaunchKernelGGL((slab_step_kernel<MOM, LONG>), grid, block, 0, stream, m.blocks, m.slabs, m.rb, m.tc, m.row, \
-                       m.col, m.val, b, ctrl, partials, parity, beta)
+                       m.col, m.val, b, ctrl, own, parity, beta)
     if (m.long_runs) {
         if (momentum == kMomentumImmediate) HGQP_SLAB_STEP(kMomentumImmediate, true);
         else if (momentum == kMomentumReference) HGQP_SLAB_STEP(kMomentumReference, true);
@@ -782,8 +902,8 @@ void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control*
         else HGQP_SLAB_STEP(kMomentumRestart, false);
     }
 #undef HGQP_SLAB_STEP
-    if (momentum == kMomentumRestart)
-        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kFinal), 0, stream, ctrl, partials, int(grid.x), parity,
+    if (momentum == kMomentumRestart && link.finalize)
+        hipLaunchKernelGGL(finalize_restart, dim3(1), dim3(kDecideThreads), 0, stream, ctrl, own, int(grid.x), parity,
                            switch_stat);
 }
 

@@ -38,8 +38,7 @@ struct Control {
     int32_t use_x[2];  // 1: the step gathers x instead of the extrapolated point (after a restart)
     int32_t done;
     int32_t restarts;
-    uint32_t reserved0;
-    uint32_t reserved1;
+    int32_t stall_p[2];     // restart schedule: stall count before the step of parity p
     int64_t check_iter;     // iteration of the last evaluated check
     int32_t check_parity;   // buffer parity of the last evaluated check
     int32_t checks;
@@ -49,6 +48,20 @@ struct Control {
     int32_t pad;
     double objective, primal_residual, stationarity, dual_bound, relative_gap;
     double best_stat;       // FP32 phase: best stationarity seen
+    double best_p[2];       // restart schedule: best residual before the step of parity p
+};
+
+// How a restart-schedule step launch hands its restart decision on. The
+// decision (from the step's workgroup partials) is made either by a
+// finalize_restart launch right after the step (`finalize`), or by every
+// workgroup of the next step launch before its own work (`decide` on that
+// launch), which saves a launch per step. A step of parity p writes its
+// partials to bank p, `bank` doubles from bank 0, so a launch that decides
+// for the previous step reads one bank while it writes the other.
+struct StepLink {
+    int decide = 0;    // 1: first make the previous step's decision
+    int finalize = 1;  // 1: a finalize_restart launch follows the step
+    int bank = 0;      // doubles between the two partial banks
 };
 
 template <class T>
@@ -110,7 +123,8 @@ void launch_row_abs(int64_t n, const int* start, const double* val, double fallb
 
 template <class T>
 void launch_step(const PanelMatrix<T>& m, const StepBuffers<T>& b, Control* ctrl, double* partials, int parity,
-                 int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat = 0.0);
+                 int momentum, bool ordered, double beta, hipStream_t stream, double switch_stat = 0.0,
+                 StepLink link = StepLink{});
 
 // Sets *ctrl to `value` in stream order (a one-thread launch).
 void launch_set_control(Control* ctrl, const Control& value, hipStream_t stream);

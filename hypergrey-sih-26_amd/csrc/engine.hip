@@ -228,6 +228,7 @@ Control fresh_control() {
     std::memset(&c, 0, sizeof(c));
     c.t[0] = c.t[1] = 1.0;
     c.best_stat = std::numeric_limits<double>::infinity();
+    c.best_p[0] = c.best_p[1] = c.best_stat;
     c.objective = std::numeric_limits<double>::quiet_NaN();
     c.primal_residual = c.stationarity = c.relative_gap = std::numeric_limits<double>::infinity();
     c.dual_bound = -std::numeric_limits<double>::infinity();
@@ -263,16 +264,31 @@ bool residual_phase(int precision, int momentum) {
     return precision != HGQP_PRECISION_FP64 && momentum == kMomentumRestart;
 }
 
-void launch_one_step(hgqp_box* b, int precision, int momentum, const CheckParams& cp) {
+// Step s of a run of `steps` back-to-back step launches: under the restart
+// schedule each step after the first makes its predecessor's restart decision
+// in its own workgroups, and only the run's last step is followed by a
+// finalize launch, so the control block is whole when the run ends. The two
+// partial banks fit the handle's partial buffer (b->groups * kPartial doubles).
+StepLink step_link(const hgqp_box* b, int s, int steps) {
+    StepLink l;
+    l.decide = s > 0;
+    l.finalize = s + 1 == steps;
+    l.bank = 2 * b->groups;
+    return l;
+}
+
+void launch_one_step(hgqp_box* b, int precision, int momentum, const CheckParams& cp, StepLink link) {
     const bool ord = b->ordered();
+    // FP64 steps never end a phase on their own residual
+    const double sw = residual_phase(precision, momentum) ? cp.switch_stat : 0.0;
     if (precision == HGQP_PRECISION_FP64)
-        launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream);
+        launch_step<double>(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream, 0.0,
+                            link);
     else if (b->slab.m.blocks)
-        launch_slab_step(b->slab.m, b->b32(), b->ctrl, b->partials, b->parity, momentum, 0.0, b->stream,
-                         residual_phase(precision, momentum) ? cp.switch_stat : 0.0);
+        launch_slab_step(b->slab.m, b->b32(), b->ctrl, b->partials, b->parity, momentum, 0.0, b->stream, sw, link);
     else
-        launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream,
-                           residual_phase(precision, momentum) ? cp.switch_stat : 0.0);
+        launch_step<float>(b->m32(), b->b32(), b->ctrl, b->partials, b->parity, momentum, ord, 0.0, b->stream, sw,
+                           link);
     b->parity ^= 1;
 }
 
@@ -286,7 +302,7 @@ void launch_period_check(hgqp_box* b, int precision, int momentum, const CheckPa
 
 // Enqueues one check period: `steps` steps from parity b->parity, then a check.
 void enqueue_period(hgqp_box* b, int precision, int steps, int momentum, const CheckParams& cp) {
-    for (int s = 0; s < steps; ++s) launch_one_step(b, precision, momentum, cp);
+    for (int s = 0; s < steps; ++s) launch_one_step(b, precision, momentum, cp, step_link(b, s, steps));
     launch_period_check(b, precision, momentum, cp);
 }
 
@@ -319,7 +335,7 @@ void enqueue_period_timed(hgqp_box* b, int precision, int steps, int momentum, c
         b->events.push_back(ev);
     }
     HIPCHECK(hipEventRecord(b->events[e], b->stream));
-    for (int s = 0; s < steps; ++s) launch_one_step(b, precision, momentum, cp);
+    for (int s = 0; s < steps; ++s) launch_one_step(b, precision, momentum, cp, step_link(b, s, steps));
     HIPCHECK(hipEventRecord(b->events[e + 1], b->stream));
     timed->push_back({e, first, steps, precision, true});
     launch_period_check(b, precision, momentum, cp);
@@ -967,6 +983,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             cc.iter[b->parity] = int64_t(at);
             cc.best_stat = std::numeric_limits<double>::infinity();
             cc.stall = 0;
+            cc.best_p[b->parity] = cc.best_stat;
+            cc.stall_p[b->parity] = 0;
             set_control(b, cc);
         };
         // One FP32 phase from the current parity; on return x32/w32[b->parity]

@@ -491,25 +491,63 @@ __device__ __forceinline__ void summer_tile(int P, const Bounds& tb, const uint4
     }
 }
 
-// slab_rows with split roles (same contract) for NT threads: waves 0 .. S - 1
-// stage, the others sum. The role is wave-uniform (a scalar branch), and each
-// role runs its own loop with one barrier per tile, so the barriers pair up
-// across the roles and each loop's waits count only its own stream.
-template <int NT, int S, int K, int W, bool LONG, class Bounds>
-__device__ __forceinline__ void slab_rows_split(int P, int R, int last, const Bounds& tb,
+// A workgroup barrier that waits for this wave's LDS operations only: vector
+// loads stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    wait_vm<63>();  // lgkmcnt(0); vmcnt at its maximum, so no wait
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+// What slab_rows_split runs before its tiles when pre.active: a decision that
+// picks the gathered vector. The first two slabs are staged from `gat` ahead of
+// it while the summing threads' first chunk loads are in flight; the stager
+// threads (the workgroup's first Pre::kThreads) then call pre.partials(red),
+// and after a barrier every thread calls pre.outcome(red): -1 ends the pass
+// with no sums, 1 switches to `gat_alt` (the two slabs are staged again once
+// the first copies have landed), 0 keeps `gat`. `red` lies in the row sums,
+// which are cleared only after a second barrier.
+//
+// slab_rows with split roles (same contract; false when the prologue ended the
+// pass) for NT threads: waves 0 .. S - 1 stage, the others sum. The role is
+// wave-uniform (a scalar branch), and each role runs its own loop with one
+// barrier per tile, so the barriers pair up across the roles and each loop's
+// waits count only its own stream.
+template <int NT, int S, int K, int W, bool LONG, class Bounds, class Pre>
+__device__ __forceinline__ bool slab_rows_split(int P, int R, int last, const Bounds& tb,
                                                 const uint4* __restrict__ row, const uint4* __restrict__ col,
                                                 const float* __restrict__ val, const float* __restrict__ gat,
-                                                float* ws0, float* ws1, float* ws2, float* acc) {
+                                                const float* __restrict__ gat_alt, float* ws0, float* ws1,
+                                                float* ws2, float* acc, const Pre& pre) {
     static_assert(K % 3 == 0, "chunk registers rotate with the three slab buffers");
+    static_assert(S * kWave >= Pre::kThreads, "the stager threads sum the prologue's partials");
     constexpr int SUM = NT - S * kWave;
     const int tid = threadIdx.x;
-    for (int r = tid; r < R; r += NT) acc[r] = 0.0f;
+    double* const red = reinterpret_cast<double*>(acc);
     float* const bufs[3] = {ws0, ws1, ws2};
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     if (wave < S) {
         const int lane = tid % kWave;
         stage_slab_wave<W, S>(gat, ws0, wave, lane);
         stage_slab_wave<W, S>(gat + size_t(min(1, P - 1)) * W, ws1, wave, lane);
+        if (pre.active) {  // workgroup-uniform
+            if (tid < Pre::kThreads) pre.partials(red);
+            lds_barrier();
+            const int o = pre.outcome(red);
+            lds_barrier();
+            if (o < 0) {
+                wait_vm<0>();
+                return false;
+            }
+            if (o > 0) {
+                gat = gat_alt;
+                wait_vm<0>();  // the copies from the first source have landed
+                stage_slab_wave<W, S>(gat, ws0, wave, lane);
+                stage_slab_wave<W, S>(gat + size_t(min(1, P - 1)) * W, ws1, wave, lane);
+            }
+        }
+        for (int r = tid; r < R; r += NT) acc[r] = 0.0f;
         int s = 0;
         for (; s + 3 <= P; s += 3) {
 #pragma unroll
@@ -526,6 +564,16 @@ __device__ __forceinline__ void slab_rows_split(int P, int R, int last, const Bo
             load_chunk(q[j], row, col, val, tile_chunk(tb, min(j, P - 1), ctid, last));
             __asm__ volatile("" ::: "memory");
         }
+        if (pre.active) {
+            lds_barrier();
+            const int o = pre.outcome(red);
+            lds_barrier();
+            if (o < 0) {
+                wait_vm<0>();
+                return false;
+            }
+        }
+        for (int r = tid; r < R; r += NT) acc[r] = 0.0f;
         int s = 0;
         for (; s + K <= P; s += K) {
 #pragma unroll
@@ -538,6 +586,7 @@ __device__ __forceinline__ void slab_rows_split(int P, int R, int last, const Bo
                 summer_tile<K, SUM, LONG, true>(P, tb, row, col, val, s + j, last, bufs[j % 3], acc, q[j], ctid);
     }
     barrier_vm<0>();
+    return true;
 }
 
 }  // namespace dev

@@ -152,6 +152,6 @@ bool build_slabs(int64_t n, int64_t nnz, const std::vector<int>& blocks, int gro
 // order. The gathered vectors x[parity] and w[parity] must hold
 // slab_padded(n, m.width) elements.
 void launch_slab_step(const SlabMatrix& m, const StepBuffers<float>& b, Control* ctrl, double* partials, int parity,
-                      int momentum, double beta, hipStream_t stream, double switch_stat);
+                      int momentum, double beta, hipStream_t stream, double switch_stat, StepLink link = StepLink{});
 
 }  // namespace hgqp
