@@ -327,21 +327,25 @@ __device__ __forceinline__ void slab_tile(int P, const Bounds& tb, const uint4* 
     __asm__ volatile("" ::: "memory");
     const int k0 = tb(s), k1 = tb(s + 1);
     if (CHAIN) {
-        // every lane takes part in each round, with or without a chunk of its own;
-        // the next round's chunk is loaded before this one is summed
-        sum_chunk_chain(q, k0 + tid < k1, cur_buf, acc);
-        int base = k0 + kSlabThreads;
-        if (base < k1) {
-            SlabChunk nx;
-            load_chunk(nx, row, col, val, min(base + tid, last));
-            for (;;) {
-                const SlabChunk here = nx;
-                const int hb = base;
-                base += kSlabThreads;
-                if (base < k1) load_chunk(nx, row, col, val, min(base + tid, last));
-                sum_chunk_chain(here, hb + tid < k1, cur_buf, acc);
-                if (base >= k1) break;
-            }
+        // Every lane takes part in each round of kSlabThreads chunks, with or
+        // without a chunk of its own. Round j's registers are loaded two rounds
+        // ahead: rounds 1 and 2 as the tile starts (round 0 is q), round j + 3
+        // into round j's registers once j is summed, so two rounds are in flight
+        // while one is summed. The three register sets rotate over an unrolled
+        // trip of three rounds.
+        const int nr = (k1 - k0 + kSlabThreads - 1) / kSlabThreads;  // rounds of this tile (0 when empty)
+        SlabChunk r1, r2;
+        if (nr > 1) load_chunk(r1, row, col, val, min(k0 + kSlabThreads + tid, last));
+        if (nr > 2) load_chunk(r2, row, col, val, min(k0 + 2 * kSlabThreads + tid, last));
+        for (int j = 0; j < nr; j += 3) {
+            sum_chunk_chain(q, k0 + j * kSlabThreads + tid < k1, cur_buf, acc);
+            if (j + 3 < nr) load_chunk(q, row, col, val, min(k0 + (j + 3) * kSlabThreads + tid, last));
+            if (j + 1 >= nr) break;
+            sum_chunk_chain(r1, k0 + (j + 1) * kSlabThreads + tid < k1, cur_buf, acc);
+            if (j + 4 < nr) load_chunk(r1, row, col, val, min(k0 + (j + 4) * kSlabThreads + tid, last));
+            if (j + 2 >= nr) break;
+            sum_chunk_chain(r2, k0 + (j + 2) * kSlabThreads + tid < k1, cur_buf, acc);
+            if (j + 5 < nr) load_chunk(r2, row, col, val, min(k0 + (j + 5) * kSlabThreads + tid, last));
         }
         __asm__ volatile("" ::: "memory");
         load_chunk(q, row, col, val, tile_chunk(tb, min(s + 3, P - 1), tid, last));
