@@ -97,7 +97,8 @@ struct hgqp_pool {
     int* sblk_local = nullptr;
     PoolSlab* slab_dev = nullptr;    // per slot
     PoolInst* table_host = nullptr;  // pinned mirror of the slot table
-    int32_t* free_phase = nullptr;   // pinned constant kSlotFree (source of slot releases)
+    double* x_stage = nullptr;       // pinned staging of collected iterates (grown on demand)
+    size_t x_stage_bytes = 0;
     int32_t next_gen = 1;
     hipEvent_t table_event = nullptr;
     bool table_pending = false;
@@ -116,7 +117,7 @@ struct hgqp_pool {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (table_event) (void)hipEventDestroy(table_event);
         if (table_host) (void)hipHostFree(table_host);
-        if (free_phase) (void)hipHostFree(free_phase);
+        if (x_stage) (void)hipHostFree(x_stage);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -403,8 +404,6 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     }
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->table_host), sizeof(PoolInst) * S, hipHostMallocDefault));
     std::memset(p->table_host, 0, sizeof(PoolInst) * S);  // all kSlotFree
-    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->free_phase), sizeof(int32_t), hipHostMallocDefault));
-    *p->free_phase = kSlotFree;
     HIPCHECK(hipMemcpyAsync(v.inst, p->table_host, sizeof(PoolInst) * S, hipMemcpyHostToDevice, p->stream));
     int zero[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(v.counts, zero, sizeof zero, hipMemcpyHostToDevice, p->stream));
@@ -634,20 +633,25 @@ hgqp_pool_result result_of(const SlotHost& h, const PoolInst& t, int s) {
     return r;
 }
 
-// Frees a finished slot: its device entry goes FREE in stream order, its ranges
-// and slot return to the host allocators at once (later uploads are ordered
-// after the release on the stream).
-void release_slot(hgqp_pool* p, int s) {
-    SlotHost& h = p->slots[s];
-    HIPCHECK(hipMemcpyAsync(&p->v.inst[s].phase, p->free_phase, sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
-    p->rows.release(h.row0, h.rows_len);
-    p->ents.release(h.ent0, std::max<int64_t>(h.nnz, 1));
-    p->blks.release(h.blk0, h.nblk);
-    if (h.nsblk) p->sblks.release(h.sblk0, h.nsblk);
-    if (h.slab) p->retired.push_back(std::move(h.slab));  // queued launches may still read it
-    h = SlotHost();
-    p->free_slots.push_back(s);
-    --p->resident;
+// Frees finished slots: their device entries go FREE in stream order (one
+// small launch per 62 slots), their ranges and slots return to the host
+// allocators at once (later uploads are ordered after the release on the
+// stream).
+void release_slots(hgqp_pool* p, const std::vector<int>& ids) {
+    if (ids.empty()) return;
+    launch_pool_release(p->v.inst, ids.data(), int(ids.size()), p->stream);
+    HIPCHECK(hipGetLastError());
+    for (int s : ids) {
+        SlotHost& h = p->slots[s];
+        p->rows.release(h.row0, h.rows_len);
+        p->ents.release(h.ent0, std::max<int64_t>(h.nnz, 1));
+        p->blks.release(h.blk0, h.nblk);
+        if (h.nsblk) p->sblks.release(h.sblk0, h.nsblk);
+        if (h.slab) p->retired.push_back(std::move(h.slab));  // queued launches may still read it
+        h = SlotHost();
+        p->free_slots.push_back(s);
+        --p->resident;
+    }
 }
 
 // Slots that a table snapshot reports finished and that still hold the
@@ -659,8 +663,53 @@ std::vector<int> finished_slots(const hgqp_pool* p, const PoolInst* table) {
     return done;
 }
 
+struct HostCopy {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+// Host copies of collected iterates: pieces of at most 4 MiB dealt to up to
+// eight threads once there are more than 8 MiB in all. The copy into a
+// caller's array is also its first touch of those pages, and both the page
+// faults and the pinned-to-pageable bandwidth scale with threads.
+void host_copies(const std::vector<HostCopy>& jobs) {
+    constexpr size_t kPiece = size_t(4) << 20;
+    size_t total = 0;
+    for (const HostCopy& j : jobs) total += j.bytes;
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const int threads = total <= 2 * kPiece ? 1 : int(std::min<size_t>(std::min<size_t>(8, hw), total / kPiece));
+    if (threads <= 1) {
+        for (const HostCopy& j : jobs) std::memcpy(j.dst, j.src, j.bytes);
+        return;
+    }
+    std::vector<HostCopy> pieces;
+    for (const HostCopy& j : jobs)
+        for (size_t o = 0; o < j.bytes; o += kPiece)
+            pieces.push_back({static_cast<char*>(j.dst) + o, static_cast<const char*>(j.src) + o,
+                              std::min(kPiece, j.bytes - o)});
+    auto work = [&](int t) {
+        for (size_t i = size_t(t); i < pieces.size(); i += size_t(threads))
+            std::memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+    };
+    std::vector<std::thread> crew;
+    int started = 1;
+    try {
+        for (int t = 1; t < threads; ++t) {
+            crew.emplace_back(work, t);
+            ++started;
+        }
+    } catch (...) {  // no more threads: this one takes the shares left over
+    }
+    for (int t = started; t < threads; ++t) work(t);
+    work(0);
+    for (std::thread& th : crew) th.join();
+}
+
 // hgqp_pool_collect: reads the slot table after the queued periods, copies x of
-// finished slots to their callers and frees the slots.
+// finished slots to their callers and frees the slots. The iterates go to a
+// pinned staging buffer first, one device-to-host copy per run of adjacent row
+// ranges, then to the callers' arrays on several host threads.
 void harvest(hgqp_pool* p) {
     if (!p->table_pending) {
         HIPCHECK(hipMemcpyAsync(p->table_host, p->v.inst, sizeof(PoolInst) * p->opt.max_instances,
@@ -671,15 +720,38 @@ void harvest(hgqp_pool* p) {
     p->table_pending = false;
     const std::vector<int> done = finished_slots(p, p->table_host);
     if (done.empty()) return;
-    for (int s : done) {
-        const SlotHost& h = p->slots[s];
-        HIPCHECK(hipMemcpyAsync(h.x_out, p->v.x + h.row0, h.n * 8, hipMemcpyDeviceToHost, p->stream));
+    std::vector<int> by_row(done);
+    std::sort(by_row.begin(), by_row.end(), [&](int a, int b) { return p->slots[a].row0 < p->slots[b].row0; });
+    size_t rows = 0;
+    for (int s : by_row) rows += size_t(p->slots[s].rows_len);
+    if (p->x_stage_bytes < rows * 8) {
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (p->x_stage) HIPCHECK(hipHostFree(p->x_stage));
+        p->x_stage = nullptr;
+        p->x_stage_bytes = 0;
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&p->x_stage), rows * 8, hipHostMallocDefault));
+        p->x_stage_bytes = rows * 8;
     }
+    std::vector<HostCopy> jobs;
+    size_t at = 0;  // staging offset (rows) of the current run
+    for (size_t k = 0; k < by_row.size();) {
+        const int64_t r0 = p->slots[by_row[k]].row0;
+        int64_t end = r0;
+        size_t e = k;
+        for (; e < by_row.size() && p->slots[by_row[e]].row0 == end; ++e) {
+            const SlotHost& h = p->slots[by_row[e]];
+            jobs.push_back({h.x_out, p->x_stage + at + size_t(h.row0 - r0), size_t(h.n) * 8});
+            end += h.rows_len;
+        }
+        HIPCHECK(hipMemcpyAsync(p->x_stage + at, p->v.x + r0, size_t(end - r0) * 8, hipMemcpyDeviceToHost,
+                                p->stream));
+        at += size_t(end - r0);
+        k = e;
+    }
+    for (int s : done) p->ready.push_back(result_of(p->slots[s], p->table_host[s], s));
+    release_slots(p, done);
     HIPCHECK(hipStreamSynchronize(p->stream));
-    for (int s : done) {
-        p->ready.push_back(result_of(p->slots[s], p->table_host[s], s));
-        release_slot(p, s);
-    }
+    host_copies(jobs);
 }
 
 }  // namespace
@@ -865,8 +937,8 @@ struct hgqp_server {
             sv.queued_seconds = std::chrono::duration<double>(r->admitted - r->submitted).count();
             sv.latency_seconds = std::chrono::duration<double>(now - r->submitted).count();
             out.push_back(sv);
-            release_slot(pool, s);
         }
+        release_slots(pool, fin);
         std::lock_guard<std::mutex> lk(mu);
         for (const auto& sv : out) done.push_back(sv);
         for (auto& d : dead) retired_dev.push_back(std::move(d));

@@ -369,6 +369,17 @@ __global__ __launch_bounds__(kGroup) void pool_anchor(PoolView v, int parity) {
     }
 }
 
+// Slot ids of one release launch, passed by value.
+constexpr int kReleaseIds = 62;
+struct ReleaseIds {
+    int count;
+    int id[kReleaseIds];
+};
+
+__global__ __launch_bounds__(64) void pool_release(PoolInst* inst, ReleaseIds r) {
+    if (int(threadIdx.x) < r.count) inst[r.id[threadIdx.x]].phase = kSlotFree;
+}
+
 __global__ __launch_bounds__(kGroup) void pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, int r0) {
     const int64_t stride = int64_t(gridDim.x) * kGroup;
     for (int64_t i = int64_t(blockIdx.x) * kGroup + threadIdx.x; i <= n; i += stride) start[i] += e0;
@@ -427,6 +438,15 @@ void launch_pool_checks(const PoolView& v, int loads, int max_blocks, hipStream_
 void launch_pool_lists(const PoolView& v, hipStream_t stream) {
     hipLaunchKernelGGL(pool_list_count, dim3(1), dim3(kListThreads), 0, stream, v, 0);
     hipLaunchKernelGGL(pool_list_expand, dim3(v.max_inst), dim3(kGroup), 0, stream, v, 0);
+}
+
+void launch_pool_release(PoolInst* inst, const int* ids, int count, hipStream_t stream) {
+    for (int k = 0; k < count; k += kReleaseIds) {
+        ReleaseIds r;
+        r.count = std::min(kReleaseIds, count - k);
+        for (int j = 0; j < r.count; ++j) r.id[j] = ids[k + j];
+        hipLaunchKernelGGL(pool_release, dim3(1), dim3(64), 0, stream, inst, r);
+    }
 }
 
 void launch_pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, int r0, hipStream_t stream) {

@@ -116,6 +116,11 @@ void launch_pool_checks(const PoolView& v, int loads, int max_blocks, hipStream_
 // Builds the active list from the slot table (after host admission edits).
 void launch_pool_lists(const PoolView& v, hipStream_t stream);
 
+// Sets the slot-table entries of `count` slots (ids[0..count)) to kSlotFree in
+// stream order. The ids travel in kernel arguments, so the host list may be
+// reused as soon as the call returns.
+void launch_pool_release(PoolInst* inst, const int* ids, int count, hipStream_t stream);
+
 // Moves an uploaded instance from local to pool numbering: start[0..n] += e0
 // (n + 1 row starts) and col[0..nnz) += r0.
 void launch_pool_offset(int* start, int64_t n, int e0, int* col, int64_t nnz, int r0, hipStream_t stream);
