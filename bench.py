@@ -250,7 +250,8 @@ def batch_workload(args):
     def make_pool(entries=0):
         return hgqp.Pool(max_rows=rows, max_nnz=nnz, max_instances=len(batch), block_loads=0, block_rows=0,
                          slabs=args.pool_slabs, slab_block_entries=entries,
-                         check_every=args.check_every, step_metric=STEP[args.pool_step])
+                         check_every=args.check_every, step_metric=STEP[args.pool_step],
+                         fp32_switch=args.pool_switch)
 
     # slab pools: the slab-block size, timed (all instances active, six FP32 steps) for each candidate
     slab_tuning, entries = None, 0
@@ -313,6 +314,7 @@ def batch_workload(args):
                                f"draws/row uniform in 4..32, bounds l~U(-2,-0.5), u~U(0.5,2), KKT<=1e-6, "
                                f"one pool, graph-replayed check periods; one step = the whole batch",
                    "rows": rows, "nnz": nnz, "check_every": args.check_every, "step_metric": args.pool_step,
+                   "first_cycle_target": f"{args.pool_switch} x tol",
                    "slabs": args.pool_slabs,
                    # what was timed: the slab-block size of a slab pool, the row-block shape of the others
                    # (a slab pool's row blocks serve its FP64 checks only and keep a fixed shape)
@@ -384,6 +386,7 @@ def serve_workload(args):
         """ready: instances generated beforehand (the capacity run); else generated ahead of arrival."""
         srv = hgqp.Server(max_rows=args.budget_rows, max_nnz=args.budget_nnz, max_instances=args.slots,
                           check_every=args.check_every, use_graph=0 if args.no_graph else 1,
+                          fp32_switch=args.pool_switch,
                           step_metric=STEP[args.pool_step])
         done, optimum = {}, {}
         ahead = None if ready is not None else Ahead(count)
@@ -472,7 +475,7 @@ def serve_workload(args):
                                f"[1e3, 2e5], 8 draws/row, bounds [-1,1], KKT<=1e-6, continuous batching in one pool",
                    "slots": args.slots, "budget_rows": args.budget_rows, "budget_nnz": args.budget_nnz,
                    "distinct_instances": args.distinct if args.distinct > 0 else "one per request",
-                   "check_every": args.check_every,
+                   "check_every": args.check_every, "first_cycle_target": f"{args.pool_switch} x tol",
                    "step_metric": args.pool_step,
                    "graph_replay": not args.no_graph},
         "p50_ms": round(served["p50_ms"], 3), "p99_ms": round(served["p99_ms"], 3),
@@ -798,6 +801,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the one-at-a-time FP64 loop")
     ap.add_argument("--pool-slabs", type=int, choices=(0, 1), default=1,
                     help="batch: 1 = FP32 steps over per-instance column slabs, 0 = row blocks")
+    ap.add_argument("--pool-switch", type=float, default=None,
+                    help="batch/serve: a pool's first FP32 cycle ends at this multiple of the tolerance "
+                         "(default: 10 for batch, the library's default, best for throughput; 2 for serve, "
+                         "where one cycle more often verifies and latency drops)")
     ap.add_argument("--pool-step", choices=sorted(STEP), default="jacobi",
                     help="batch/serve: step metric of the pool (per-row Jacobi or the scalar 1/L)")
     ap.add_argument("--no-graph", action="store_true",
@@ -805,6 +812,8 @@ def main():
     ap.add_argument("--saturate", type=int, default=2000,
                     help="serve: also replay this many requests queued at once (capacity); 0 = skip")
     args = ap.parse_args()
+    if args.pool_switch is None:
+        args.pool_switch = 2.0 if args.workload == "serve" else 10.0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (instances would shard "
               f"across GPUs with no exchange; see the design notes).", file=sys.stderr)
