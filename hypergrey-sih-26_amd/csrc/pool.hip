@@ -152,7 +152,7 @@ size_t pool_bytes(const hgqp_pool_options& o) {
                    5 * Arena::span(R, 8) +                                                             // c lo up x g
                    8 * Arena::span(R, 4) +                                             // c32 lo32 up32 d[2] w[2] dstep
                    Arena::span(S, sizeof(PoolInst)) + 2 * Arena::span(B, 4) + Arena::span(4, 4) + Arena::span(S, 4) +
-                   Arena::span(B * kPartial, 8) + Arena::span(B, 8);
+                   Arena::span(B * kPartial, 8) + 2 * Arena::span(B, 8);
     if (o.slabs) bytes += 2 * Arena::span(B, 4) + Arena::span(S, sizeof(PoolSlab));
     return bytes;
 }
@@ -386,6 +386,7 @@ hgqp_pool* create_pool(const hgqp_pool_options& o) {
     v.counts = p->arena.take<int>(4);
     v.offsets = p->arena.take<int>(S);
     v.partials = p->arena.take<double>(B * kPartial);
+    v.dots = p->arena.take<double>(B);  // apart from the check partials, whose index space is 10 b
     v.resid = p->arena.take<double>(B);
     v.switch_stat = o.fp32_switch;
     v.restart = o.momentum == HGQP_MOMENTUM_RESTART ? 1 : 0;

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kGroup) void pool_step(PoolView v, int parity) {
                                        epi);
         const double dot = wave_sum(epi.dot), res = wave_max(epi.resid);
         if (lane == 0) {
-            v.partials[b] = dot;
+            v.dots[b] = dot;
             v.resid[b] = res;
         }
     }
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kSlabThreads) void pool_slab_step(PoolView v, int p
                 tot += red[w];
                 mx = fmax(mx, red[kSlabWaves + w]);
             }
-            v.partials[sb] = tot;
+            v.dots[sb] = tot;
             v.resid[sb] = mx;
         }
     }
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kGroup) void pool_finalize_step(PoolView v, int par
     PoolInst* s = v.inst + blockIdx.x;
     if (s->phase != kSlotActive) return;
     const int b0 = v.slabs ? s->sb0 : s->b0, nb = v.slabs ? s->nsblk : s->nblk;
-    const double tot = slot_sum(v.partials, b0, nb, red);
+    const double tot = slot_sum(v.dots, b0, nb, red);
     const double res = slot_max(v.resid, b0, nb, red2);
     if (threadIdx.x != 0) return;
     double next_t, beta;

@@ -89,7 +89,8 @@ struct PoolView {
     int* list_verify;
     int* counts;    // [0] active length, [1] verify length
     int* offsets;   // [max_inst] scratch for list building
-    double* partials;  // per block
+    double* partials;  // per row block: the kPartial doubles of an FP64 check
+    double* dots;      // per block (slab block in a pool with slabs): the step's restart dot product
     double* resid;     // per block: the step's residual maximum
     double switch_stat;
     int restart;       // 1: gradient restart, 0: reference momentum
