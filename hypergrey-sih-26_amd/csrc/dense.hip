@@ -521,6 +521,31 @@ __global__ __launch_bounds__(64) void dense_finalize_check(DenseView v, double t
     s->phase = 0;
 }
 
+// Fresh solve state: x0 = clip(0, l, u) for every RHS (accelerated_qp.cpp:34)
+// and its momentum, restart and cycle state; the padding RHS past `k` are done.
+__global__ __launch_bounds__(256) void dense_init(DenseView v, int k) {
+    const int j = blockIdx.y, r = blockIdx.x * kSeg + threadIdx.x;
+    if (r < v.n) v.x[size_t(j) * v.n + r] = clip(0.0, v.lo[r], v.up[r]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Rhs s;
+        memset(&s, 0, sizeof s);
+        s.t[0] = s.t[1] = 1.0;
+        s.best = INFINITY;
+        s.status = HGQP_STATUS_RUNNING;
+        if (j >= k) {
+            s.phase = 2;
+            s.status = HGQP_STATUS_OPTIMAL;
+        }
+        v.rhs[j] = s;
+    }
+}
+
+// g64 = 0 (the iteration-0 check at x0 = 0 needs no product).
+__global__ void dense_zero(size_t count, double* g) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+        g[i] = 0.0;
+}
+
 // New cycle for every active RHS: anchor at x with gradient g64 + c.
 __global__ __launch_bounds__(256) void dense_anchor(DenseView v, int parity) {
     const int j = blockIdx.y, r = blockIdx.x * kSeg + threadIdx.x;
@@ -552,6 +577,7 @@ struct hgqp_dense {
     bool gemm_wide = false;  // the BF16 GEMM runs with 128 x 128 tiles (gemm_bf16_wide)
     hgqp_dense_options opt{};
     double lipschitz = 0;
+    bool zero_start = false;  // every box holds 0: x0 = 0 for every RHS
     hipStream_t stream = nullptr;
     Arena arena;
     double* q64 = nullptr;
@@ -623,7 +649,13 @@ void read_rhs(hgqp_dense* d) {
 // One FP64 gradient + verdict pass for every RHS that is not done.
 void fp64_check(hgqp_dense* d, double tol, int64_t limit, double first_target, int initial) {
     const dim3 tiles((d->np / kTile) * (d->kp / kTile)), segs(d->v.nseg, d->kp);
-    hipLaunchKernelGGL(gemm_f64, tiles, dim3(256), 0, d->stream, d->np, d->q64, d->v.x, d->v.g64);
+    if (initial && d->zero_start) {  // Q x0 = Q 0 = 0 exactly
+        const size_t count = size_t(d->np) * size_t(d->kp);
+        hipLaunchKernelGGL(dense_zero, dim3(unsigned(std::min<size_t>((count + 255) / 256, 8192))), dim3(256), 0,
+                           d->stream, count, d->v.g64);
+    } else {
+        hipLaunchKernelGGL(gemm_f64, tiles, dim3(256), 0, d->stream, d->np, d->q64, d->v.x, d->v.g64);
+    }
     hipLaunchKernelGGL(dense_check, segs, dim3(256), 0, d->stream, d->v);
     hipLaunchKernelGGL(dense_finalize_check, dim3(d->kp), dim3(64), 0, d->stream, d->v, tol, limit, first_target,
                        initial);
@@ -673,10 +705,12 @@ int hgqp_dense_create(int64_t n, int64_t k, const double* q, const double* c, co
         // padded model: extra variables fixed at 0 (bounds [0, 0], zero rows and
         // columns of Q, zero cost); extra RHS have zero cost and are ignored
         std::vector<double> qp(N * N, 0.0), cp(KP * N, 0.0), lp(N, 0.0), up(N, 0.0);
+        d->zero_start = true;  // padding rows have [0, 0]
         for (int64_t r = 0; r < n; ++r) {
             std::memcpy(&qp[size_t(r) * N], q + r * n, size_t(n) * 8);
             lp[r] = lower[r];
             up[r] = upper[r];
+            d->zero_start = d->zero_start && lower[r] <= 0 && 0 <= upper[r];
         }
         for (int64_t j = 0; j < k; ++j) std::memcpy(&cp[size_t(j) * N], c + j * n, size_t(n) * 8);
         HIPCHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
@@ -777,31 +811,14 @@ int hgqp_dense_solve(hgqp_dense* d, double tolerance, uint64_t iteration_limit, 
         std::memset(report, 0, sizeof(*report));
         const auto t0 = std::chrono::steady_clock::now();
         const DenseView& v = d->v;
-        const size_t N = size_t(d->np), KP = size_t(d->kp);
         const dim3 segs(v.nseg, d->kp);
         const int64_t limit = int64_t(std::min<uint64_t>(iteration_limit, uint64_t(1) << 62));
         // x0 = clip(0, l, u) for every RHS, fresh state
-        {
-            std::vector<double> lo(N), up(N), x0(KP * N);
-            HIPCHECK(hipMemcpyAsync(lo.data(), v.lo, N * 8, hipMemcpyDeviceToHost, d->stream));
-            HIPCHECK(hipMemcpyAsync(up.data(), v.up, N * 8, hipMemcpyDeviceToHost, d->stream));
-            HIPCHECK(hipStreamSynchronize(d->stream));
-            for (size_t j = 0; j < KP; ++j)
-                for (size_t r = 0; r < N; ++r) x0[j * N + r] = std::max(lo[r], std::min(up[r], 0.0));
-            HIPCHECK(hipMemcpyAsync(v.x, x0.data(), KP * N * 8, hipMemcpyHostToDevice, d->stream));
-            std::vector<Rhs> init(KP);
-            std::memset(init.data(), 0, sizeof(Rhs) * KP);
-            for (size_t j = 0; j < KP; ++j) {
-                init[j].t[0] = init[j].t[1] = 1.0;
-                init[j].best = std::numeric_limits<double>::infinity();
-                init[j].status = HGQP_STATUS_RUNNING;
-                if (j >= size_t(d->k)) { init[j].phase = 2; init[j].status = HGQP_STATUS_OPTIMAL; }
-            }
-            HIPCHECK(hipMemcpyAsync(v.rhs, init.data(), sizeof(Rhs) * KP, hipMemcpyHostToDevice, d->stream));
-        }
+        hipLaunchKernelGGL(dense_init, segs, dim3(256), 0, d->stream, v, int(d->k));
+        HIPCHECK(hipGetLastError());
         const double first_target = d->opt.fp32_switch * tolerance;
         fp64_check(d, tolerance, limit, first_target, 1);  // iteration 0
-        report->fp64_gemms = 1;
+        report->fp64_gemms = d->zero_start ? 0 : 1;
         int parity = 0;
         for (int cycle = 0; cycle < 64; ++cycle) {
             read_rhs(d);
