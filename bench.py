@@ -42,6 +42,8 @@ MODES = {
     "fp32_refine_restart": (hgqp.PRECISION_FP32_REFINE, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE, hgqp.STEP_SCALAR),
     "fp32_refine_restart_jacobi": (hgqp.PRECISION_FP32_REFINE, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE,
                                    hgqp.STEP_JACOBI),
+    "fp32_refine_restart_spectral": (hgqp.PRECISION_FP32_REFINE, hgqp.MOMENTUM_RESTART, hgqp.REDUCE_TREE,
+                                     hgqp.STEP_SPECTRAL),
 }
 
 
@@ -109,7 +111,7 @@ def run_mode(name, inst, L, steps, warmup, check_every, polish_every):
         timed.close()
     return dict(name=name, wall=wall, lat=lat, reps=reps, timed=treps, x=x,
                 panels=(info["column_panels"], info["column_panels32"]), slabs=info["slabs32"],
-                slab_groups=info["slab_groups"])
+                slab_groups=info["slab_groups"], jacobi_scale=info["jacobi_scale"])
 
 
 def host_inclusive(name, inst, L, check_every, polish_every, reps=3):
@@ -884,7 +886,7 @@ def main():
             "fp64_step_us": f64_ms / f64_n * 1e3 if f64_n else None,
             "fp32_step_us": f32_ms / f32_n * 1e3 if f32_n else None,
             "column_panels": r["panels"][0], "column_panels32": r["panels"][1], "slabs32": r["slabs"],
-            "slab_groups": r["slab_groups"],
+            "slab_groups": r["slab_groups"], "jacobi_scale": r["jacobi_scale"],
         }
         if ref_x is not None:
             m["max_abs_err_vs_fp64_reference_x"] = float(np.abs(r["x"] - ref_x).max())
@@ -898,7 +900,7 @@ def main():
     # roofline of the head mode's dominant kernel: its step kernel in the precision that ran most
     use32 = hm["fp32_step_us"] is not None and hm["fp32_iterations"] >= hm["iterations"] - hm["fp32_iterations"]
     us = hm["fp32_step_us"] if use32 else hm["fp64_step_us"]
-    nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8, jacobi=MODES[head][3] == hgqp.STEP_JACOBI)
+    nbytes = step_bytes(n, nnz, 4 if use32 else 8, 4 if use32 else 8, jacobi=MODES[head][3] != hgqp.STEP_SCALAR)
     achieved = nbytes / (us * 1e-6) / 1e9
     panels = hm["column_panels32"] if use32 else hm["column_panels"]
     slabs = hm["slabs32"] if use32 else 0

@@ -23,12 +23,15 @@
 // matrix-vector multiplication on GPUs using the CSR storage format", SC 2014),
 // written for wave64 with the solver step fused into the row epilogue.
 #include "box_kernels.hpp"
+#include "device_util.hpp"
 #include "slab_pass.hpp"
 #include "slabs.hpp"
 #include "sparse_pass.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
+#include <vector>
 
 namespace hgqp {
 namespace {
@@ -779,6 +782,72 @@ int elementwise_grid(int64_t n) {
     return int(g < 8192 ? (g < 1 ? 1 : g) : 8192);
 }
 
+// ---- the spectral scale of the Jacobi metric ------------------------------------
+// Power iteration u <- D^-1 H u from a fixed pseudo-random start: D^-1 H has
+// the eigenvalues of D^-1/2 H D^-1/2, whose largest the Rayleigh quotient
+// (u, D u') / (u, D u) of two successive iterates approaches from below.
+
+// u[r]: SplitMix64 of r, mapped to [-1, 1)
+__global__ void power_start(int64_t n, double* u) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        uint64_t z = uint64_t(i) + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        u[i] = double(z >> 11) * 0x1p-52 - 1.0;
+    }
+}
+
+struct PowerEpilogue {
+    const double* __restrict__ d;
+    double* __restrict__ out;
+    __device__ inline void operator()(int r, double hu) { out[r] = hu / d[r]; }
+};
+
+// out = D^-1 H u over the last column panel of m (earlier panels in carry)
+template <bool ORDERED>
+__global__ __launch_bounds__(kGroup) void power_kernel(Matrix<double> m, const double* u, const double* carry,
+                                                       const double* d, double* out) {
+    __shared__ double stage[kWavesPerGroup][kCap];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    PowerEpilogue epi{d, out};
+    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+        row_block<double, ORDERED>(m, u, b, stage[wave], lane, epi, carry);
+}
+
+// Per-workgroup sums of d u v and d u u (pairs at partials[2 g], [2 g + 1]).
+__global__ __launch_bounds__(256) void power_dots(int64_t n, const double* d, const double* u, const double* v,
+                                                  double* partials) {
+    __shared__ double red[2][4];
+    double a = 0, b = 0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double du = d[i] * u[i];
+        a += du * v[i];
+        b += du * u[i];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int wave = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+        red[0][wave] = a;
+        red[1][wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        partials[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+
+// d64 = d64 theta, d32 = FP32 of it
+__global__ void scale_steps(int64_t n, double theta, double* d64, float* d32) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double d = d64[i] * theta;
+        d64[i] = d;
+        if (d32) d32[i] = float(d);
+    }
+}
+
 }  // namespace
 
 // Workgroups of a single-instance sparse launch: one per four row blocks (one
@@ -982,6 +1051,41 @@ void launch_row_abs(int64_t n, const int* start, const double* val, double fallb
                     hipStream_t stream) {
     hipLaunchKernelGGL(row_abs_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, start, val, fallback, d64,
                        d32);
+}
+
+double jacobi_spectral_scale(const PanelMatrix<double>& pm, bool ordered, int64_t n, double* d64, float* d32,
+                             double* u0, double* u1, double* partials, int64_t capacity, hipStream_t stream) {
+    constexpr int kIterations = 40;
+    constexpr double kMargin = 1.05;
+    const int grid_e = elementwise_grid(n);
+    const Matrix<double>& last = pm.pass[pm.count - 1];
+    const dim3 grid(groups_for(last.nblocks)), block(kGroup);
+    hipLaunchKernelGGL(power_start, dim3(grid_e), dim3(256), 0, stream, n, u0);
+    double* u = u0;
+    double* v = u1;
+    for (int k = 0; k < kIterations; ++k) {
+        const double* carry = launch_carry(pm, PassSource<double>{u, u, nullptr, 0, 0, 0}, ordered, stream);
+        if (ordered) hipLaunchKernelGGL((power_kernel<true>), grid, block, 0, stream, last, u, carry, d64, v);
+        else hipLaunchKernelGGL((power_kernel<false>), grid, block, 0, stream, last, u, carry, d64, v);
+        if (k + 1 < kIterations) std::swap(u, v);
+    }
+    // Rayleigh quotient of the last pair: u = u_{K-1}, v = u_K
+    const int groups = int(std::max<int64_t>(1, std::min<int64_t>(capacity / 2, std::min<int64_t>(256, (n + 255) / 256))));
+    hipLaunchKernelGGL(power_dots, dim3(groups), dim3(256), 0, stream, n, d64, u, v, partials);
+    std::vector<double> host(size_t(2 * groups));
+    HIPCHECK(hipMemcpyAsync(host.data(), partials, host.size() * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    double num = 0, den = 0;
+    for (int g = 0; g < groups; ++g) {
+        num += host[size_t(2 * g)];
+        den += host[size_t(2 * g + 1)];
+    }
+    const double lambda = num / den;
+    // Gershgorin already bounds the scale by 1; a failed estimate keeps D
+    const double theta = std::isfinite(lambda) && lambda > 0 ? std::min(1.0, kMargin * lambda) : 1.0;
+    if (theta < 1.0) hipLaunchKernelGGL(scale_steps, dim3(grid_e), dim3(256), 0, stream, n, theta, d64, d32);
+    HIPCHECK(hipGetLastError());
+    return theta;
 }
 
 template void launch_convert<float, double>(int64_t, const double*, float*, hipStream_t);

@@ -156,4 +156,16 @@ void launch_convert(int64_t n, const From* src, To* dst, hipStream_t stream);
 
 int groups_for(int nblocks);
 
+// Spectral scale of the Jacobi metric (HGQP_STEP_SPECTRAL): 40 power
+// iterations of D^-1 H from a fixed pseudo-random start over the FP64 layout
+// pm (u0, u1: n-vectors of scratch), the Rayleigh quotient lambda of the last
+// pair (its sums over at most `capacity` / 2 workgroup partials in `partials`,
+// added on the host in order), and theta = min(1, 1.05 lambda): d64 (and d32)
+// become theta d64 in place. Power iteration approaches lambda_max from below;
+// on the planted family 40 iterations reach 98% of it, so theta D still
+// majorizes H. Synchronizes `stream`; returns theta (1 when the estimate is not
+// finite or not positive).
+double jacobi_spectral_scale(const PanelMatrix<double>& pm, bool ordered, int64_t n, double* d64, float* d32,
+                             double* u0, double* u1, double* partials, int64_t capacity, hipStream_t stream);
+
 }  // namespace hgqp

@@ -4,7 +4,7 @@
 //
 //   hgqp_cli MODEL.qplib [--json] [--backend hip] [--threads N] [--tolerance T]
 //            [--iteration-limit N] [--time-limit S] [--precision fp64|fp32-polish|fp32-refine]
-//            [--momentum reference|restart] [--step scalar|jacobi] [--reduce ordered|tree]
+//            [--momentum reference|restart] [--step scalar|jacobi|spectral] [--reduce ordered|tree]
 //
 // Output keys, number formatting (17 significant digits, null for non-finite)
 // and exit codes (0 optimal, 1 other outcome, 2 rejected input or option)
@@ -58,7 +58,7 @@ int main(int argc, char** argv) {
     if (argc < 2) {
         std::cerr << "Usage: hgqp_cli MODEL.qplib [--json] [--backend hip] [--threads N] [--tolerance T] "
                      "[--iteration-limit N] [--time-limit S] [--precision fp64|fp32-polish|fp32-refine] "
-                     "[--momentum reference|restart] [--step scalar|jacobi] [--reduce ordered|tree]\n";
+                     "[--momentum reference|restart] [--step scalar|jacobi|spectral] [--reduce ordered|tree]\n";
         return 2;
     }
     try {
@@ -92,7 +92,8 @@ int main(int argc, char** argv) {
                 const std::string v = argv[++i];
                 if (v == "scalar") qo.step_metric = HGQP_STEP_SCALAR;
                 else if (v == "jacobi") qo.step_metric = HGQP_STEP_JACOBI;
-                else throw std::runtime_error("Step must be scalar or jacobi");
+                else if (v == "spectral") qo.step_metric = HGQP_STEP_SPECTRAL;
+                else throw std::runtime_error("Step must be scalar, jacobi or spectral");
             } else if (arg == "--reduce" && i + 1 < argc) {
                 const std::string v = argv[++i];
                 if (v == "ordered") qo.reduce = HGQP_REDUCE_ORDERED;

@@ -121,6 +121,7 @@ struct hgqp_box {
     hgqp_box_options opt{};
     int diagonal = 0;
     double lipschitz = 0;
+    double jacobi_scale = 1;  // theta of the spectral step metric
     hipStream_t stream = nullptr;
     Arena arena;
     Arena blocks;  // row blocks of every panel and the launch partials
@@ -581,7 +582,8 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown momentum schedule");
     if (opt.reduce != HGQP_REDUCE_ORDERED && opt.reduce != HGQP_REDUCE_TREE)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown reduction order");
-    if (opt.step_metric != HGQP_STEP_SCALAR && opt.step_metric != HGQP_STEP_JACOBI)
+    if (opt.step_metric != HGQP_STEP_SCALAR && opt.step_metric != HGQP_STEP_JACOBI &&
+        opt.step_metric != HGQP_STEP_SPECTRAL)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: unknown step metric");
     if (opt.column_panels < 0 || opt.column_panels > kMaxPanels)
         throw Error(HGQP_INVALID_ARGUMENT, "hgqp_box_create: column_panels must be in [0, 16]");
@@ -603,7 +605,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     b->at_zero = b->zero_start;
     const bool f32 = opt.precision != HGQP_PRECISION_FP64;
     const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
-    const bool jacobi = opt.step_metric == HGQP_STEP_JACOBI;
+    const bool jacobi = opt.step_metric == HGQP_STEP_JACOBI || opt.step_metric == HGQP_STEP_SPECTRAL;
     const size_t un = static_cast<size_t>(n), unnz = static_cast<size_t>(std::max<int64_t>(nnz, 1));
     HIPCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
 
@@ -833,6 +835,9 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     b->partials = b->blocks.take<double>(size_t(groups) * 10 + 16);
     b->nblocks = 0;
     for (int p = 0; p < p64; ++p) b->nblocks += b->l64.nblocks[p];
+    if (opt.step_metric == HGQP_STEP_SPECTRAL)  // x[1] and w[1] are free until the first solve
+        b->jacobi_scale = jacobi_spectral_scale(b->m64(), b->ordered(), n, b->dstep, b->dstep32, b->x[1], b->w[1],
+                                                b->partials, int64_t(groups) * 10 + 16, b->stream);
     put_control(b.get(), fresh_control());  // synchronizes: the caller's arrays and the temporaries may be released
     return b.release();
 }
@@ -1089,6 +1094,7 @@ int hgqp_box_info_get(const hgqp_box* b, hgqp_box_info* info) {
         info->slabs32 = b->slab.m.blocks ? b->slab.m.slabs : 0;
         info->slab_groups = b->slab.m.blocks ? b->slab.m.groups : 0;
         info->lipschitz = b->lipschitz;
+        info->jacobi_scale = b->jacobi_scale;
     });
 }
 

@@ -266,6 +266,7 @@ bool prepare_model(const ModelView& m, const QpOptions& opt, QpResult& r, Prepar
                                                                 : "hip_fista_box";
     if (opt.momentum == HGQP_MOMENTUM_RESTART) r.algorithm += "_restart";
     if (opt.step_metric == HGQP_STEP_JACOBI) r.algorithm += "_jacobi";
+    if (opt.step_metric == HGQP_STEP_SPECTRAL) r.algorithm += "_spectral";
     const double sign = m.maximize ? -1 : 1;
     const auto term = [&](std::size_t k, int64_t* rr, int64_t* cc, double* vv) { m.term(k, rr, cc, vv); };
     double host_check = 0;

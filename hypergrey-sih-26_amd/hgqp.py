@@ -18,7 +18,7 @@ STATUS = ["optimal", "iteration_limit", "time_limit", "numerical_error", "infeas
 PRECISION_FP64, PRECISION_FP32_POLISH, PRECISION_FP32_REFINE = 0, 1, 2
 MOMENTUM_REFERENCE, MOMENTUM_RESTART = 0, 1
 REDUCE_ORDERED, REDUCE_TREE = 0, 1
-STEP_SCALAR, STEP_JACOBI = 0, 1
+STEP_SCALAR, STEP_JACOBI, STEP_SPECTRAL = 0, 1, 2
 NOALLOC_MAX_ROWS = 2097152  # HGQP_NOALLOC_MAX_ROWS
 
 
@@ -77,7 +77,7 @@ class BoxReport(C.Structure):
 class BoxInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("row_blocks", C.c_int64), ("device_bytes", C.c_int64),
                 ("lipschitz", C.c_double), ("column_panels", C.c_int32), ("column_panels32", C.c_int32),
-                ("slabs32", C.c_int32), ("slab_groups", C.c_int32)]
+                ("slabs32", C.c_int32), ("slab_groups", C.c_int32), ("jacobi_scale", C.c_double)]
 
 
 class Problem(C.Structure):

@@ -72,6 +72,18 @@ extern "C" {
  * counterpart: results are held to the same verdict. */
 #define HGQP_STEP_SCALAR 0
 #define HGQP_STEP_JACOBI 1
+/* SPECTRAL scales the JACOBI denominators by theta = min(1, 1.05 lambda),
+ * lambda the largest eigenvalue of D^-1 H as 40 power iterations from a fixed
+ * pseudo-random start estimate it when the handle is created (a Rayleigh
+ * quotient, so lambda <= the true value; by Gershgorin the true value is < 1).
+ * theta D - H is then no longer guaranteed positive semidefinite, but a
+ * gradient step along an eigenvector contracts while its eigenvalue stays below
+ * 2 theta, and an accelerated one while it stays below 4/3 theta, so the steps
+ * converge unless the estimate is short by more than a quarter. Convex
+ * problems only (H positive semidefinite); a non-finite or non-positive
+ * estimate keeps theta = 1. Single handles only (hgqp_box_*, hgqp_solve). No
+ * reference counterpart: results are held to the same verdict. */
+#define HGQP_STEP_SPECTRAL 2
 
 typedef struct hgqp_box hgqp_box;
 
@@ -215,6 +227,7 @@ typedef struct hgqp_box_info {
     int32_t column_panels32;
     int32_t slabs32;
     int32_t slab_groups;     /* workgroups per slab row block (0: no slabs)    */
+    double jacobi_scale;     /* theta of HGQP_STEP_SPECTRAL (1 otherwise)        */
 } hgqp_box_info;
 int hgqp_box_info_get(const hgqp_box* box, hgqp_box_info* info);
 

@@ -91,7 +91,8 @@ def test_invalid_input_rejected_before_device(hg):
                 {"slab_groups": 2, "reduce": hg.REDUCE_ORDERED}):
         with pytest.raises(hg.InvalidArgument):
             hg.Box(*args, 5.0, **opt)
-    for opt in ({"slabs": 2}, {"slabs": -1}, {"check_every": 3}):
+    # the pool keeps one Jacobi metric for all its instances: no spectral scale
+    for opt in ({"slabs": 2}, {"slabs": -1}, {"check_every": 3}, {"step_metric": hg.STEP_SPECTRAL}):
         with pytest.raises(hg.InvalidArgument):
             hg.Pool(**opt)
     for kw in ({"threads": 0}, {"tolerance": 0.0}, {"time_limit": -1.0}):
