@@ -163,6 +163,7 @@ __device__ RestartOutcome restart_outcome(Control* ctrl, const double* red, int 
     o.t = o.restart ? 1.0 : next_t;
     int stall = ctrl->stall_p[p];
     double best = ctrl->best_p[p];
+    const bool first = !(best < INFINITY);  // a new phase starts with best = inf
     bool end = false;
     if (switch_stat > 0) {
         if (!(rmax <= best * 0.5)) stall += 1;
@@ -181,6 +182,10 @@ __device__ RestartOutcome restart_outcome(Control* ctrl, const double* red, int 
         if (switch_stat > 0) {
             ctrl->stall = stall;
             ctrl->best_stat = best;
+            if (first) {
+                ctrl->first_stat = rmax;
+                ctrl->first_iter = ctrl->iter[p] + 1;
+            }
             if (end) {
                 ctrl->stationarity = rmax;
                 ctrl->check_iter = ctrl->iter[p] + 1;

@@ -49,6 +49,8 @@ struct Control {
     double objective, primal_residual, stationarity, dual_bound, relative_gap;
     double best_stat;       // FP32 phase: best stationarity seen
     double best_p[2];       // restart schedule: best residual before the step of parity p
+    double first_stat;      // FP32 residual phase: the residual of its first step
+    int64_t first_iter;     // and that step's iteration
 };
 
 // How a restart-schedule step launch hands its restart decision on. The

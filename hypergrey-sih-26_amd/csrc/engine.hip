@@ -460,6 +460,13 @@ LoopOutcome run_phase(hgqp_box* b, int precision, int momentum, uint64_t& it, ui
         iter_prev = iter_last;
         best_last = b->host_slots[s].best_stat;
         iter_last = slot_iter[s];
+        // the first read-back of a phase: its first step's residual is the earlier point
+        const Control& h = b->host_slots[s];
+        if (!std::isfinite(best_prev) && h.first_iter > 0 && uint64_t(h.first_iter) < iter_last &&
+            std::isfinite(h.first_stat)) {
+            best_prev = h.first_stat;
+            iter_prev = uint64_t(h.first_iter);
+        }
         if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= time_limit) {
             out.time_up = true;
             return true;
@@ -959,8 +966,6 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         std::vector<TimedPeriod> timed;
         std::vector<TimedPeriod>* timed_ptr = b->opt.timing ? &timed : nullptr;
 
-        // fresh momentum state at the handle's current iterate (accelerated_qp.cpp:37)
-        set_control(b, fresh_control());
         uint64_t it = 0;
         bool time_up = false;
         // Check at iteration 0 (accelerated_qp.cpp:40); at x0 = 0 the gradient is c, no
@@ -970,7 +975,10 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         // exactly what the check would write. That saves the check, its read-back and
         // the host's wait for it.
         const bool skip_first = f32 && b->at_zero && so.iteration_limit > 0;
-        Control c = fresh_control();  // the block set above
+        // fresh momentum state at the handle's current iterate (accelerated_qp.cpp:37); without the
+        // first check the first phase's own block (new_phase, below) is the first the device reads
+        Control c = fresh_control();
+        if (!skip_first) set_control(b, c);
         if (!skip_first) {
             CheckParams first = cp64;
             first.zero_x = b->at_zero ? 1 : 0;
@@ -990,6 +998,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             cc.stall = 0;
             cc.best_p[b->parity] = cc.best_stat;
             cc.stall_p[b->parity] = 0;
+            cc.first_stat = std::numeric_limits<double>::quiet_NaN();
+            cc.first_iter = 0;
             set_control(b, cc);
         };
         // One FP32 phase from the current parity; on return x32/w32[b->parity]
