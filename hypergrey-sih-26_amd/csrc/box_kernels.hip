@@ -721,8 +721,13 @@ __global__ __launch_bounds__(kFinal) void finalize_check32(Control* ctrl, const 
 // Anchors an FP32 correction cycle at the FP64 iterate xa: the correction
 // d = x - xa has gradient g(xa) + H d, so the FP32 step kernel runs unchanged
 // with c := g(xa) and the box shifted by -xa, starting from d = 0.
+__device__ inline void store_control(const ControlWrite& cw) {
+    if (cw.ctrl && blockIdx.x == 0 && threadIdx.x == 0) *cw.ctrl = cw.value;
+}
+
 __global__ void anchor_kernel(int64_t n, const double* g, const double* xa, const double* lo, const double* up,
-                              float* c32, float* lo32, float* up32, float* d, float* w) {
+                              float* c32, float* lo32, float* up32, float* d, float* w, ControlWrite cw) {
+    store_control(cw);
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         c32[j] = float(g[j]);
         lo32[j] = float(lo[j] - xa[j]);
@@ -736,7 +741,8 @@ __global__ void anchor_kernel(int64_t n, const double* g, const double* xa, cons
 // A correction at its FP32 bound (lo32 = fl(lower - xa)) means the variable is
 // at its bound: x takes the exact FP64 bound, not xa + fl(lower - xa).
 __global__ void apply_kernel(int64_t n, const double* xa, const float* d, const float* lo32, const float* up32,
-                             const double* lo, const double* up, double* x, double* w) {
+                             const double* lo, const double* up, double* x, double* w, ControlWrite cw) {
+    store_control(cw);
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const float dj = d[j];
         const double v = dj <= lo32[j] ? lo[j] : dj >= up32[j] ? up[j] : clip(xa[j] + double(dj), lo[j], up[j]);
@@ -747,7 +753,8 @@ __global__ void apply_kernel(int64_t n, const double* xa, const float* d, const 
 
 // FP32 iterate back to FP64 with the same snapping to exact bounds.
 __global__ void widen_kernel(int64_t n, const float* x32, const float* lo32, const float* up32, const double* lo,
-                             const double* up, double* x, double* w) {
+                             const double* up, double* x, double* w, ControlWrite cw) {
+    store_control(cw);
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const float v32 = x32[j];
         const double v = v32 <= lo32[j] ? lo[j] : v32 >= up32[j] ? up[j] : clip(double(v32), lo[j], up[j]);
@@ -757,7 +764,8 @@ __global__ void widen_kernel(int64_t n, const float* x32, const float* lo32, con
 }
 
 template <class T>
-__global__ void init_kernel(int64_t n, const T* lo, const T* up, T* x, T* w) {
+__global__ void init_kernel(int64_t n, const T* lo, const T* up, T* x, T* w, ControlWrite cw) {
+    store_control(cw);
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         T v = clip(T(0), lo[j], up[j]);
         x[j] = v;
@@ -1025,28 +1033,31 @@ void launch_check32(const PanelMatrix<float>& pm, const StepBuffers<float>& b, C
 }
 
 void launch_anchor(int64_t n, const double* g, const double* xa, const double* lo, const double* up, float* c32,
-                   float* lo32, float* up32, float* d, float* w, hipStream_t stream) {
+                   float* lo32, float* up32, float* d, float* w, hipStream_t stream, const ControlWrite& cw) {
     hipLaunchKernelGGL(anchor_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, g, xa, lo, up, c32, lo32, up32,
-                       d, w);
+                       d, w, cw);
 }
 
 void launch_apply(int64_t n, const double* xa, const float* d, const float* lo32, const float* up32, const double* lo,
-                  const double* up, double* x, double* w, hipStream_t stream) {
+                  const double* up, double* x, double* w, hipStream_t stream, const ControlWrite& cw) {
     hipLaunchKernelGGL(apply_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, xa, d, lo32, up32, lo, up, x,
-                       w);
+                       w, cw);
 }
 
 void launch_widen(int64_t n, const float* x32, const float* lo32, const float* up32, const double* lo,
-                  const double* up, double* x, double* w, hipStream_t stream) {
-    hipLaunchKernelGGL(widen_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, x32, lo32, up32, lo, up, x, w);
+                  const double* up, double* x, double* w, hipStream_t stream, const ControlWrite& cw) {
+    hipLaunchKernelGGL(widen_kernel, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, x32, lo32, up32, lo, up, x, w,
+                       cw);
 }
 
 template <class T>
-void launch_init(int64_t n, const T* lo, const T* up, T* x, T* w, hipStream_t stream) {
-    hipLaunchKernelGGL(init_kernel<T>, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, lo, up, x, w);
+void launch_init(int64_t n, const T* lo, const T* up, T* x, T* w, hipStream_t stream, const ControlWrite& cw) {
+    hipLaunchKernelGGL(init_kernel<T>, dim3(elementwise_grid(n)), dim3(256), 0, stream, n, lo, up, x, w, cw);
 }
-template void launch_init<double>(int64_t, const double*, const double*, double*, double*, hipStream_t);
-template void launch_init<float>(int64_t, const float*, const float*, float*, float*, hipStream_t);
+template void launch_init<double>(int64_t, const double*, const double*, double*, double*, hipStream_t,
+                                  const ControlWrite&);
+template void launch_init<float>(int64_t, const float*, const float*, float*, float*, hipStream_t,
+                                 const ControlWrite&);
 
 template <class To, class From>
 void launch_convert(int64_t n, const From* src, To* dst, hipStream_t stream) {

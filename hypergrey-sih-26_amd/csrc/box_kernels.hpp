@@ -138,19 +138,27 @@ void launch_check(const PanelMatrix<double>& m, const StepBuffers<double>& b, Co
 void launch_check32(const PanelMatrix<float>& m, const StepBuffers<float>& b, Control* ctrl, double* partials,
                     int parity, bool ordered, const CheckParams& params, hipStream_t stream);
 
+// A control block that an elementwise launch stores as well (workgroup 0,
+// thread 0), in place of a set_control launch right after it: the launches
+// after it see the same block, one launch sooner. ctrl null: none.
+struct ControlWrite {
+    Control* ctrl = nullptr;
+    Control value{};
+};
+
 // FP32 correction cycle anchored at xa with FP64 gradient g (see anchor_kernel)
 void launch_anchor(int64_t n, const double* g, const double* xa, const double* lo, const double* up, float* c32,
-                   float* lo32, float* up32, float* d, float* w, hipStream_t stream);
+                   float* lo32, float* up32, float* d, float* w, hipStream_t stream, const ControlWrite& cw = {});
 // x = w = clip(xa + d, lo, up), exact bounds where d sits at its FP32 bound
 void launch_apply(int64_t n, const double* xa, const float* d, const float* lo32, const float* up32, const double* lo,
-                  const double* up, double* x, double* w, hipStream_t stream);
+                  const double* up, double* x, double* w, hipStream_t stream, const ControlWrite& cw = {});
 // x = w = FP64 of the FP32 iterate, exact bounds where it sits at an FP32 bound
 void launch_widen(int64_t n, const float* x32, const float* lo32, const float* up32, const double* lo,
-                  const double* up, double* x, double* w, hipStream_t stream);
+                  const double* up, double* x, double* w, hipStream_t stream, const ControlWrite& cw = {});
 
 // x[0] = w[0] = clip(0, lower, upper)
 template <class T>
-void launch_init(int64_t n, const T* lo, const T* up, T* x, T* w, hipStream_t stream);
+void launch_init(int64_t n, const T* lo, const T* up, T* x, T* w, hipStream_t stream, const ControlWrite& cw = {});
 
 // dst = (To)src, elementwise
 template <class To, class From>

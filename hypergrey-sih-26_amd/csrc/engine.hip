@@ -932,9 +932,8 @@ int hgqp_box_reset(hgqp_box* b) {
     return guarded([&] {
         check_box(b);
         b->parity = 0;
-        launch_init<double>(b->n, b->lo, b->up, b->x[0], b->w[0], b->stream);
+        launch_init<double>(b->n, b->lo, b->up, b->x[0], b->w[0], b->stream, ControlWrite{b->ctrl, fresh_control()});
         HIPCHECK(hipGetLastError());
-        set_control(b, fresh_control());
         b->at_zero = b->zero_start;
     });
 }
@@ -976,7 +975,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
         // the host's wait for it.
         const bool skip_first = f32 && b->at_zero && so.iteration_limit > 0;
         // fresh momentum state at the handle's current iterate (accelerated_qp.cpp:37); without the
-        // first check the first phase's own block (new_phase, below) is the first the device reads
+        // first check the first phase's own block (stored with its anchor or by new_phase, below) is the
+        // first the device reads
         Control c = fresh_control();
         if (!skip_first) set_control(b, c);
         if (!skip_first) {
@@ -987,8 +987,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             c = get_control(b);
         }
         b->at_zero = false;
-        // Resets the momentum state and the done flag at iteration `at` for a new phase.
-        auto new_phase = [&](uint64_t at) {
+        // The control block of a new phase at iteration `at`: momentum state and done flag reset.
+        auto phase_block = [&](uint64_t at) {
             Control cc = c;
             cc.done = kRunning;
             cc.t[b->parity] = 1.0;
@@ -1000,8 +1000,9 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             cc.stall_p[b->parity] = 0;
             cc.first_stat = std::numeric_limits<double>::quiet_NaN();
             cc.first_iter = 0;
-            set_control(b, cc);
+            return cc;
         };
+        auto new_phase = [&](uint64_t at) { set_control(b, phase_block(at)); };
         // One FP32 phase from the current parity; on return x32/w32[b->parity]
         // hold the phase's last checked iterate.
         auto fp32_phase = [&](double stat_target) {
@@ -1019,8 +1020,8 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             }
             settle_periods(timed_ptr, from, it);
         };
+        // the FP64 check of the phase's point; its control block was stored by the launch before
         auto fp64_check = [&] {
-            new_phase(it);
             launch_check(b->m64(), b->b64(), b->ctrl, b->partials, b->parity, b->ordered(), cp64, b->stream);
             HIPCHECK(hipGetLastError());
             c = get_control(b);
@@ -1032,12 +1033,11 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
                 const int pa = b->parity;
                 const double* g = cycle == 0 && skip_first ? b->c : b->g;  // g(x0) = H 0 + c = c
                 launch_anchor(b->n, g, b->x[pa], b->lo, b->up, b->c32, b->lo32, b->up32, b->x32[pa], b->w32[pa],
-                              b->stream);
+                              b->stream, ControlWrite{b->ctrl, phase_block(it)});
                 HIPCHECK(hipGetLastError());
-                new_phase(it);
                 fp32_phase(cycle == 0 ? switch_stat : 0.5 * so.tolerance);
                 launch_apply(b->n, b->x[pa], b->x32[b->parity], b->lo32, b->up32, b->lo, b->up, b->x[b->parity],
-                             b->w[b->parity], b->stream);
+                             b->w[b->parity], b->stream, ControlWrite{b->ctrl, phase_block(it)});
                 fp64_check();
             }
             report->fp32_iterations = it;
@@ -1049,7 +1049,7 @@ int hgqp_box_solve(hgqp_box* b, const hgqp_solve_options* options, hgqp_box_repo
             fp32_phase(switch_stat);
             report->fp32_iterations = it;
             launch_widen(b->n, b->x32[b->parity], b->lo32, b->up32, b->lo, b->up, b->x[b->parity], b->w[b->parity],
-                         b->stream);
+                         b->stream, ControlWrite{b->ctrl, phase_block(it)});
             fp64_check();
         }
         if (c.done == kRunning && !time_up && it < so.iteration_limit) {
