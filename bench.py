@@ -659,7 +659,7 @@ def tridiagonal_workload(args):
                                f"value = fastest passing mode at n={n}", "mode": head},
         "roofline": {"bound": "hbm", "achieved": round(nbytes / us / 1e3, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "step_kernel<double> (FP64 reference step, one panel)", "algorithmic_bytes": nbytes,
+                     "kernel": "step_kernel<double> (FP64 reference step, one panel, one row per thread)", "algorithmic_bytes": nbytes,
                      "avg_launch_us": round(us, 2)},
         "cpu_baseline": {"value": round(1.0 / big["cpu"]["all_threads"]["seconds"], 4), "unit": "solves/s",
                          "cores": threads, "kind": kind,

@@ -230,6 +230,70 @@ struct SumDecision {
 // step's restart decision (restart_decide over prev, that step's partial bank,
 // `prev_groups` pairs, under switch_stat) in every workgroup; workgroup 0
 // stores it. The launch's own partials go to `partials`.
+// The rows of a step launch's workgroups (the rows of its kWavesPerGroup row
+// blocks) one per thread, for a matrix whose rows hold at most NW entries: a
+// row's NW loads and gathers are issued together (slots past the row re-read
+// its last entry), and the rounded products are added in stored order. That is
+// row_block's ORDERED sum, and its TREE sum for blocks of more than 32 rows.
+template <int NW, class T>
+__device__ inline T narrow_sum(const T* p, const T* g, int len, T s) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int j = 0; j < NW; ++j)
+        if (j < len) s += p[j] * g[j];
+    return s;
+}
+
+// Each thread takes two rows (kGroup apart) at a time, and loads both rows'
+// entries, gathers and step operands before any of their arithmetic, so that
+// the loads of both rows are in flight together.
+template <int NW, class T>
+__device__ inline void narrow_rows(const Matrix<T>& m, StepEpilogue<T>& epi, const T* carry) {
+    constexpr int R = 2;  // 18.2 us per FP64 step on the tridiagonal n = 1e6 instance; 22.0 with 4 rows
+    const T* __restrict__ gat = epi.gat;
+    for (int b0 = blockIdx.x * kWavesPerGroup; b0 < m.nblocks; b0 += gridDim.x * kWavesPerGroup) {
+        const int r0 = m.blk[b0], r1 = m.blk[min(b0 + kWavesPerGroup, m.nblocks)];
+        for (int base = r0 + int(threadIdx.x); base < r1; base += R * kGroup) {
+            int a[R], e[R];
+            T p[R][NW], g[R][NW], wv[R], xv[R], den[R], cr[R], lr[R], ur[R], s0[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = min(base + i * kGroup, r1 - 1);  // a row past the block re-reads the last row
+                a[i] = m.start[r];
+                e[i] = m.start[r + 1];
+                wv[i] = gat[r];
+                xv[i] = epi.xin[r];
+                den[i] = epi.dstep ? epi.dstep[r] : epi.lipschitz;
+                cr[i] = epi.c[r];
+                lr[i] = epi.lo[r];
+                ur[i] = epi.up[r];
+                s0[i] = carry ? carry[r] : T(0);
+            }
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (e[i] > a[i]) {  // an empty row reads nothing
+#pragma unroll
+                    for (int j = 0; j < NW; ++j) {
+                        const int k = min(a[i] + j, e[i] - 1);
+                        p[i][j] = m.val[k];
+                        g[i][j] = gat[m.col[k]];
+                    }
+                }
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = base + i * kGroup;
+                if (r < r1) {
+                    T next, wnext;
+                    epi.compute(narrow_sum<NW>(p[i], g[i], e[i] - a[i], s0[i]), wv[i], xv[i], den[i], cr[i], lr[i],
+                                ur[i], next, wnext);
+                    epi.wout[r] = wnext;
+                    epi.xout[r] = next;
+                }
+            }
+        }
+    }
+}
+
 template <class T, bool ORDERED, int MOM>
 __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T> bufs, Control* ctrl,
                                                       double* partials, int parity, double beta_imm,
@@ -266,8 +330,11 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     epi.beta = T(beta);
     epi.dot = 0;
     epi.resid = 0;
-    for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
-        row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi, carry);
+    if (m.narrow == 4) narrow_rows<4>(m, epi, carry);  // grid-uniform
+    else if (m.narrow) narrow_rows<kNarrowRow>(m, epi, carry);
+    else
+        for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)
+            row_block<T, ORDERED>(m, epi.gat, b, stage[wave], lane, epi, carry);
     if (MOM == kMomentumReference) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctrl->t[parity ^ 1] = next_t;

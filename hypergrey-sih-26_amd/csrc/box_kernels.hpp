@@ -73,7 +73,12 @@ struct Matrix {
     const int* start;    // [n + 1]
     const int* col;      // [nnz]
     const T* val;        // [nnz]
+    int narrow = 0;      // 4 or 8: no row has more stored entries (step launches then take one row per
+                         // thread); 0: some row has more
 };
+
+// Longest row the narrow form of a step takes.
+constexpr int kNarrowRow = 8;
 
 // A single instance's matrix as column panels: pass p holds, for every row, the
 // row's entries whose columns lie in panel p (a CSR of its own, columns in

@@ -122,6 +122,7 @@ struct hgqp_box {
     int diagonal = 0;
     double lipschitz = 0;
     double jacobi_scale = 1;  // theta of the spectral step metric
+    int narrow = 0;           // Matrix::narrow of the one-panel layouts
     hipStream_t stream = nullptr;
     Arena arena;
     Arena blocks;  // row blocks of every panel and the launch partials
@@ -173,7 +174,8 @@ struct hgqp_box {
         m.count = l.count;
         m.carry = carry;
         for (int p = 0; p < l.count; ++p)
-            m.pass[p] = Matrix<T>{l.nblocks[p], l.blk[p], l.start + size_t(p) * size_t(n + 1), l.col, v};
+            m.pass[p] = Matrix<T>{l.nblocks[p], l.blk[p], l.start + size_t(p) * size_t(n + 1), l.col, v,
+                                  l.count == 1 ? narrow : 0};
         return m;
     }
     PanelMatrix<double> m64() const { return panels<double>(l64, val, carry64); }
@@ -609,6 +611,9 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     b->diagonal = opt.diagonal_hessian < 0 ? (diag ? 1 : 0) : (opt.diagonal_hessian ? 1 : 0);
     b->zero_start = true;
     for (int64_t j = 0; j < n && b->zero_start; ++j) b->zero_start = lower[j] <= 0 && 0 <= upper[j];
+    int longest = 0;  // stored entries of the longest row
+    for (int64_t j = 0; j < n && longest <= kNarrowRow; ++j) longest = std::max(longest, start[j + 1] - start[j]);
+    b->narrow = longest <= 4 ? 4 : longest <= kNarrowRow ? kNarrowRow : 0;
     b->at_zero = b->zero_start;
     const bool f32 = opt.precision != HGQP_PRECISION_FP64;
     const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
