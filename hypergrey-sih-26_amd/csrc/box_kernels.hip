@@ -244,12 +244,13 @@ __device__ inline T narrow_sum(const T* p, const T* g, int len, T s) {
     return s;
 }
 
-// Each thread takes two rows (kGroup apart) at a time, and loads both rows'
+// Each thread takes three rows (kGroup apart) at a time, and loads their
 // entries, gathers and step operands before any of their arithmetic, so that
-// the loads of both rows are in flight together.
+// the loads of all three rows are in flight together (a workgroup's 4 row
+// blocks of a tridiagonal matrix, about 680 rows, in one round).
 template <int NW, class T>
 __device__ inline void narrow_rows(const Matrix<T>& m, StepEpilogue<T>& epi, const T* carry) {
-    constexpr int R = 2;  // 18.2 us per FP64 step on the tridiagonal n = 1e6 instance; 22.0 with 4 rows
+    constexpr int R = 3;  // FP64 step, tridiagonal n = 1e6 (NW = 3): 17.6 us; NW = 4: 17.9, 2 rows 18.2, 4 rows 22.0
     const T* __restrict__ gat = epi.gat;
     for (int b0 = blockIdx.x * kWavesPerGroup; b0 < m.nblocks; b0 += gridDim.x * kWavesPerGroup) {
         const int r0 = m.blk[b0], r1 = m.blk[min(b0 + kWavesPerGroup, m.nblocks)];
@@ -330,7 +331,8 @@ __global__ __launch_bounds__(kGroup) void step_kernel(Matrix<T> m, StepBuffers<T
     epi.beta = T(beta);
     epi.dot = 0;
     epi.resid = 0;
-    if (m.narrow == 4) narrow_rows<4>(m, epi, carry);  // grid-uniform
+    if (m.narrow == 3) narrow_rows<3>(m, epi, carry);  // grid-uniform
+    else if (m.narrow == 4) narrow_rows<4>(m, epi, carry);
     else if (m.narrow) narrow_rows<kNarrowRow>(m, epi, carry);
     else
         for (int b = blockIdx.x * kWavesPerGroup + wave; b < m.nblocks; b += gridDim.x * kWavesPerGroup)

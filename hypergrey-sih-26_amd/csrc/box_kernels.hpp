@@ -73,8 +73,8 @@ struct Matrix {
     const int* start;    // [n + 1]
     const int* col;      // [nnz]
     const T* val;        // [nnz]
-    int narrow = 0;      // 4 or 8: no row has more stored entries (step launches then take one row per
-                         // thread); 0: some row has more
+    int narrow = 0;      // 3, 4 or 8: no row has more stored entries (step launches then take one row
+                         // per thread); 0: some row has more
 };
 
 // Longest row the narrow form of a step takes.

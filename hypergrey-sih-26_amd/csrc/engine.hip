@@ -613,7 +613,7 @@ hgqp_box* create_box(int64_t n, int64_t nnz, const CsrSource& src, const double*
     for (int64_t j = 0; j < n && b->zero_start; ++j) b->zero_start = lower[j] <= 0 && 0 <= upper[j];
     int longest = 0;  // stored entries of the longest row
     for (int64_t j = 0; j < n && longest <= kNarrowRow; ++j) longest = std::max(longest, start[j + 1] - start[j]);
-    b->narrow = longest <= 4 ? 4 : longest <= kNarrowRow ? kNarrowRow : 0;
+    b->narrow = longest <= 3 ? 3 : longest <= 4 ? 4 : longest <= kNarrowRow ? kNarrowRow : 0;
     b->at_zero = b->zero_start;
     const bool f32 = opt.precision != HGQP_PRECISION_FP64;
     const bool refine = opt.precision == HGQP_PRECISION_FP32_REFINE;
