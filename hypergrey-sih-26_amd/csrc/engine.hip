@@ -1110,6 +1110,8 @@ int hgqp_box_info_get(const hgqp_box* b, hgqp_box_info* info) {
         info->slab_groups = b->slab.m.blocks ? b->slab.m.groups : 0;
         info->lipschitz = b->lipschitz;
         info->jacobi_scale = b->jacobi_scale;
+        info->narrow_rows = b->l64.count == 1 ? b->narrow : 0;
+        info->reserved = 0;
     });
 }
 

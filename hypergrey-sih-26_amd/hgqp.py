@@ -77,7 +77,8 @@ class BoxReport(C.Structure):
 class BoxInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("row_blocks", C.c_int64), ("device_bytes", C.c_int64),
                 ("lipschitz", C.c_double), ("column_panels", C.c_int32), ("column_panels32", C.c_int32),
-                ("slabs32", C.c_int32), ("slab_groups", C.c_int32), ("jacobi_scale", C.c_double)]
+                ("slabs32", C.c_int32), ("slab_groups", C.c_int32), ("jacobi_scale", C.c_double),
+                ("narrow_rows", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Problem(C.Structure):

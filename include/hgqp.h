@@ -228,6 +228,9 @@ typedef struct hgqp_box_info {
     int32_t slabs32;
     int32_t slab_groups;     /* workgroups per slab row block (0: no slabs)    */
     double jacobi_scale;     /* theta of HGQP_STEP_SPECTRAL (1 otherwise)        */
+    int32_t narrow_rows;     /* 3, 4 or 8: one-panel steps take one row per thread,
+                                no row holding more entries; 0: row blocks        */
+    int32_t reserved;
 } hgqp_box_info;
 int hgqp_box_info_get(const hgqp_box* box, hgqp_box_info* info);
 
